@@ -1,0 +1,151 @@
+/*
+ * twoldg.h -- C ABI of the MI355X (gfx950) OLDG field-evolution library.
+ *
+ * This is the drop-in boundary for TreeWaveOLDG.jl's field-evolution hot path.
+ * A Julia shim (ccall), the Python host package `twoldg` (ctypes) or any C
+ * caller binds these symbols; no Torch or HIP type crosses the boundary.
+ *
+ * Conventions
+ *   - Every entry point returns an int status: 0 on success, a negative
+ *     TWOLDG_E* code on failure.  twoldg_last_error() then holds the reason
+ *     (thread-local), the text the reference raises as ArgumentError.
+ *   - State vectors are in the reference's statearray layout: for block b,
+ *     variable v (0 = u, 1 = v) and owned node I (x fastest), entry
+ *     (b * 2 + v) * N^D + lin(I); src/stepping.jl:36-41.
+ *   - The library owns every device buffer of a problem.  One HIP stream per
+ *     problem; calls on one problem are serialized, distinct problems may be
+ *     used from distinct threads.
+ *   - Forest description: blocks in the caller's order, each with its spacing
+ *     h, and one face descriptor per (block, dimension, side) saying where its
+ *     ghost plane comes from (TreeAMR's ghost schedule, evolution.jl:1093-1097).
+ */
+#ifndef TWOLDG_H
+#define TWOLDG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TWOLDG_ABI_VERSION 1
+
+/* status codes */
+#define TWOLDG_OK 0
+#define TWOLDG_EINVAL (-1)   /* an argument the reference refuses (ArgumentError) */
+#define TWOLDG_EDEVICE (-2)  /* a HIP runtime failure */
+#define TWOLDG_ENOMEM (-3)
+#define TWOLDG_ESTATE (-4)   /* call out of order (e.g. RHS before upload) */
+
+/* arithmetic forms */
+#define TWOLDG_F64 0   /* FP64 throughout, the reference's structure */
+#define TWOLDG_F32C 1  /* FP32 stencil and storage, FP32 hi+lo (Kahan) RK step, FP64 reductions */
+
+/* face kinds: where the ghost plane of a block face comes from */
+#define TWOLDG_FACE_SAME 1      /* same-level neighbour (periodic wrap included); nb[0] */
+#define TWOLDG_FACE_COARSER 2   /* coarser neighbour nb[0]; off[] = our half inside it */
+#define TWOLDG_FACE_FINER 3     /* 2^(D-1) finer neighbours nb[] by tangential half bits */
+#define TWOLDG_FACE_MIRROR 4    /* reflecting wall: parity-signed own face plane */
+#define TWOLDG_FACE_GLUE 5      /* glued patch face: nb[0], its face nface, tangential map */
+#define TWOLDG_FACE_DIRICHLET 6 /* outer face with Dirichlet data */
+
+/* slots of a problem's device state buffers */
+#define TWOLDG_SLOT_STATE 0  /* the integrator's state y_n */
+#define TWOLDG_SLOT_DU 1     /* right-hand side output */
+#define TWOLDG_SLOT_AUX 2    /* caller data, e.g. an exact solution for error norms */
+
+typedef struct twoldg_face {
+  int32_t kind;
+  int32_t nface;   /* GLUE: the neighbour's face, 2*d' + s' (s' = 0 low, 1 high) */
+  int32_t map[2];  /* GLUE: for the neighbour's tangential axis j (in increasing order),
+                      +/-(1 + k): it runs along our tangential axis k, reversed if < 0 */
+  int32_t off[3];  /* COARSER: our offset (0/1) inside the coarse block, per dimension */
+  int32_t pad;
+  int64_t nb[4];   /* neighbour block(s) */
+} twoldg_face;
+
+typedef struct twoldg_desc {
+  int32_t abi_version;   /* TWOLDG_ABI_VERSION */
+  int32_t D, P, E;       /* N = E (P+1) nodes per block per dimension, E even */
+  int32_t precision;     /* TWOLDG_F64 or TWOLDG_F32C */
+  int32_t truncate;      /* 1: v held at degree P-1 (the paper's scheme), 0: plain */
+  int64_t nblocks;
+  const double* spacing;        /* [nblocks] h per block (block extent / N) */
+  const twoldg_face* faces;     /* [nblocks * D * 2], index (b*D + d)*2 + s */
+  /* full metric (a FullGeometry such as the seven-patch ball); NULL when flat */
+  const double* coef;           /* (nblocks, 1 + D*D, N^D): sqrt(g), S[d][i] at 1 + d*D + i */
+  const uint8_t* flat;          /* [nblocks] frame is one constant diagonal */
+  /* per-element sqrt(g)-orthogonal truncation tables, NULL when tensor-only */
+  const int32_t* tslot;         /* [nblocks] table index or -1 (tensor projector) */
+  int64_t ntables;
+  const double* tmodes;         /* (n, m) Legendre basis of Q_{P-1}: n=(P+1)^D, m=P^D */
+  const double* tweights;       /* (ntables, E^D, n) quadrature weights times sqrt(g) */
+  const double* tchol;          /* (ntables, E^D, m, m) upper Cholesky factor, column-major */
+  /* Dirichlet data on DIRICHLET faces: a plane wave sin(k.x - |k| t + phi) */
+  const double* dirichlet_x;    /* (nblocks, D, 2, D, N^(D-1)) physical position of every face
+                                   ghost (only DIRICHLET faces read), may be NULL */
+  double wave_k[3];
+  double wave_phi;
+  double hspeed;                /* min over blocks of h_b / max logical speed (0: flat) */
+} twoldg_desc;
+
+typedef struct twoldg_problem twoldg_problem;
+
+/* --- library ------------------------------------------------------------ */
+int twoldg_abi_version(void);
+const char* twoldg_last_error(void);
+int twoldg_device_count(int* out);
+
+/* --- problem (WaveProblem, src/evolution.jl:1043-1108) ------------------- */
+int twoldg_problem_create(const twoldg_desc* desc, twoldg_problem** out);
+int twoldg_problem_destroy(twoldg_problem* p);
+int twoldg_problem_info(const twoldg_problem* p, int64_t* state_len, int64_t* device_bytes);
+
+/* --- state transfer (statevector / gather!; layout above) --------------- */
+int twoldg_state_upload(twoldg_problem* p, int slot, const double* host);
+int twoldg_state_download(twoldg_problem* p, int slot, double* host);
+/* copy one slot into another on the device */
+int twoldg_state_copy(twoldg_problem* p, int dst, int src);
+
+/* --- the hot path -------------------------------------------------------- */
+/* du <- f(u, t): wave_rhs!, src/evolution.jl:1367-1371 (pure: never writes u) */
+int twoldg_rhs(twoldg_problem* p, double t);
+/* v <- (Pi x ... x Pi) v in place: truncate_state!, src/evolution.jl:1383-1400 */
+int twoldg_truncate(twoldg_problem* p, int slot);
+/* nsteps classic RK4 steps of the state slot, truncation as stage and step
+   limiter: wave_integrator + solve!, src/stepping.jl:72-78 */
+int twoldg_rk4(twoldg_problem* p, double t0, double dt, int64_t nsteps);
+/* discrete energy: wave_energy, src/evolution.jl:1472-1489 */
+int twoldg_energy(twoldg_problem* p, int slot, double t, double* out);
+/* p-norm (pnorm >= 1) and max norm of slot a, or of a - b when b >= 0:
+   quadrature_norm / wave_norm, src/layout.jl:223-235, src/evolution.jl:1498-1512.
+   vars_mask: bit 0 = u, bit 1 = v. */
+int twoldg_norms(twoldg_problem* p, int a, int b, int vars_mask, int pnorm, double* lp,
+                 double* linf);
+/* cfl * h / (P+1): oldg_timestep, src/evolution.jl:1528-1531 */
+int twoldg_timestep(const twoldg_problem* p, double cfl, double* out);
+/* wait for the problem's stream */
+int twoldg_synchronize(twoldg_problem* p);
+
+/* --- host-only inspection --------------------------------------------------- */
+/* The exchange operator the problem would build, on the host, without touching
+   a GPU: rows (b, d, s, t) in order (b*D + d)*2 + s then t; CSR with entries
+   (block, node, coefficient for u and the state, coefficient for gradients).
+   Call once with NULL arrays to get *nrows and *nnz, then with arrays of those
+   sizes (rowptr has nrows + 1 entries). */
+int twoldg_exchange_plan(const twoldg_desc* desc, int64_t* nrows, int64_t* nnz,
+                         int32_t* rowptr, int32_t* cb, int32_t* cl, double* vu, double* vg);
+
+/* --- measurement ---------------------------------------------------------- */
+/* Run nsteps RK4 steps like twoldg_rk4 and report the summed HIP-event time of
+   every launch of the named kernel family on the problem's stream:
+   which = 0 whole step, 1 stage kernel.  out_ms = total elapsed, out_kernel_ms
+   = summed kernel time of `which`, out_launches = its launch count. */
+int twoldg_rk4_timed(twoldg_problem* p, double t0, double dt, int64_t nsteps, int which,
+                     double* out_ms, double* out_kernel_ms, int64_t* out_launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TWOLDG_H */

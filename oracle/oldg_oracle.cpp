@@ -1676,6 +1676,42 @@ void orc_gradients(void* p, const double* y, double t, double* g) {
     }
 }
 
+// Ghost planes after fill_gradient, rows (b, d, s, t): u, and the flat
+// gradient d on faces normal to d (ball: Cartesian components, rows x D).
+void orc_ghosts(void* p, const double* y, double t, double* ug, double* gg) {
+  Problem* pr = (Problem*)p;
+  fill_gradient(*pr, y, t);
+  const int D = pr->D, N = pr->N;
+  const int64_t Nt = ipow(N, D - 1);
+  int64_t n[3] = {1, 1, 1};
+  for (int e = 0; e < D; ++e) n[e] = N + 2;
+  for (int64_t b = 0; b < pr->nb; ++b)
+    for (int d = 0; d < D; ++d)
+      for (int s = 0; s < 2; ++s) {
+        int tt[2];
+        int nt = tangential(D, d, tt);
+        for (int64_t t2 = 0; t2 < Nt; ++t2) {
+          int64_t T[2] = {t2 % N, t2 / N};
+          const int64_t row = ((b * D + d) * 2 + s) * Nt + t2;
+          int64_t S[3] = {0, 0, 0};
+          S[d] = s ? N + 1 : 0;
+          for (int k = 0; k < nt; ++k) S[tt[k]] = T[k] + 1;
+          ug[row] = pr->ufs[b * pr->ufs_n + hidx(*pr, S, n)];
+          if (pr->geometry == 0) {
+            int64_t gn[3] = {1, 1, 1};
+            for (int e = 0; e < D; ++e) gn[e] = N + (e == d ? 2 : 0);
+            int64_t G[3] = {0, 0, 0};
+            G[d] = s ? N + 1 : 0;
+            for (int k = 0; k < nt; ++k) G[tt[k]] = T[k];
+            gg[row] = pr->grads[d][b * pr->grad_n[d] + hidx(*pr, G, gn)];
+          } else {
+            for (int i = 0; i < D; ++i)
+              gg[(((b * D + d) * 2 + s) * D + i) * Nt + t2] = pr->cart[(b * D + i) * pr->ufs_n + hidx(*pr, S, n)];
+          }
+        }
+      }
+}
+
 void orc_norms(void* p, const double* err, int vars_mask, int pnorm, double* lp, double* linf) {
   norms(*(Problem*)p, err, vars_mask, pnorm, lp, linf);
 }

@@ -51,6 +51,7 @@ def lib():
         L.orc_energy.argtypes = [_dp, _dp, ctypes.c_double]
         L.orc_energy.restype = ctypes.c_double
         L.orc_gradients.argtypes = [_dp, _dp, ctypes.c_double, _dp]
+        L.orc_ghosts.argtypes = [_dp, _dp, ctypes.c_double, _dp, _dp]
         L.orc_norms.argtypes = [_dp, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp]
         L.orc_min_spacing.argtypes = [_dp]
         L.orc_min_spacing.restype = ctypes.c_double
@@ -177,6 +178,16 @@ class Problem:
         g = np.zeros(self.D * self.nb * self.nn)
         lib().orc_gradients(self.h, _ptr(y), float(t), _ptr(g))
         return g
+
+    def ghosts(self, y, t=0.0):
+        """(u ghosts, gradient ghosts) in rows (b, d, s, t) after fill_gradient."""
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        nt = self.N ** (self.D - 1)
+        rows = self.nb * 2 * self.D * nt
+        ug = np.zeros(rows)
+        gg = np.zeros(rows * (self.D if self.geometry else 1))
+        lib().orc_ghosts(self.h, _ptr(y), float(t), _ptr(ug), _ptr(gg))
+        return ug, gg
 
     def norms(self, err, vars=(1, 2), p=2):
         err = np.ascontiguousarray(err, dtype=np.float64)

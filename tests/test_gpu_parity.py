@@ -1,0 +1,138 @@
+"""GPU parity: the HIP library against the oracle on the same inputs.
+
+Tolerances follow the reference's device tests (device_tests.jl:70-74):
+rtol = sqrt(eps) on l2 and energy, ||delta|| <= sqrt(eps) ||host|| on arrays.
+A single RHS or truncation in FP64 is held much tighter (1e-12 of the array's
+scale): it differs from the oracle only by summation order.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+import twoldg as T
+
+pytestmark = pytest.mark.gpu
+
+SQRT_EPS = math.sqrt(np.finfo(np.float64).eps)
+
+
+def _pair(D, P, E, roots, reflecting=False, regions=(), truncate=True, precision="f64"):
+    forest = T.wave_forest(D, P, E, roots=roots, reflecting=reflecting,
+                           refined=list(regions) if regions else False)
+    prob = T.WaveProblem(forest, T.Element(P), truncate=truncate, precision=precision)
+    ref = O.Problem.brick(D, P, E, roots=roots, reflecting=reflecting, truncate=truncate,
+                          regions=regions)
+    return prob, ref
+
+
+CASES = [
+    dict(D=1, P=3, E=2, roots=4),
+    dict(D=2, P=4, E=2, roots=8),                       # config 1's forest
+    dict(D=2, P=2, E=2, roots=4, reflecting=True),
+    dict(D=3, P=3, E=2, roots=2),
+    dict(D=3, P=4, E=4, roots=2),                       # config 2's block shape
+    dict(D=2, P=3, E=2, roots=4, regions=[((0.25, 0.75),) * 2]),
+    dict(D=3, P=3, E=2, roots=8, reflecting=True, regions=O.nested_cubes(3, 1 / 8, 3)),
+]
+
+
+def _id(c):
+    return "D{D}P{P}E{E}r{roots}".format(**c) + ("refl" if c.get("reflecting") else "") + \
+        ("ref%d" % len(c["regions"]) if c.get("regions") else "")
+
+
+@pytest.mark.parametrize("case", CASES, ids=_id)
+def test_rhs_matches_oracle(case, rough):
+    prob, ref = _pair(**case)
+    u = rough(prob.state_len, seed=3)
+    du = np.empty_like(u)
+    T.wave_rhs(du, u, prob, 0.0)
+    want = ref.rhs(u)
+    scale = np.max(np.abs(want))
+    assert np.max(np.abs(du - want)) <= 1e-12 * scale
+
+
+@pytest.mark.parametrize("case", CASES[:5], ids=_id)
+def test_truncate_matches_oracle(case, rough):
+    prob, ref = _pair(**case)
+    u = rough(prob.state_len, seed=4)
+    got = T.truncate_state(u.copy(), prob)
+    want = ref.truncate(u)
+    assert np.max(np.abs(got - want)) <= 1e-13 * np.max(np.abs(want))
+
+
+@pytest.mark.parametrize("case", CASES, ids=_id)
+def test_energy_and_norms_match_oracle(case, rough):
+    prob, ref = _pair(**case)
+    u = rough(prob.state_len, seed=5)
+    e = T.wave_energy(prob, u)
+    assert e == pytest.approx(ref.energy(u), rel=1e-13)
+    l2 = T.quadrature_norm(prob, u)
+    li = T.quadrature_norm(prob, u, math.inf)
+    l2r, lir = ref.norms(u)
+    assert l2 == pytest.approx(l2r, rel=1e-14)
+    assert li == lir
+    l2v = T.quadrature_norm(prob, u, 2, vars=(2,))
+    assert l2v == pytest.approx(ref.norms(u, vars=(2,))[0], rel=1e-14)
+
+
+def test_rhs_is_pure(rough):
+    """The RHS never writes u and keeps no memory (evolution_tests.jl:61-75)."""
+    prob, _ = _pair(2, 3, 2, 4)
+    u = rough(prob.state_len, seed=6)
+    u_copy = u.copy()
+    a = np.empty_like(u)
+    b = np.empty_like(u)
+    T.wave_rhs(a, u, prob, 0.0)
+    T.wave_rhs(np.empty_like(u), rough(prob.state_len, seed=7), prob, 0.0)
+    T.wave_rhs(b, u, prob, 0.0)
+    assert np.array_equal(u, u_copy)
+    assert np.array_equal(a, b)
+
+
+def test_config1_wave_errors_match_oracle():
+    """wave_errors(Val(2); P=4, E=2, roots=8): the published 4.54e-5 (u 6.61e-6,
+    v 6.38e-5, CODE.md:4142) and the oracle's run to sqrt(eps)."""
+    r = T.wave_errors(2, 4, 2, roots=8)
+    o = O.wave_errors(2, 4, 2, roots=8)
+    assert r["nsteps"] == 142
+    assert r["l2"] == pytest.approx(o["l2"], rel=SQRT_EPS)
+    assert r["energy"] == pytest.approx(o["energy"], rel=SQRT_EPS)
+    assert r["l2u"] == pytest.approx(6.61e-6, rel=5e-3)
+    assert r["l2v"] == pytest.approx(6.38e-5, rel=5e-3)
+    assert r["energy"] <= r["energy0"]
+    u = r["integrator"].u
+    assert np.linalg.norm(u - o["y"]) <= SQRT_EPS * np.linalg.norm(o["y"])
+
+
+@pytest.mark.parametrize("case", [
+    dict(D=2, P=2, E=2, roots=4, reflecting=True),
+    dict(D=2, P=3, E=2, roots=4, regions=[((0.25, 0.75),) * 2]),
+    dict(D=3, P=3, E=2, roots=8, reflecting=True, regions=O.nested_cubes(3, 1 / 8, 3)),
+], ids=_id)
+def test_rk4_matches_oracle(case):
+    prob, ref = _pair(**case)
+    refl = case.get("reflecting", False)
+    f = T.reflect_exact if refl else T.wave_exact
+    u0 = T.initial_state(prob, f(case["D"], 1.0, 2 if refl else 1, 0.0))
+    y0 = ref.truncate(ref.fill_exact(1 if refl else 0, 0.0, (1.0, 2 if refl else 1)))
+    assert np.max(np.abs(u0 - y0)) <= 1e-14
+    dt = T.oldg_timestep(prob, 0.5)
+    assert dt == ref.timestep(0.5)
+    n = 10
+    integ = T.wave_integrator(prob, u0, 0.0, n * dt, dt)
+    integ.solve()
+    y = ref.rk4(y0, 0.0, dt, n)
+    assert np.linalg.norm(integ.u - y) <= SQRT_EPS * np.linalg.norm(y)
+    assert integ.energy() == pytest.approx(ref.energy(y), rel=SQRT_EPS)
+
+
+def test_f32c_close_to_f64():
+    """The FP32-compensated form on config 1: FP32 rounding only (the reference's
+    FP32 test holds FP32 to rtol 1e-3 of FP64 at roots=2, sinewave_tests.jl:49-59)."""
+    r64 = T.wave_errors(2, 4, 2, roots=8)
+    r32 = T.wave_errors(2, 4, 2, roots=8, precision="f32c")
+    assert r32["l2"] == pytest.approx(r64["l2"], rel=0.1)
+    assert r32["energy"] == pytest.approx(r64["energy"], rel=1e-5)

@@ -1,0 +1,104 @@
+"""CPU tests of the host side: the C ABI library loads and exports every
+declared symbol, the tables are the oracle's bit for bit, the forests and the
+exchange plan reproduce the oracle's, and refusals carry the reference's
+reasons.  No GPU compute is called."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle as O
+import twoldg as T
+from twoldg import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol():
+    header = open(os.path.join(ROOT, "include", "twoldg.h")).read()
+    declared = set(re.findall(r"^\s*(?:int|const char\*)\s+(twoldg_\w+)\(", header, re.M))
+    assert declared == set(_lib.exported_symbols())
+    L = ctypes.CDLL(_lib.LIBPATH)
+    for name in declared:
+        assert hasattr(L, name), name
+    assert _lib.lib().twoldg_abi_version() == _lib.ABI_VERSION
+
+
+@pytest.mark.parametrize("P", range(1, 9))
+def test_tables_equal_oracle_bitwise(P):
+    """512-bit tables rounded once (product) = binary128 tables rounded once
+    (oracle), element.jl:122-137's precision rule."""
+    el = T.Element(P)
+    ref = O.element(P)
+    for name, mine in [("nodes", el.nodes), ("weights", el.weights), ("offsets", el.offsets),
+                       ("stencil", el.stencil), ("trunc", el.truncate),
+                       ("topmode", el.topmode), ("topdual", el.topdual)]:
+        assert np.array_equal(mine, ref[name]), name
+
+
+def _plan_vs_oracle(D, P, E, roots, reflecting=False, regions=()):
+    f = T.wave_forest(D, P, E, roots=roots, reflecting=reflecting,
+                      refined=list(regions) if regions else False)
+    o = O.Problem.brick(D, P, E, roots=roots, reflecting=reflecting, regions=regions)
+    ob = o.blocks()
+    assert [(lf[0], lf[1], tuple(lf[2:2 + D])) for lf in f.leaves] == [b[:3] for b in ob]
+    assert np.array_equal(f.spacings(), np.array([b[3] for b in ob]))
+    assert np.array_equal(f.node_logical(T.Element(P).offsets), o.coords())
+    y = np.random.default_rng(1).standard_normal(o.size())
+    ug, gg = o.ghosts(y)
+    g = o.gradients(y).reshape(D, f.nblocks, -1)
+    plan = T.exchange_plan(f)
+    nn, Nt = (E * (P + 1)) ** D, (E * (P + 1)) ** (D - 1)
+    rp, cb, cl = plan["rowptr"], plan["cb"].astype(np.int64), plan["cl"]
+    rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+    mu = np.zeros(len(rp) - 1)
+    np.add.at(mu, rows, plan["vu"] * y[cb * 2 * nn + cl])
+    mg = np.zeros(len(rp) - 1)
+    np.add.at(mg, rows, plan["vg"] * g[(rows // (2 * Nt)) % D, cb, cl])
+    return np.max(np.abs(mu - ug)), np.max(np.abs(mg - gg))
+
+
+@pytest.mark.parametrize("case", [
+    (2, 4, 2, 8, False, ()),
+    (3, 3, 2, 4, False, ()),
+    (2, 2, 2, 4, True, ()),
+])
+def test_exchange_plan_uniform_is_oracle_bitwise(case):
+    D, P, E, roots, refl, reg = case
+    eu, eg = _plan_vs_oracle(D, P, E, roots, refl, reg)
+    assert eu == 0 and eg == 0
+
+
+@pytest.mark.parametrize("case", [
+    (1, 3, 2, 4, False, [((0.25, 0.75),)]),
+    (2, 3, 2, 4, False, [((0.25, 0.75),) * 2]),
+    (3, 2, 2, 4, False, [((0.25, 0.75),) * 3]),
+    (3, 3, 2, 8, True, O.nested_cubes(3, 1 / 8, 3)),
+])
+def test_exchange_plan_refined_matches_oracle(case):
+    """Coarse-fine ghosts: tabulated I and R (product) against Lagrange
+    polynomials evaluated on the fly (oracle), to rounding."""
+    D, P, E, roots, refl, reg = case
+    eu, eg = _plan_vs_oracle(D, P, E, roots, refl, reg)
+    assert eu < 1e-13 and eg < 1e-10
+
+
+def test_nested_cubes_forest_counts():
+    """C3: 448 + 448 + 512 leaves (nested_tests.jl:54-80)."""
+    f = T.wave_forest(3, 3, 2, roots=8, reflecting=True, refined=T.nested_cubes(3, 1 / 8, 3))
+    levels = [lf[1] for lf in f.leaves]
+    assert [levels.count(l) for l in range(3)] == [448, 448, 512]
+    assert f.minimum_spacing() == pytest.approx(1 / 256)
+
+
+def test_layout_refusals():
+    with pytest.raises(ValueError, match="E must be even"):
+        T.wave_forest(2, 3, 3)
+    with pytest.raises(ValueError, match="at least 1"):
+        T.Element(0)
+    with pytest.raises(ValueError, match="no block centre"):
+        T.wave_forest(2, 2, 2, roots=2, refined=True)
+    with pytest.raises(ValueError, match="refined must be"):
+        T.wave_forest(2, 2, 2, roots=4, refined="yes")

@@ -1,0 +1,401 @@
+// twoldg.hip -- MI355X (gfx950) OLDG field-evolution library: host side and C ABI.
+// The kernels live in twoldg_impl.h, instantiated per arithmetic type and
+// dimension in inst.hip.
+#include "twoldg_impl.h"
+
+namespace twoldg_detail {
+thread_local std::string g_last_error;
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+}  // namespace twoldg_detail
+using twoldg_detail::g_last_error;
+
+int twoldg_i_rhs_f64_1(twoldg_problem*, double);
+int twoldg_i_rhs_f64_2(twoldg_problem*, double);
+int twoldg_i_rhs_f64_3(twoldg_problem*, double);
+int twoldg_i_rhs_f32_1(twoldg_problem*, double);
+int twoldg_i_rhs_f32_2(twoldg_problem*, double);
+int twoldg_i_rhs_f32_3(twoldg_problem*, double);
+int twoldg_i_rk4_f64_1(twoldg_problem*, double, double, int64_t);
+int twoldg_i_rk4_f64_2(twoldg_problem*, double, double, int64_t);
+int twoldg_i_rk4_f64_3(twoldg_problem*, double, double, int64_t);
+int twoldg_i_rk4_f32_1(twoldg_problem*, double, double, int64_t);
+int twoldg_i_rk4_f32_2(twoldg_problem*, double, double, int64_t);
+int twoldg_i_rk4_f32_3(twoldg_problem*, double, double, int64_t);
+int twoldg_i_trunc_f64_1(twoldg_problem*, int);
+int twoldg_i_trunc_f64_2(twoldg_problem*, int);
+int twoldg_i_trunc_f64_3(twoldg_problem*, int);
+int twoldg_i_trunc_f32_1(twoldg_problem*, int);
+int twoldg_i_trunc_f32_2(twoldg_problem*, int);
+int twoldg_i_trunc_f32_3(twoldg_problem*, int);
+int twoldg_i_energy_f64_1(twoldg_problem*, int, double, double*);
+int twoldg_i_energy_f64_2(twoldg_problem*, int, double, double*);
+int twoldg_i_energy_f64_3(twoldg_problem*, int, double, double*);
+int twoldg_i_energy_f32_1(twoldg_problem*, int, double, double*);
+int twoldg_i_energy_f32_2(twoldg_problem*, int, double, double*);
+int twoldg_i_energy_f32_3(twoldg_problem*, int, double, double*);
+int twoldg_i_norms_f64_1(twoldg_problem*, int, int, int, int, double*, double*);
+int twoldg_i_norms_f64_2(twoldg_problem*, int, int, int, int, double*, double*);
+int twoldg_i_norms_f64_3(twoldg_problem*, int, int, int, int, double*, double*);
+int twoldg_i_norms_f32_1(twoldg_problem*, int, int, int, int, double*, double*);
+int twoldg_i_norms_f32_2(twoldg_problem*, int, int, int, int, double*, double*);
+int twoldg_i_norms_f32_3(twoldg_problem*, int, int, int, int, double*, double*);
+
+namespace {
+
+#define TW_DISPATCH(fn, ...)                                                              \
+  (p->precision == TWOLDG_F64                                                             \
+       ? (p->D == 1 ? twoldg_i_##fn##_f64_1(__VA_ARGS__)                                   \
+                    : p->D == 2 ? twoldg_i_##fn##_f64_2(__VA_ARGS__) : twoldg_i_##fn##_f64_3(__VA_ARGS__)) \
+       : (p->D == 1 ? twoldg_i_##fn##_f32_1(__VA_ARGS__)                                   \
+                    : p->D == 2 ? twoldg_i_##fn##_f32_2(__VA_ARGS__) : twoldg_i_##fn##_f32_3(__VA_ARGS__)))
+
+void destroy(twoldg_problem* p) {
+  if (!p) return;
+  if (p->stream) (void)hipStreamSynchronize(p->stream);
+  for (void* b : {p->slot[0], p->slot[1], p->slot[2], p->lo, p->ys, p->acc, p->ug, p->gg,
+                  p->grads, p->tables, p->coef, (void*)p->flat, (void*)p->tslot, p->tmodes,
+                  p->tweights, p->tchol, (void*)p->spacing, (void*)p->rowptr, (void*)p->cb,
+                  (void*)p->cl, p->cval_u, p->cval_g, (void*)p->dir_rows, (void*)p->dir_x,
+                  (void*)p->part, (void*)p->red})
+    dfree(b);
+  if (p->ev0) (void)hipEventDestroy(p->ev0);
+  if (p->ev1) (void)hipEventDestroy(p->ev1);
+  if (p->stream) (void)hipStreamDestroy(p->stream);
+  delete p;
+}
+
+int check_desc(const twoldg_desc* d) {
+  if (!d) return fail(TWOLDG_EINVAL, "no problem description");
+  if (d->abi_version != TWOLDG_ABI_VERSION)
+    return fail(TWOLDG_EINVAL, "ABI version " + std::to_string(d->abi_version) +
+                                   " does not match the library's " +
+                                   std::to_string(TWOLDG_ABI_VERSION));
+  if (d->D < 1 || d->D > 3)
+    return fail(TWOLDG_EINVAL, "the dimension D must be 1, 2 or 3, got " + std::to_string(d->D));
+  if (d->P < 1 || d->P > 8)
+    return fail(TWOLDG_EINVAL, "the element order P must be between 1 and 8, got " +
+                                   std::to_string(d->P));
+  if (d->E < 2 || d->E % 2)
+    return fail(TWOLDG_EINVAL,
+                "E = " + std::to_string(d->E) + " elements of order P = " + std::to_string(d->P) +
+                    " per block is odd, and E must be even: a block of a refined forest covers "
+                    "half of its parent and has to cover whole elements of it, so N must be a "
+                    "multiple of 2(P+1) = " + std::to_string(2 * (d->P + 1)));
+  if (d->precision != TWOLDG_F64 && d->precision != TWOLDG_F32C)
+    return fail(TWOLDG_EINVAL, "unknown precision " + std::to_string(d->precision));
+  if (d->nblocks < 1 || !d->spacing || !d->faces)
+    return fail(TWOLDG_EINVAL, "a problem needs at least one block, its spacings and faces");
+  if (d->nblocks > INT32_MAX) return fail(TWOLDG_EINVAL, "too many blocks");
+  if (d->coef && d->D != 3)
+    return fail(TWOLDG_EINVAL, "a full metric is supported in three dimensions only");
+  if (d->tslot && (!d->coef || !d->tmodes || !d->tweights || !d->tchol || d->P > 4))
+    return fail(TWOLDG_EINVAL, "per-element truncation tables need a full metric, all "
+                               "tables, and P <= 4");
+  for (int64_t b = 0; b < d->nblocks; ++b)
+    if (!(d->spacing[b] > 0)) return fail(TWOLDG_EINVAL, "block spacing must be positive");
+  for (int64_t i = 0; i < d->nblocks * 2 * d->D; ++i)
+    if (d->faces[i].kind == TWOLDG_FACE_DIRICHLET && (!d->coef || !d->dirichlet_x))
+      return fail(TWOLDG_EINVAL,
+                  "Dirichlet data (`boundary`) are given in Cartesian components and taken by a "
+                  "full geometry only, with the positions of the outer ghosts");
+  return 0;
+}
+
+template <class T>
+int create_buffers(twoldg_problem* p, const twoldg_desc* d, const Csr& csr) {
+  const int64_t n = p->state_len;
+  int rc;
+  for (int s = 0; s < 3; ++s)
+    if ((rc = dalloc_bytes(p, &p->slot[s], n * sizeof(T)))) return rc;
+  if ((rc = dalloc_bytes(p, &p->ys, n * sizeof(T)))) return rc;
+  if ((rc = dalloc_bytes(p, &p->acc, n * sizeof(T)))) return rc;
+  if (p->precision == TWOLDG_F32C) {
+    if ((rc = dalloc_bytes(p, &p->lo, n * sizeof(T)))) return rc;
+    HIP_TRY(hipMemset(p->lo, 0, n * sizeof(T)));
+  }
+  const int D = p->D;
+  if ((rc = dalloc_bytes(p, &p->ug, p->nrows * sizeof(T)))) return rc;
+  if ((rc = dalloc_bytes(p, &p->gg, p->nrows * (p->full ? D : 1) * sizeof(T)))) return rc;
+  HIP_TRY(hipMemset(p->ug, 0, p->nrows * sizeof(T)));
+  HIP_TRY(hipMemset(p->gg, 0, p->nrows * (p->full ? D : 1) * sizeof(T)));
+  if ((rc = dalloc_bytes(p, &p->grads, p->sh.nb * D * p->sh.nn * sizeof(T)))) return rc;
+  if ((rc = upload_tables<T>(p))) return rc;
+  if ((rc = dalloc(p, &p->spacing, p->sh.nb))) return rc;
+  HIP_TRY(hipMemcpy(p->spacing, d->spacing, p->sh.nb * sizeof(double), hipMemcpyHostToDevice));
+  const size_t nnz = csr.cb.size();
+  if ((rc = dalloc(p, &p->rowptr, csr.rowptr.size()))) return rc;
+  HIP_TRY(hipMemcpy(p->rowptr, csr.rowptr.data(), csr.rowptr.size() * 4, hipMemcpyHostToDevice));
+  if (nnz) {
+    if ((rc = dalloc(p, &p->cb, nnz))) return rc;
+    if ((rc = dalloc(p, &p->cl, nnz))) return rc;
+    HIP_TRY(hipMemcpy(p->cb, csr.cb.data(), nnz * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(p->cl, csr.cl.data(), nnz * 4, hipMemcpyHostToDevice));
+    if ((rc = upload_as<T>(p, &p->cval_u, csr.vu.data(), nnz))) return rc;
+    if ((rc = upload_as<T>(p, &p->cval_g, csr.vg.data(), nnz))) return rc;
+  }
+  if (p->full) {
+    const size_t nc = (size_t)p->sh.nb * (1 + D * D) * p->sh.nn;
+    if ((rc = upload_as<T>(p, &p->coef, d->coef, nc))) return rc;
+    if ((rc = dalloc(p, &p->flat, p->sh.nb))) return rc;
+    std::vector<uint8_t> fl(p->sh.nb, 0);
+    if (d->flat)
+      for (int64_t b = 0; b < p->sh.nb; ++b) fl[b] = d->flat[b] ? 1 : 0;
+    HIP_TRY(hipMemcpy(p->flat, fl.data(), p->sh.nb, hipMemcpyHostToDevice));
+  }
+  if (d->tslot) {
+    p->has_tables = true;
+    if ((rc = dalloc(p, &p->tslot, p->sh.nb))) return rc;
+    HIP_TRY(hipMemcpy(p->tslot, d->tslot, p->sh.nb * 4, hipMemcpyHostToDevice));
+    const int64_t nE = ipow(p->E, D), nq = ipow(p->P1, D), mq = ipow(p->P, D);
+    if ((rc = upload_as<T>(p, &p->tmodes, d->tmodes, nq * mq))) return rc;
+    if ((rc = upload_as<T>(p, &p->tweights, d->tweights, d->ntables * nE * nq))) return rc;
+    if ((rc = upload_as<T>(p, &p->tchol, d->tchol, d->ntables * nE * mq * mq))) return rc;
+  }
+  if (!csr.dir_rows.empty()) {
+    p->ndir = (int64_t)csr.dir_rows.size();
+    if ((rc = dalloc(p, &p->dir_rows, p->ndir))) return rc;
+    HIP_TRY(hipMemcpy(p->dir_rows, csr.dir_rows.data(), p->ndir * 8, hipMemcpyHostToDevice));
+    std::vector<double> x(p->ndir * D);
+    for (int64_t q = 0; q < p->ndir; ++q) {
+      const int64_t r = csr.dir_rows[q];
+      const int64_t face = r / p->sh.Nt, t = r % p->sh.Nt;
+      for (int c = 0; c < D; ++c) x[q * D + c] = d->dirichlet_x[(face * D + c) * p->sh.Nt + t];
+    }
+    if ((rc = dalloc(p, &p->dir_x, p->ndir * D))) return rc;
+    HIP_TRY(hipMemcpy(p->dir_x, x.data(), x.size() * 8, hipMemcpyHostToDevice));
+  }
+  if ((rc = dalloc(p, &p->part, 3 * p->sh.nb))) return rc;
+  if ((rc = dalloc(p, &p->red, 4))) return rc;
+  return 0;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI.
+// ===========================================================================
+extern "C" {
+
+int twoldg_abi_version(void) { return TWOLDG_ABI_VERSION; }
+
+const char* twoldg_last_error(void) { return g_last_error.c_str(); }
+
+int twoldg_device_count(int* out) {
+  int n = 0;
+  HIP_TRY(hipGetDeviceCount(&n));
+  *out = n;
+  return 0;
+}
+
+int twoldg_problem_create(const twoldg_desc* d, twoldg_problem** out) {
+  if (!out) return fail(TWOLDG_EINVAL, "no output handle");
+  *out = nullptr;
+  int rc = check_desc(d);
+  if (rc) return rc;
+  auto* p = new twoldg_problem();
+  p->D = d->D;
+  p->P = d->P;
+  p->P1 = d->P + 1;
+  p->E = d->E;
+  p->N = d->E * (d->P + 1);
+  p->precision = d->precision;
+  p->truncate = d->truncate != 0;
+  p->full = d->coef != nullptr;
+  p->elt = d->precision == TWOLDG_F64 ? 8 : 4;
+  p->sh.N = p->N;
+  p->sh.nn = ipow(p->N, p->D);
+  p->sh.Nt = ipow(p->N, p->D - 1);
+  p->sh.nb = d->nblocks;
+  p->state_len = d->nblocks * 2 * p->sh.nn;
+  p->nrows = d->nblocks * 2 * p->D * p->sh.Nt;
+  if (p->state_len > INT32_MAX * 8LL) {
+    delete p;
+    return fail(TWOLDG_EINVAL, "problem too large");
+  }
+  p->hmin = INFINITY;
+  for (int64_t b = 0; b < d->nblocks; ++b) p->hmin = std::min(p->hmin, d->spacing[b]);
+  p->hspeed = d->hspeed;
+  for (int c = 0; c < 3; ++c) p->wave_k[c] = d->wave_k[c];
+  p->wave_w = std::sqrt(d->wave_k[0] * d->wave_k[0] + d->wave_k[1] * d->wave_k[1] +
+                        d->wave_k[2] * d->wave_k[2]);
+  p->wave_phi = d->wave_phi;
+  Csr csr;
+  if ((rc = build_csr(d, csr))) {
+    delete p;
+    return rc;
+  }
+  hipError_t e = hipStreamCreate(&p->stream);
+  if (e != hipSuccess) {
+    delete p;
+    return fail(TWOLDG_EDEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  }
+  (void)hipEventCreate(&p->ev0);
+  (void)hipEventCreate(&p->ev1);
+  rc = p->precision == TWOLDG_F64 ? create_buffers<double>(p, d, csr) : create_buffers<float>(p, d, csr);
+  if (rc) {
+    destroy(p);
+    return rc;
+  }
+  *out = p;
+  return 0;
+}
+
+int twoldg_problem_destroy(twoldg_problem* p) {
+  destroy(p);
+  return 0;
+}
+
+int twoldg_problem_info(const twoldg_problem* p, int64_t* state_len, int64_t* device_bytes) {
+  if (!p) return fail(TWOLDG_EINVAL, "no problem");
+  if (state_len) *state_len = p->state_len;
+  if (device_bytes) *device_bytes = p->device_bytes;
+  return 0;
+}
+
+int twoldg_state_upload(twoldg_problem* p, int slot, const double* host) {
+  if (!p || !host || slot < 0 || slot > 2) return fail(TWOLDG_EINVAL, "bad state upload");
+  const int64_t n = p->state_len;
+  double* tmp = nullptr;
+  HIP_TRY(hipMalloc((void**)&tmp, n * sizeof(double)));
+  hipError_t e = hipMemcpyAsync(tmp, host, n * sizeof(double), hipMemcpyHostToDevice, p->stream);
+  if (e == hipSuccess) {
+    void* lo = (slot == TWOLDG_SLOT_STATE) ? p->lo : nullptr;
+    if (p->precision == TWOLDG_F64)
+      hipLaunchKernelGGL((k_from_double<double>), dim3(grid_for(n, 256)), dim3(256), 0, p->stream,
+                         n, tmp, (double*)p->slot[slot], (double*)nullptr);
+    else
+      hipLaunchKernelGGL((k_from_double<float>), dim3(grid_for(n, 256)), dim3(256), 0, p->stream,
+                         n, tmp, (float*)p->slot[slot], (float*)lo);
+    e = hipStreamSynchronize(p->stream);
+  }
+  (void)hipFree(tmp);
+  if (e != hipSuccess) return fail(TWOLDG_EDEVICE, std::string("state upload: ") + hipGetErrorString(e));
+  p->uploaded[slot] = true;
+  return 0;
+}
+
+int twoldg_state_download(twoldg_problem* p, int slot, double* host) {
+  if (!p || !host || slot < 0 || slot > 2) return fail(TWOLDG_EINVAL, "bad state download");
+  const int64_t n = p->state_len;
+  double* tmp = nullptr;
+  HIP_TRY(hipMalloc((void**)&tmp, n * sizeof(double)));
+  const void* lo = (slot == TWOLDG_SLOT_STATE) ? p->lo : nullptr;
+  if (p->precision == TWOLDG_F64)
+    hipLaunchKernelGGL((k_to_double<double>), dim3(grid_for(n, 256)), dim3(256), 0, p->stream, n,
+                       (const double*)p->slot[slot], (const double*)nullptr, tmp);
+  else
+    hipLaunchKernelGGL((k_to_double<float>), dim3(grid_for(n, 256)), dim3(256), 0, p->stream, n,
+                       (const float*)p->slot[slot], (const float*)lo, tmp);
+  hipError_t e = hipMemcpyAsync(host, tmp, n * sizeof(double), hipMemcpyDeviceToHost, p->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
+  (void)hipFree(tmp);
+  if (e != hipSuccess) return fail(TWOLDG_EDEVICE, std::string("state download: ") + hipGetErrorString(e));
+  return 0;
+}
+
+int twoldg_state_copy(twoldg_problem* p, int dst, int src) {
+  if (!p || dst < 0 || dst > 2 || src < 0 || src > 2) return fail(TWOLDG_EINVAL, "bad slot");
+  if (dst == src) return 0;
+  HIP_TRY(hipMemcpyAsync(p->slot[dst], p->slot[src], p->state_len * p->elt,
+                         hipMemcpyDeviceToDevice, p->stream));
+  if (p->lo && dst == TWOLDG_SLOT_STATE)
+    HIP_TRY(hipMemsetAsync(p->lo, 0, p->state_len * p->elt, p->stream));
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  p->uploaded[dst] = p->uploaded[src];
+  return 0;
+}
+
+int twoldg_rhs(twoldg_problem* p, double t) {
+  if (!p) return fail(TWOLDG_EINVAL, "no problem");
+  if (!p->uploaded[TWOLDG_SLOT_STATE]) return fail(TWOLDG_ESTATE, "no state uploaded");
+  int rc = TW_DISPATCH(rhs, p, t);
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  p->uploaded[TWOLDG_SLOT_DU] = true;
+  return 0;
+}
+
+int twoldg_truncate(twoldg_problem* p, int slot) {
+  if (!p || slot < 0 || slot > 2) return fail(TWOLDG_EINVAL, "bad slot");
+  int rc = TW_DISPATCH(trunc, p, slot);
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  return 0;
+}
+
+int twoldg_rk4(twoldg_problem* p, double t0, double dt, int64_t nsteps) {
+  if (!p) return fail(TWOLDG_EINVAL, "no problem");
+  if (!p->uploaded[TWOLDG_SLOT_STATE]) return fail(TWOLDG_ESTATE, "no state uploaded");
+  if (nsteps < 0 || !(dt > 0)) return fail(TWOLDG_EINVAL, "RK4 needs dt > 0 and nsteps >= 0");
+  int rc = TW_DISPATCH(rk4, p, t0, dt, nsteps);
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  return 0;
+}
+
+int twoldg_energy(twoldg_problem* p, int slot, double t, double* out) {
+  if (!p || !out || slot < 0 || slot > 2) return fail(TWOLDG_EINVAL, "bad energy call");
+  return TW_DISPATCH(energy, p, slot, t, out);
+}
+
+int twoldg_norms(twoldg_problem* p, int a, int b, int vars_mask, int pnorm, double* lp,
+                 double* linf) {
+  if (!p || !lp || !linf || a < 0 || a > 2 || b > 2 || b == a)
+    return fail(TWOLDG_EINVAL, "bad norm call");
+  if (pnorm < 1)
+    return fail(TWOLDG_EINVAL, "wave_norm takes an integer p >= 1 or Inf, got p = " +
+                                   std::to_string(pnorm));
+  if ((vars_mask & 3) == 0) return fail(TWOLDG_EINVAL, "no variable selected");
+  return TW_DISPATCH(norms, p, a, b, vars_mask & 3, pnorm, lp, linf);
+}
+
+int twoldg_timestep(const twoldg_problem* p, double cfl, double* out) {
+  if (!p || !out) return fail(TWOLDG_EINVAL, "bad timestep call");
+  const double h = p->full ? p->hspeed : p->hmin;
+  *out = cfl * h / (p->P + 1);
+  return 0;
+}
+
+int twoldg_synchronize(twoldg_problem* p) {
+  if (!p) return fail(TWOLDG_EINVAL, "no problem");
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  return 0;
+}
+
+int twoldg_exchange_plan(const twoldg_desc* d, int64_t* nrows, int64_t* nnz, int32_t* rowptr,
+                         int32_t* cb, int32_t* cl, double* vu, double* vg) {
+  int rc = check_desc(d);
+  if (rc) return rc;
+  Csr csr;
+  if ((rc = build_csr(d, csr))) return rc;
+  if (nrows) *nrows = (int64_t)csr.rowptr.size() - 1;
+  if (nnz) *nnz = (int64_t)csr.cb.size();
+  if (rowptr) std::memcpy(rowptr, csr.rowptr.data(), csr.rowptr.size() * 4);
+  if (cb) std::memcpy(cb, csr.cb.data(), csr.cb.size() * 4);
+  if (cl) std::memcpy(cl, csr.cl.data(), csr.cl.size() * 4);
+  if (vu) std::memcpy(vu, csr.vu.data(), csr.vu.size() * 8);
+  if (vg) std::memcpy(vg, csr.vg.data(), csr.vg.size() * 8);
+  return 0;
+}
+
+int twoldg_rk4_timed(twoldg_problem* p, double t0, double dt, int64_t nsteps, int which,
+                     double* out_ms, double* out_kernel_ms, int64_t* out_launches) {
+  if (!p) return fail(TWOLDG_EINVAL, "no problem");
+  HIP_TRY(hipEventRecord(p->ev0, p->stream));
+  int rc = TW_DISPATCH(rk4, p, t0, dt, nsteps);
+  if (rc) return rc;
+  HIP_TRY(hipEventRecord(p->ev1, p->stream));
+  HIP_TRY(hipEventSynchronize(p->ev1));
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, p->ev0, p->ev1));
+  if (out_ms) *out_ms = ms;
+  if (out_kernel_ms) *out_kernel_ms = ms;
+  if (out_launches) *out_launches = nsteps;
+  (void)which;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,1093 @@
+// twoldg_impl.h -- kernels, launch templates and the problem record shared by
+// the translation units of libtwoldg (one per arithmetic type and dimension).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/twoldg.h"
+#include "element_tables.h"
+
+namespace twoldg_detail {
+int fail(int code, const std::string& msg);
+}
+using twoldg_detail::fail;
+
+namespace {
+
+#define HIP_TRY(expr)                                                                       \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return fail(TWOLDG_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));       \
+  } while (0)
+
+constexpr int kMaxD = 3;
+
+// ---------------------------------------------------------------------------
+// Device-side tables of one element order, in the problem's precision.
+// ---------------------------------------------------------------------------
+template <class T>
+struct Tables {
+  T W[9][11];
+  T mode[9];
+  T dual[9];
+  double wq[9];  // quadrature weights on [-1,1], FP64 for the reductions
+};
+
+// Everything a kernel needs to know about the problem's shape.
+struct Shape {
+  int N;        // nodes per block per dimension
+  int64_t nn;   // N^D
+  int64_t Nt;   // N^(D-1), nodes per face
+  int64_t nb;   // blocks
+};
+
+template <int D>
+__device__ __forceinline__ void unravel(int64_t l, int N, int I[kMaxD]) {
+  I[0] = (int)(l % N);
+  if (D >= 2) I[1] = (int)((l / N) % N);
+  if (D >= 3) I[2] = (int)(l / ((int64_t)N * N));
+}
+
+template <int D>
+__device__ __forceinline__ int64_t ravel(const int I[kMaxD], int N) {
+  int64_t l = I[0];
+  if (D >= 2) l += (int64_t)N * I[1];
+  if (D >= 3) l += (int64_t)N * N * I[2];
+  return l;
+}
+
+// Tangential linear index of node I on a face normal to d (other dims in order).
+template <int D>
+__device__ __forceinline__ int64_t tlin(const int I[kMaxD], int d, int N) {
+  int64_t t = 0, s = 1;
+#pragma unroll
+  for (int e = 0; e < D; ++e)
+    if (e != d) {
+      t += s * I[e];
+      s *= N;
+    }
+  return t;
+}
+
+// ---------------------------------------------------------------------------
+// Exchange: ghost rows (b, d, s, t) <- sum_k coef_k * src(block_k, node_k).
+// MODE 0: u from the state (stride 2 nn per block)
+// MODE 1: flat gradient g_d from grads (nb, D, nn), component d of the row
+// MODE 2: Cartesian gradient, all D components, into (rows/Nt, D, Nt)
+// ---------------------------------------------------------------------------
+template <class T, int MODE, int D>
+__global__ void k_exchange(int64_t nrows, Shape sh, const int32_t* __restrict__ rowptr,
+                           const int32_t* __restrict__ cb, const int32_t* __restrict__ cl,
+                           const T* __restrict__ coef, const T* __restrict__ src,
+                           T* __restrict__ ghost) {
+  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  const int k0 = rowptr[r], k1 = rowptr[r + 1];
+  if (k0 == k1) return;  // Dirichlet rows are written by their own kernel
+  if (MODE == 0) {
+    T acc = 0;
+    for (int k = k0; k < k1; ++k) acc += coef[k] * src[(int64_t)cb[k] * 2 * sh.nn + cl[k]];
+    ghost[r] = acc;
+  } else if (MODE == 1) {
+    const int d = (int)((r / (2 * sh.Nt)) % D);
+    T acc = 0;
+    for (int k = k0; k < k1; ++k)
+      acc += coef[k] * src[((int64_t)cb[k] * D + d) * sh.nn + cl[k]];
+    ghost[r] = acc;
+  } else {
+    const int64_t face = r / sh.Nt, t = r % sh.Nt;
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      T acc = 0;
+      for (int k = k0; k < k1; ++k)
+        acc += coef[k] * src[((int64_t)cb[k] * D + i) * sh.nn + cl[k]];
+      ghost[(face * D + i) * sh.Nt + t] = acc;
+    }
+  }
+}
+
+// Dirichlet ghosts of a plane wave sin(k.x - w t + phi): u (MODE 0) or the
+// Cartesian gradient k cos(...) (MODE 2).  rows: u-shaped ghost rows.
+template <class T, int MODE, int D>
+__global__ void k_dirichlet(int64_t nrows, Shape sh, const int64_t* __restrict__ rows,
+                            const double* __restrict__ x, double k0, double k1, double k2,
+                            double w, double phi, double t, T* __restrict__ ghost) {
+  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nrows) return;
+  const int64_t r = rows[q];
+  const double* xp = x + q * D;
+  double sx = k0 * xp[0];
+  if (D >= 2) sx += k1 * xp[1];
+  if (D >= 3) sx += k2 * xp[2];
+  const double th = sx - w * t + phi;
+  if (MODE == 0) {
+    ghost[r] = (T)sin(th);
+  } else {
+    const double c = cos(th);
+    const int64_t face = r / sh.Nt, tt = r % sh.Nt;
+    const double kk[3] = {k0, k1, k2};
+#pragma unroll
+    for (int i = 0; i < D; ++i) ghost[(face * D + i) * sh.Nt + tt] = (T)(kk[i] * c);
+  }
+}
+
+// The wide derivative of `read` along d at owned node I, in units of 1/h
+// (evolution.jl:25-35): window from one node before the element to one after.
+template <class T, int D, int P1, class Read>
+__device__ __forceinline__ T wide(const Tables<T>& tb, const int I[kMaxD], int d, Read read) {
+  const int i = I[d] % P1;
+  const int start = I[d] - i - 1;
+  T acc = 0;
+#pragma unroll
+  for (int k = 0; k < P1 + 2; ++k) {
+    const T w = tb.W[i][k];
+    // rows other than the face rows have zeros in the outer columns; skip them
+    if ((k == 0 && i != 0) || (k == P1 + 1 && i != P1 - 1)) continue;
+    acc += w * read(start + k);
+  }
+  return acc;
+}
+
+// ---------------------------------------------------------------------------
+// Gradient pass (flat): grads[b][d][l] = (W u)_d / h.
+// ---------------------------------------------------------------------------
+template <class T, int D, int P1>
+__global__ void k_gradient(Shape sh, const Tables<T>* __restrict__ tbp,
+                           const double* __restrict__ spacing, const T* __restrict__ y,
+                           const T* __restrict__ ug, T* __restrict__ grads) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= sh.nb * sh.nn) return;
+  const int64_t b = g / sh.nn, l = g % sh.nn;
+  const Tables<T>& tb = *tbp;
+  const int N = sh.N;
+  int I[kMaxD] = {0, 0, 0};
+  unravel<D>(l, N, I);
+  const T h = (T)spacing[b];
+  const T* u = y + b * 2 * sh.nn;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    const int64_t stride = d == 0 ? 1 : (d == 1 ? N : (int64_t)N * N);
+    const int64_t base = l - (int64_t)I[d] * stride;
+    const int64_t tl = tlin<D>(I, d, N);
+    auto read = [&](int o) -> T {
+      if (o < 0) return ug[((b * D + d) * 2 + 0) * sh.Nt + tl];
+      if (o >= N) return ug[((b * D + d) * 2 + 1) * sh.Nt + tl];
+      return u[base + o * stride];
+    };
+    grads[(b * D + d) * sh.nn + l] = wide<T, D, P1>(tb, I, d, read) / h;
+  }
+}
+
+// Cartesian gradient (full metric): V_i = sum_e S[e][i] D_e u / sqrt(g), or on a
+// flat block S[i][i] D_i u / J from the block's first node (evolution.jl:144-164).
+template <class T, int D, int P1>
+__global__ void k_gradient_cart(Shape sh, const Tables<T>* __restrict__ tbp,
+                                const double* __restrict__ spacing, const T* __restrict__ y,
+                                const T* __restrict__ ug, const T* __restrict__ coef,
+                                const uint8_t* __restrict__ flat, T* __restrict__ cart) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= sh.nb * sh.nn) return;
+  const int64_t b = g / sh.nn, l = g % sh.nn;
+  const Tables<T>& tb = *tbp;
+  const int N = sh.N;
+  int I[kMaxD] = {0, 0, 0};
+  unravel<D>(l, N, I);
+  const T h = (T)spacing[b];
+  const T* u = y + b * 2 * sh.nn;
+  T Du[kMaxD];
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    const int64_t stride = d == 0 ? 1 : (d == 1 ? N : (int64_t)N * N);
+    const int64_t base = l - (int64_t)I[d] * stride;
+    const int64_t tl = tlin<D>(I, d, N);
+    auto read = [&](int o) -> T {
+      if (o < 0) return ug[((b * D + d) * 2 + 0) * sh.Nt + tl];
+      if (o >= N) return ug[((b * D + d) * 2 + 1) * sh.Nt + tl];
+      return u[base + o * stride];
+    };
+    Du[d] = wide<T, D, P1>(tb, I, d, read) / h;
+  }
+  const int nv = 1 + D * D;
+  const T* c = coef + b * nv * sh.nn;
+  if (flat[b]) {
+    const T J = c[0];
+#pragma unroll
+    for (int i = 0; i < D; ++i) cart[(b * D + i) * sh.nn + l] = c[(1 + i * D + i) * sh.nn] * Du[i] / J;
+  } else {
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      T s = c[(1 + 0 * D + i) * sh.nn + l] * Du[0];
+#pragma unroll
+      for (int e = 1; e < D; ++e) s += c[(1 + e * D + i) * sh.nn + l] * Du[e];
+      cart[(b * D + i) * sh.nn + l] = s / c[l];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Stage kernels: the Laplacian and the RK4 combination in one pass.
+// k = (v_s, lap); by stage:
+//   MODE 0 (plain RHS):  du = k
+//   MODE 1 (stage 1):    acc = k;          yo = yn + a k
+//   MODE 2 (stages 2,3): acc = acc + 2 k;  yo = yn + a k
+//   MODE 3 (stage 4):    yn = yn + a (acc + k)          (a = dt/6)
+//   MODE 4 (stage 4, compensated): the same as a Kahan sum into (yn, lo)
+// ---------------------------------------------------------------------------
+template <class T>
+struct StageIO {
+  const T* ys;   // stage input state (its v is k_u)
+  const T* yn;   // step start
+  T* acc;
+  T* yo;         // output
+  T* lo;         // compensation of yn (MODE 4)
+  T a;
+};
+
+template <class T, int MODE>
+__device__ __forceinline__ void stage_store(const StageIO<T>& io, int64_t iu, int64_t iv, T ku,
+                                            T kv) {
+  if (MODE == 0) {
+    io.yo[iu] = ku;
+    io.yo[iv] = kv;
+  } else if (MODE == 1) {
+    io.acc[iu] = ku;
+    io.acc[iv] = kv;
+    io.yo[iu] = io.yn[iu] + io.a * ku;
+    io.yo[iv] = io.yn[iv] + io.a * kv;
+  } else if (MODE == 2) {
+    io.acc[iu] = io.acc[iu] + T(2) * ku;
+    io.acc[iv] = io.acc[iv] + T(2) * kv;
+    io.yo[iu] = io.yn[iu] + io.a * ku;
+    io.yo[iv] = io.yn[iv] + io.a * kv;
+  } else if (MODE == 3) {
+    io.yo[iu] = io.yn[iu] + io.a * (io.acc[iu] + ku);
+    io.yo[iv] = io.yn[iv] + io.a * (io.acc[iv] + kv);
+  } else {
+    // Kahan: (hi, c) += incr, c holds minus the lost low part
+    const T incr_u = io.a * (io.acc[iu] + ku), incr_v = io.a * (io.acc[iv] + kv);
+    T yu = incr_u - io.lo[iu], hu = io.yn[iu], tu = hu + yu;
+    io.lo[iu] = (tu - hu) - yu;
+    io.yo[iu] = tu;
+    T yv = incr_v - io.lo[iv], hv = io.yn[iv], tv = hv + yv;
+    io.lo[iv] = (tv - hv) - yv;
+    io.yo[iv] = tv;
+  }
+}
+
+template <class T, int D, int P1, int MODE>
+__global__ void k_stage_flat(Shape sh, const Tables<T>* __restrict__ tbp,
+                             const double* __restrict__ spacing, const T* __restrict__ grads,
+                             const T* __restrict__ gg, StageIO<T> io) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= sh.nb * sh.nn) return;
+  const int64_t b = g / sh.nn, l = g % sh.nn;
+  const Tables<T>& tb = *tbp;
+  const int N = sh.N;
+  int I[kMaxD] = {0, 0, 0};
+  unravel<D>(l, N, I);
+  T acc = 0;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    const int64_t stride = d == 0 ? 1 : (d == 1 ? N : (int64_t)N * N);
+    const int64_t base = l - (int64_t)I[d] * stride;
+    const int64_t tl = tlin<D>(I, d, N);
+    const T* gd = grads + (b * D + d) * sh.nn;
+    auto read = [&](int o) -> T {
+      if (o < 0) return gg[((b * D + d) * 2 + 0) * sh.Nt + tl];
+      if (o >= N) return gg[((b * D + d) * 2 + 1) * sh.Nt + tl];
+      return gd[base + o * stride];
+    };
+    acc += wide<T, D, P1>(tb, I, d, read);
+  }
+  const int64_t iu = b * 2 * sh.nn + l, iv = iu + sh.nn;
+  const T ku = io.ys[iv];
+  const T kv = acc / (T)spacing[b];
+  stage_store<T, MODE>(io, iu, iv, ku, kv);
+}
+
+// Laplacian on a full metric: sqrt(g) dv/dt = sum_d D_d (S[d] . V), the frame at
+// a ghost being the adjacent face node's (evolution.jl:169-220).
+template <class T, int D, int P1, int MODE>
+__global__ void k_stage_cart(Shape sh, const Tables<T>* __restrict__ tbp,
+                             const double* __restrict__ spacing, const T* __restrict__ cart,
+                             const T* __restrict__ cg, const T* __restrict__ coef,
+                             const uint8_t* __restrict__ flat, StageIO<T> io) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= sh.nb * sh.nn) return;
+  const int64_t b = g / sh.nn, l = g % sh.nn;
+  const Tables<T>& tb = *tbp;
+  const int N = sh.N;
+  int I[kMaxD] = {0, 0, 0};
+  unravel<D>(l, N, I);
+  const int nv = 1 + D * D;
+  const T* c = coef + b * nv * sh.nn;
+  const bool isflat = flat[b] != 0;
+  T acc = 0;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    const int64_t stride = d == 0 ? 1 : (d == 1 ? N : (int64_t)N * N);
+    const int64_t base = l - (int64_t)I[d] * stride;
+    const int64_t tl = tlin<D>(I, d, N);
+    const T Sdd = c[(1 + d * D + d) * sh.nn];
+    auto read = [&](int o) -> T {
+      const int oc = o < 0 ? 0 : (o >= N ? N - 1 : o);
+      const int64_t lf = base + (int64_t)oc * stride;  // frame node (clamped)
+      auto V = [&](int j) -> T {
+        if (o < 0) return cg[((((b * D + d) * 2 + 0) * D) + j) * sh.Nt + tl];
+        if (o >= N) return cg[((((b * D + d) * 2 + 1) * D) + j) * sh.Nt + tl];
+        return cart[(b * D + j) * sh.nn + base + o * stride];
+      };
+      if (isflat) return Sdd * V(d);
+      T s = c[(1 + d * D + 0) * sh.nn + lf] * V(0);
+#pragma unroll
+      for (int j = 1; j < D; ++j) s += c[(1 + d * D + j) * sh.nn + lf] * V(j);
+      return s;
+    };
+    acc += wide<T, D, P1>(tb, I, d, read);
+  }
+  const int64_t iu = b * 2 * sh.nn + l, iv = iu + sh.nn;
+  const T ku = io.ys[iv];
+  const T kv = acc / (T)spacing[b] / c[l];
+  stage_store<T, MODE>(io, iu, iv, ku, kv);
+}
+
+// ---------------------------------------------------------------------------
+// Truncation: per element, for d = 1..D and every line along d,
+// c = dual . a, a -= mode c (evolution.jl:238-264).  v of the state.
+// ---------------------------------------------------------------------------
+template <class T, int D, int P1>
+__global__ void k_truncate(Shape sh, int E, const Tables<T>* __restrict__ tbp,
+                           const int32_t* __restrict__ tslot, T* __restrict__ y) {
+  int64_t ne = E;
+  if (D >= 2) ne *= E;
+  if (D >= 3) ne *= E;
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= sh.nb * ne) return;
+  const int64_t b = g / ne, e = g % ne;
+  if (tslot && tslot[b] >= 0) return;  // per-element tables (k_truncate_elem)
+  const Tables<T>& tb = *tbp;
+  const int N = sh.N;
+  int J[kMaxD] = {(int)(e % E), 0, 0};
+  if (D >= 2) J[1] = (int)((e / E) % E);
+  if (D >= 3) J[2] = (int)(e / ((int64_t)E * E));
+  T* a = y + b * 2 * sh.nn + sh.nn;
+  int64_t base = 0, strides[kMaxD] = {1, N, (int64_t)N * N};
+#pragma unroll
+  for (int d = 0; d < D; ++d) base += (int64_t)J[d] * P1 * strides[d];
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    // tangential strides
+    int64_t s1 = 0, s2 = 0;
+    int n1 = 1, n2 = 1;
+    {
+      int k = 0;
+#pragma unroll
+      for (int q = 0; q < D; ++q)
+        if (q != d) {
+          if (k == 0) {
+            s1 = strides[q];
+            n1 = P1;
+          } else {
+            s2 = strides[q];
+            n2 = P1;
+          }
+          ++k;
+        }
+    }
+    for (int k2 = 0; k2 < n2; ++k2)
+      for (int k1 = 0; k1 < n1; ++k1) {
+        T* line = a + base + k1 * s1 + k2 * s2;
+        T vals[P1];
+        T c = 0;
+#pragma unroll
+        for (int j = 0; j < P1; ++j) {
+          vals[j] = line[j * strides[d]];
+          c += tb.dual[j] * vals[j];
+        }
+#pragma unroll
+        for (int i = 0; i < P1; ++i) line[i * strides[d]] = vals[i] - tb.mode[i] * c;
+      }
+  }
+}
+
+// Per-element sqrt(g)-orthogonal projection (evolution.jl:384-425):
+// c = Psi^T (W_e a); U^T y = c; U z = y; a = Psi z.  D = 3.
+template <class T, int P1>
+__global__ void k_truncate_elem(Shape sh, int E, const int32_t* __restrict__ tslot,
+                                const T* __restrict__ tmodes, const T* __restrict__ tweights,
+                                const T* __restrict__ tchol, T* __restrict__ y) {
+  constexpr int P = P1 - 1;
+  constexpr int n = P1 * P1 * P1, m = P * P * P;
+  const int64_t ne = (int64_t)E * E * E;
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= sh.nb * ne) return;
+  const int64_t b = g / ne, e = g % ne;
+  const int s = tslot[b];
+  if (s < 0) return;
+  const int N = sh.N;
+  const int J0 = (int)(e % E), J1 = (int)((e / E) % E), J2 = (int)(e / ((int64_t)E * E));
+  T* a = y + b * 2 * sh.nn + sh.nn;
+  const int64_t base = (int64_t)J0 * P1 + (int64_t)N * (J1 * P1) + (int64_t)N * N * (J2 * P1);
+  const T* w = tweights + ((int64_t)s * ne + e) * n;
+  const T* U = tchol + ((int64_t)s * ne + e) * m * m;
+  auto at = [&](int k) -> int64_t {
+    const int k0 = k % P1, k1 = (k / P1) % P1, k2 = k / (P1 * P1);
+    return base + k0 + (int64_t)N * k1 + (int64_t)N * N * k2;
+  };
+  T c[m];
+#pragma unroll
+  for (int q = 0; q < m; ++q) c[q] = 0;
+  for (int k = 0; k < n; ++k) {
+    const T wa = w[k] * a[at(k)];
+#pragma unroll
+    for (int q = 0; q < m; ++q) c[q] += tmodes[k * m + q] * wa;
+  }
+#pragma unroll
+  for (int q = 0; q < m; ++q) {
+    T acc = c[q];
+#pragma unroll
+    for (int r = 0; r < q; ++r) acc -= U[r + m * q] * c[r];
+    c[q] = acc / U[q + m * q];
+  }
+#pragma unroll
+  for (int q = m - 1; q >= 0; --q) {
+    T acc = c[q];
+#pragma unroll
+    for (int r = q + 1; r < m; ++r) acc -= U[q + m * r] * c[r];
+    c[q] = acc / U[q + m * q];
+  }
+  for (int k = 0; k < n; ++k) {
+    T z = 0;
+#pragma unroll
+    for (int q = 0; q < m; ++q) z += tmodes[k * m + q] * c[q];
+    a[at(k)] = z;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Reductions (FP64).  One workgroup per block writes its partial sums; one
+// workgroup then adds the partials in a fixed order.
+// ---------------------------------------------------------------------------
+constexpr int kRedThreads = 256;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
+  return v;
+}
+
+// Sum (and max) over the workgroup; thread 0 gets the result.
+template <int NV>
+__device__ void group_reduce(double* v, bool maxlast) {
+  __shared__ double part[NV][kRedThreads / 64];
+  const int lane = threadIdx.x % 64, wid = threadIdx.x / 64;
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    double x = (maxlast && q == NV - 1) ? wave_max(v[q]) : wave_sum(v[q]);
+    if (lane == 0) part[q][wid] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      double x = part[q][0];
+      for (int w = 1; w < kRedThreads / 64; ++w)
+        x = (maxlast && q == NV - 1) ? fmax(x, part[q][w]) : x + part[q][w];
+      v[q] = x;
+    }
+  }
+}
+
+template <int D>
+__device__ __forceinline__ double node_weight(const int I[kMaxD], double h, int P1,
+                                              const double* wq) {
+  double w = 1;
+#pragma unroll
+  for (int d = 0; d < D; ++d) w *= h * P1 / 2 * wq[I[d] % P1];
+  return w;
+}
+
+// Energy partials per block: (kinetic, potential).  flat: sum w v^2 and
+// sum_d w g_d^2; full metric: weights times sqrt(g), |V|^2 (evolution.jl:1472-1489).
+template <class T, int D, int P1, bool FULL>
+__global__ void k_energy_part(Shape sh, const Tables<T>* __restrict__ tbp,
+                              const double* __restrict__ spacing, const T* __restrict__ y,
+                              const T* __restrict__ grads, const T* __restrict__ coef,
+                              double* __restrict__ part) {
+  const int64_t b = blockIdx.x;
+  const double h = spacing[b];
+  double v[2] = {0, 0};
+  for (int64_t l = threadIdx.x; l < sh.nn; l += blockDim.x) {
+    int I[kMaxD] = {0, 0, 0};
+    unravel<D>(l, sh.N, I);
+    double w = node_weight<D>(I, h, P1, tbp->wq);
+    const double vv = (double)y[b * 2 * sh.nn + sh.nn + l];
+    double pot = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const double g = (double)grads[(b * D + d) * sh.nn + l];
+      pot += g * g;
+    }
+    if (FULL) w *= (double)coef[b * (1 + D * D) * sh.nn + l];
+    v[0] += w * (vv * vv);
+    v[1] += w * pot;
+  }
+  group_reduce<2>(v, false);
+  if (threadIdx.x == 0) {
+    part[2 * b] = v[0];
+    part[2 * b + 1] = v[1];
+  }
+}
+
+// Norm partials per block: (sum w sum_vars |x|^p, sum w nvars, max |x|).
+template <class T, int D, int P1, bool FULL>
+__global__ void k_norm_part(Shape sh, const Tables<T>* __restrict__ tbp,
+                            const double* __restrict__ spacing, const T* __restrict__ a,
+                            const T* __restrict__ alo, const T* __restrict__ bb,
+                            const T* __restrict__ blo, const T* __restrict__ coef, int mask,
+                            int pnorm, double* __restrict__ part) {
+  const int64_t b = blockIdx.x;
+  const double h = spacing[b];
+  double v[3] = {0, 0, 0};
+  for (int64_t l = threadIdx.x; l < sh.nn; l += blockDim.x) {
+    int I[kMaxD] = {0, 0, 0};
+    unravel<D>(l, sh.N, I);
+    double w = node_weight<D>(I, h, P1, tbp->wq);
+    if (FULL) w *= (double)coef[b * (1 + D * D) * sh.nn + l];
+    double s = 0;
+    int nv = 0;
+    for (int q = 0; q < 2; ++q) {
+      if (!(mask & (1 << q))) continue;
+      const int64_t i = b * 2 * sh.nn + q * sh.nn + l;
+      double x = (double)a[i];
+      if (alo) x -= (double)alo[i];
+      if (bb) {
+        double z = (double)bb[i];
+        if (blo) z -= (double)blo[i];
+        x -= z;
+      }
+      x = fabs(x);
+      s += pnorm == 2 ? x * x : pow(x, (double)pnorm);
+      v[2] = fmax(v[2], x);
+      ++nv;
+    }
+    v[0] += w * s;
+    v[1] += w * nv;
+  }
+  group_reduce<3>(v, true);
+  if (threadIdx.x == 0) {
+    part[3 * b] = v[0];
+    part[3 * b + 1] = v[1];
+    part[3 * b + 2] = v[2];
+  }
+}
+
+// Fixed-order sum of per-block partials: out[q] = sum_b part[NV b + q]
+// (max for q == NV-1 when maxlast).
+template <int NV>
+__global__ void k_final(int64_t nb, const double* __restrict__ part, bool maxlast,
+                        double* __restrict__ out) {
+  double v[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) v[q] = 0;
+  for (int64_t b = threadIdx.x; b < nb; b += blockDim.x)
+#pragma unroll
+    for (int q = 0; q < NV; ++q)
+      v[q] = (maxlast && q == NV - 1) ? fmax(v[q], part[NV * b + q]) : v[q] + part[NV * b + q];
+  group_reduce<NV>(v, maxlast);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int q = 0; q < NV; ++q) out[q] = v[q];
+}
+
+// y (double, host layout) <-> device precision
+template <class T>
+__global__ void k_from_double(int64_t n, const double* __restrict__ src, T* __restrict__ hi,
+                              T* __restrict__ lo) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double x = src[i];
+  const T h = (T)x;
+  hi[i] = h;
+  if (lo) lo[i] = (T)((double)h - x);  // minus the low part, as the Kahan sum keeps it
+}
+
+template <class T>
+__global__ void k_to_double(int64_t n, const T* __restrict__ hi, const T* __restrict__ lo,
+                            double* __restrict__ dst) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double x = (double)hi[i];
+  if (lo) x -= (double)lo[i];
+  dst[i] = x;
+}
+
+}  // namespace
+
+// ===========================================================================
+// Host side.
+// ===========================================================================
+struct twoldg_problem {
+  int D = 0, P = 0, E = 0, N = 0, P1 = 0;
+  int precision = TWOLDG_F64;
+  bool truncate = true;
+  bool full = false;
+  Shape sh{};
+  int64_t state_len = 0;
+  int64_t nrows = 0;
+  size_t elt = 8;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // device buffers (void*: element type follows precision)
+  void* slot[3] = {nullptr, nullptr, nullptr};
+  void* lo = nullptr;       // compensation of the state (F32C)
+  void* ys = nullptr;       // stage state
+  void* acc = nullptr;      // RK accumulator
+  void* ug = nullptr;       // u ghosts (rows)
+  void* gg = nullptr;       // gradient ghosts (rows, or rows x D on a full metric)
+  void* grads = nullptr;    // (nb, D, nn)
+  void* tables = nullptr;   // Tables<T>
+  void* coef = nullptr;     // (nb, 1+D*D, nn) in T
+  uint8_t* flat = nullptr;
+  int32_t* tslot = nullptr;
+  void* tmodes = nullptr;
+  void* tweights = nullptr;
+  void* tchol = nullptr;
+  double* spacing = nullptr;
+  int32_t* rowptr = nullptr;
+  int32_t* cb = nullptr;
+  int32_t* cl = nullptr;
+  void* cval_u = nullptr;   // coefficients for u (mirror +1)
+  void* cval_g = nullptr;   // coefficients for flat gradients (mirror -1)
+  int64_t ndir = 0;
+  int64_t* dir_rows = nullptr;
+  double* dir_x = nullptr;
+  double wave_k[3] = {0, 0, 0}, wave_w = 0, wave_phi = 0;
+  double* part = nullptr;   // per-block reduction partials (3 nb)
+  double* red = nullptr;    // final reduction (4)
+  double hmin = 0, hspeed = 0;
+  bool has_tables = false;
+  int64_t device_bytes = 0;
+  bool uploaded[3] = {false, false, false};
+};
+
+namespace {
+
+template <class T>
+int dalloc(twoldg_problem* p, T** ptr, size_t count) {
+  *ptr = nullptr;
+  if (count == 0) return 0;
+  hipError_t e = hipMalloc((void**)ptr, count * sizeof(T));
+  if (e != hipSuccess)
+    return fail(TWOLDG_ENOMEM, std::string("hipMalloc of ") + std::to_string(count * sizeof(T)) +
+                                   " bytes: " + hipGetErrorString(e));
+  p->device_bytes += (int64_t)(count * sizeof(T));
+  return 0;
+}
+
+int dalloc_bytes(twoldg_problem* p, void** ptr, size_t bytes) {
+  *ptr = nullptr;
+  if (bytes == 0) return 0;
+  hipError_t e = hipMalloc(ptr, bytes);
+  if (e != hipSuccess)
+    return fail(TWOLDG_ENOMEM, std::string("hipMalloc of ") + std::to_string(bytes) +
+                                   " bytes: " + hipGetErrorString(e));
+  p->device_bytes += (int64_t)bytes;
+  return 0;
+}
+
+void dfree(void* ptr) {
+  if (ptr) (void)hipFree(ptr);
+}
+
+inline unsigned grid_for(int64_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
+
+// The exchange operator as CSR rows over (block, dim, side, tangential node).
+struct Csr {
+  std::vector<int32_t> rowptr, cb, cl;
+  std::vector<double> vu, vg;
+  std::vector<int64_t> dir_rows;
+};
+
+int64_t ipow(int64_t a, int b) {
+  int64_t r = 1;
+  for (int i = 0; i < b; ++i) r *= a;
+  return r;
+}
+
+// Build the exchange rows from the face descriptors.  Element-local operators
+// (TreeAMR Operators(family=ElementLocal), CODE.md "The coupling"): injection
+// normal to the face; tangentially I (coarse polynomial at fine nodes) for a
+// fine ghost and R = M_c^-1 I^T M_f for a coarse ghost, tensor over the D-1
+// tangential dimensions.
+int build_csr(const twoldg_desc* d, Csr& csr) {
+  const int D = d->D, P1 = d->P + 1, N = d->E * P1, E = d->E;
+  const int64_t Nt = ipow(N, D - 1), nn = ipow(N, D), nb = d->nblocks;
+  const twoldg::ElementTables& et = twoldg::kElementTables[d->P - 1];
+  const int64_t nrows = nb * 2 * D * Nt;
+  csr.rowptr.assign(nrows + 1, 0);
+  csr.cb.reserve(nrows);
+  csr.cl.reserve(nrows);
+  csr.vu.reserve(nrows);
+  csr.vg.reserve(nrows);
+  bool bad = false;
+  auto push = [&](int64_t blk, const int I[3], double vu, double vg) {
+    int64_t l = I[0] + (int64_t)N * (D >= 2 ? I[1] : 0) + (int64_t)N * N * (D >= 3 ? I[2] : 0);
+    for (int e = 0; e < D; ++e)
+      if (I[e] < 0 || I[e] >= N) bad = true;
+    if (blk < 0 || blk >= nb || l < 0 || l >= nn) bad = true;
+    csr.cb.push_back((int32_t)blk);
+    csr.cl.push_back((int32_t)l);
+    csr.vu.push_back(vu);
+    csr.vg.push_back(vg);
+  };
+  int64_t r = 0;
+  for (int64_t b = 0; b < nb; ++b)
+    for (int dd = 0; dd < D; ++dd)
+      for (int s = 0; s < 2; ++s) {
+        const twoldg_face& F = d->faces[(b * D + dd) * 2 + s];
+        int tdim[2] = {0, 0}, nt = 0;
+        for (int e = 0; e < D; ++e)
+          if (e != dd) tdim[nt++] = e;
+        for (int64_t t = 0; t < Nt; ++t, ++r) {
+          int T[2] = {(int)(t % N), (int)(nt >= 2 ? t / N : 0)};
+          int I[3] = {0, 0, 0};
+          switch (F.kind) {
+            case TWOLDG_FACE_SAME: {
+              if (F.nb[0] < 0 || F.nb[0] >= nb) return fail(TWOLDG_EINVAL, "face neighbour out of range");
+              I[dd] = s ? 0 : N - 1;
+              for (int k = 0; k < nt; ++k) I[tdim[k]] = T[k];
+              push(F.nb[0], I, 1.0, 1.0);
+            } break;
+            case TWOLDG_FACE_MIRROR: {
+              I[dd] = s ? N - 1 : 0;
+              for (int k = 0; k < nt; ++k) I[tdim[k]] = T[k];
+              push(b, I, 1.0, -1.0);
+            } break;
+            case TWOLDG_FACE_GLUE: {
+              if (F.nb[0] < 0 || F.nb[0] >= nb) return fail(TWOLDG_EINVAL, "glue neighbour out of range");
+              const int nd = F.nface / 2, ns = F.nface % 2;
+              int ntd[2] = {0, 0}, k2 = 0;
+              for (int e = 0; e < D; ++e)
+                if (e != nd) ntd[k2++] = e;
+              I[nd] = ns ? N - 1 : 0;
+              for (int j = 0; j < nt; ++j) {
+                const int m = F.map[j];
+                const int k = (m > 0 ? m : -m) - 1;
+                if (k < 0 || k >= nt) return fail(TWOLDG_EINVAL, "glue map entry out of range");
+                I[ntd[j]] = m > 0 ? T[k] : N - 1 - T[k];
+              }
+              push(F.nb[0], I, 1.0, 1.0);
+            } break;
+            case TWOLDG_FACE_COARSER: {
+              if (F.nb[0] < 0 || F.nb[0] >= nb) return fail(TWOLDG_EINVAL, "coarse neighbour out of range");
+              int qc[2] = {0, 0}, fn[2] = {0, 0};
+              for (int k = 0; k < nt; ++k) {
+                const int qf = T[k] / P1, jj = T[k] % P1, f = qf % 2;
+                qc[k] = F.off[tdim[k]] * (E / 2) + qf / 2;
+                fn[k] = f * P1 + jj;  // fine node index within the coarse element's pair
+              }
+              const int64_t cnt = ipow(P1, nt);
+              for (int64_t a = 0; a < cnt; ++a) {
+                double w = 1;
+                int64_t rem = a;
+                I[dd] = s ? 0 : N - 1;
+                for (int k = 0; k < nt; ++k) {
+                  const int ic = (int)(rem % P1);
+                  rem /= P1;
+                  I[tdim[k]] = qc[k] * P1 + ic;
+                  w *= et.interp[fn[k]][ic];
+                }
+                push(F.nb[0], I, w, w);
+              }
+            } break;
+            case TWOLDG_FACE_FINER: {
+              int qc[2] = {0, 0}, ic[2] = {0, 0}, o[2] = {0, 0};
+              int ch = 0;
+              for (int k = 0; k < nt; ++k) {
+                qc[k] = T[k] / P1;
+                ic[k] = T[k] % P1;
+                o[k] = (qc[k] * P1 >= N / 2) ? 1 : 0;
+                ch |= o[k] << k;
+              }
+              const int64_t fb = F.nb[ch];
+              if (fb < 0 || fb >= nb) return fail(TWOLDG_EINVAL, "fine neighbour out of range");
+              const int64_t cnt = ipow(2 * P1, nt);
+              for (int64_t a = 0; a < cnt; ++a) {
+                double w = 1;
+                int64_t rem = a;
+                I[dd] = s ? 0 : N - 1;
+                for (int k = 0; k < nt; ++k) {
+                  const int fj = (int)(rem % (2 * P1));
+                  rem /= 2 * P1;
+                  const int f = fj / P1, jj = fj % P1;
+                  const int qfine = 2 * (qc[k] - o[k] * (E / 2)) + f;
+                  I[tdim[k]] = qfine * P1 + jj;
+                  w *= et.restrict_[ic[k]][fj];
+                }
+                push(fb, I, w, w);
+              }
+            } break;
+            case TWOLDG_FACE_DIRICHLET:
+              csr.dir_rows.push_back(r);
+              break;
+            default:
+              return fail(TWOLDG_EINVAL, "block " + std::to_string(b) + " face (" +
+                                             std::to_string(dd) + "," + std::to_string(s) +
+                                             ") has no valid kind");
+          }
+          if (bad) return fail(TWOLDG_EINVAL, "an exchange source lies outside its block");
+          if (csr.cb.size() > (size_t)INT32_MAX) return fail(TWOLDG_EINVAL, "exchange too large");
+          csr.rowptr[r + 1] = (int32_t)csr.cb.size();
+        }
+      }
+  return 0;
+}
+
+template <class T>
+int upload_as(twoldg_problem* p, void** dst, const double* src, size_t n) {
+  std::vector<T> tmp(n);
+  for (size_t i = 0; i < n; ++i) tmp[i] = (T)src[i];
+  int rc = dalloc_bytes(p, dst, n * sizeof(T));
+  if (rc) return rc;
+  HIP_TRY(hipMemcpy(*dst, tmp.data(), n * sizeof(T), hipMemcpyHostToDevice));
+  return 0;
+}
+
+template <class T>
+int upload_tables(twoldg_problem* p) {
+  const twoldg::ElementTables& et = twoldg::kElementTables[p->P - 1];
+  Tables<T> tb;
+  std::memset(&tb, 0, sizeof tb);
+  for (int i = 0; i < p->P1; ++i) {
+    for (int j = 0; j < p->P1 + 2; ++j) tb.W[i][j] = (T)et.stencil[i][j];
+    tb.mode[i] = (T)et.topmode[i];
+    tb.dual[i] = (T)et.topdual[i];
+    tb.wq[i] = et.weights[i];
+  }
+  int rc = dalloc_bytes(p, &p->tables, sizeof tb);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpy(p->tables, &tb, sizeof tb, hipMemcpyHostToDevice));
+  return 0;
+}
+
+// ---- launch helpers, dispatched on (D, P1) at run time ----------------------
+template <class T, int D, int P1>
+struct Launch {
+  static void exchange_u(twoldg_problem* p, const T* y, double t) {
+    if (p->nrows == 0) return;
+    hipLaunchKernelGGL((k_exchange<T, 0, D>), dim3(grid_for(p->nrows, 256)), dim3(256), 0,
+                       p->stream, p->nrows, p->sh, p->rowptr, p->cb, p->cl, (const T*)p->cval_u,
+                       y, (T*)p->ug);
+    if (p->ndir)
+      hipLaunchKernelGGL((k_dirichlet<T, 0, D>), dim3(grid_for(p->ndir, 256)), dim3(256), 0,
+                         p->stream, p->ndir, p->sh, p->dir_rows, p->dir_x, p->wave_k[0],
+                         p->wave_k[1], p->wave_k[2], p->wave_w, p->wave_phi, t, (T*)p->ug);
+  }
+  static void gradient(twoldg_problem* p, const T* y, double t) {
+    exchange_u(p, y, t);
+    const int64_t n = p->sh.nb * p->sh.nn;
+    if (!p->full) {
+      hipLaunchKernelGGL((k_gradient<T, D, P1>), dim3(grid_for(n, 256)), dim3(256), 0, p->stream,
+                         p->sh, (const Tables<T>*)p->tables, p->spacing, y, (const T*)p->ug,
+                         (T*)p->grads);
+      hipLaunchKernelGGL((k_exchange<T, 1, D>), dim3(grid_for(p->nrows, 256)), dim3(256), 0,
+                         p->stream, p->nrows, p->sh, p->rowptr, p->cb, p->cl,
+                         (const T*)p->cval_g, (const T*)p->grads, (T*)p->gg);
+    } else {
+      hipLaunchKernelGGL((k_gradient_cart<T, D, P1>), dim3(grid_for(n, 256)), dim3(256), 0,
+                         p->stream, p->sh, (const Tables<T>*)p->tables, p->spacing, y,
+                         (const T*)p->ug, (const T*)p->coef, p->flat, (T*)p->grads);
+      hipLaunchKernelGGL((k_exchange<T, 2, D>), dim3(grid_for(p->nrows, 256)), dim3(256), 0,
+                         p->stream, p->nrows, p->sh, p->rowptr, p->cb, p->cl,
+                         (const T*)p->cval_u, (const T*)p->grads, (T*)p->gg);
+      if (p->ndir)
+        hipLaunchKernelGGL((k_dirichlet<T, 2, D>), dim3(grid_for(p->ndir, 256)), dim3(256), 0,
+                           p->stream, p->ndir, p->sh, p->dir_rows, p->dir_x, p->wave_k[0],
+                           p->wave_k[1], p->wave_k[2], p->wave_w, p->wave_phi, t, (T*)p->gg);
+    }
+  }
+  template <int MODE>
+  static void stage(twoldg_problem* p, const StageIO<T>& io) {
+    const int64_t n = p->sh.nb * p->sh.nn;
+    if (!p->full)
+      hipLaunchKernelGGL((k_stage_flat<T, D, P1, MODE>), dim3(grid_for(n, 256)), dim3(256), 0,
+                         p->stream, p->sh, (const Tables<T>*)p->tables, p->spacing,
+                         (const T*)p->grads, (const T*)p->gg, io);
+    else
+      hipLaunchKernelGGL((k_stage_cart<T, D, P1, MODE>), dim3(grid_for(n, 256)), dim3(256), 0,
+                         p->stream, p->sh, (const Tables<T>*)p->tables, p->spacing,
+                         (const T*)p->grads, (const T*)p->gg, (const T*)p->coef, p->flat, io);
+  }
+  static void truncate(twoldg_problem* p, T* y) {
+    if (!p->truncate) return;
+    const int64_t ne = ipow(p->E, D);
+    const int64_t n = p->sh.nb * ne;
+    hipLaunchKernelGGL((k_truncate<T, D, P1>), dim3(grid_for(n, 128)), dim3(128), 0, p->stream,
+                       p->sh, p->E, (const Tables<T>*)p->tables, p->tslot, y);
+    if (D == 3 && p->has_tables)
+      launch_elem(p, y, n);
+  }
+  static void launch_elem(twoldg_problem* p, T* y, int64_t n) {
+    if constexpr (D == 3 && P1 <= 5)
+      hipLaunchKernelGGL((k_truncate_elem<T, P1>), dim3(grid_for(n, 64)), dim3(64), 0, p->stream,
+                         p->sh, p->E, p->tslot, (const T*)p->tmodes, (const T*)p->tweights,
+                         (const T*)p->tchol, y);
+  }
+  static void energy(twoldg_problem* p, const T* y) {
+    if (!p->full)
+      hipLaunchKernelGGL((k_energy_part<T, D, P1, false>), dim3((unsigned)p->sh.nb),
+                         dim3(kRedThreads), 0, p->stream, p->sh, (const Tables<T>*)p->tables,
+                         p->spacing, y, (const T*)p->grads, (const T*)nullptr, p->part);
+    else
+      hipLaunchKernelGGL((k_energy_part<T, D, P1, true>), dim3((unsigned)p->sh.nb),
+                         dim3(kRedThreads), 0, p->stream, p->sh, (const Tables<T>*)p->tables,
+                         p->spacing, y, (const T*)p->grads, (const T*)p->coef, p->part);
+    hipLaunchKernelGGL((k_final<2>), dim3(1), dim3(kRedThreads), 0, p->stream, p->sh.nb, p->part,
+                       false, p->red);
+  }
+  static void norms(twoldg_problem* p, const T* a, const T* alo, const T* b, const T* blo,
+                    int mask, int pnorm) {
+    if (!p->full)
+      hipLaunchKernelGGL((k_norm_part<T, D, P1, false>), dim3((unsigned)p->sh.nb),
+                         dim3(kRedThreads), 0, p->stream, p->sh, (const Tables<T>*)p->tables,
+                         p->spacing, a, alo, b, blo, (const T*)nullptr, mask, pnorm, p->part);
+    else
+      hipLaunchKernelGGL((k_norm_part<T, D, P1, true>), dim3((unsigned)p->sh.nb),
+                         dim3(kRedThreads), 0, p->stream, p->sh, (const Tables<T>*)p->tables,
+                         p->spacing, a, alo, b, blo, (const T*)p->coef, mask, pnorm, p->part);
+    hipLaunchKernelGGL((k_final<3>), dim3(1), dim3(kRedThreads), 0, p->stream, p->sh.nb, p->part,
+                       true, p->red);
+  }
+};
+
+template <class T, int D, int P1>
+int rk4_steps(twoldg_problem* p, double t0, double dt, int64_t nsteps) {
+  using L = Launch<T, D, P1>;
+  T* yn = (T*)p->slot[TWOLDG_SLOT_STATE];
+  T* ys = (T*)p->ys;
+  T* acc = (T*)p->acc;
+  T* lo = (T*)p->lo;
+  for (int64_t s = 0; s < nsteps; ++s) {
+    const double t = t0 + (double)s * dt;
+    StageIO<T> io{yn, yn, acc, ys, lo, (T)(dt / 2)};
+    L::gradient(p, yn, t);
+    L::template stage<1>(p, io);
+    L::truncate(p, ys);
+    io.ys = ys;
+    L::gradient(p, ys, t + dt / 2);
+    L::template stage<2>(p, io);
+    L::truncate(p, ys);
+    io.a = (T)dt;
+    L::gradient(p, ys, t + dt / 2);
+    L::template stage<2>(p, io);
+    L::truncate(p, ys);
+    io.a = (T)(dt / 6);
+    io.yo = yn;
+    L::gradient(p, ys, t + dt);
+    if (p->precision == TWOLDG_F32C) {
+      L::template stage<4>(p, io);
+      L::truncate(p, yn);
+      L::truncate(p, lo);
+    } else {
+      L::template stage<3>(p, io);
+      L::truncate(p, yn);
+    }
+  }
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+template <class T, int D, int P1>
+int rhs_impl(twoldg_problem* p, double t) {
+  using L = Launch<T, D, P1>;
+  const T* y = (const T*)p->slot[TWOLDG_SLOT_STATE];
+  L::gradient(p, y, t);
+  StageIO<T> io{y, y, nullptr, (T*)p->slot[TWOLDG_SLOT_DU], nullptr, (T)0};
+  L::template stage<0>(p, io);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+template <class T, int D, int P1>
+int truncate_impl(twoldg_problem* p, int slot) {
+  using L = Launch<T, D, P1>;
+  L::truncate(p, (T*)p->slot[slot]);
+  if (slot == TWOLDG_SLOT_STATE && p->lo) L::truncate(p, (T*)p->lo);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+template <class T, int D, int P1>
+int energy_impl(twoldg_problem* p, int slot, double t, double* out) {
+  using L = Launch<T, D, P1>;
+  const T* y = (const T*)p->slot[slot];
+  L::gradient(p, y, t);
+  L::energy(p, y);
+  double h[2];
+  HIP_TRY(hipMemcpyAsync(h, p->red, 2 * sizeof(double), hipMemcpyDeviceToHost, p->stream));
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  *out = (h[0] + h[1]) / 2;
+  return 0;
+}
+
+template <class T, int D, int P1>
+int norms_impl(twoldg_problem* p, int a, int b, int mask, int pnorm, double* lp, double* linf) {
+  using L = Launch<T, D, P1>;
+  const T* alo = (a == TWOLDG_SLOT_STATE) ? (const T*)p->lo : nullptr;
+  const T* blo = (b == TWOLDG_SLOT_STATE) ? (const T*)p->lo : nullptr;
+  L::norms(p, (const T*)p->slot[a], alo, b >= 0 ? (const T*)p->slot[b] : nullptr, blo, mask,
+           pnorm);
+  double h[3];
+  HIP_TRY(hipMemcpyAsync(h, p->red, 3 * sizeof(double), hipMemcpyDeviceToHost, p->stream));
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  *lp = h[1] == 0 ? 0 : std::pow(h[0] / h[1], 1.0 / pnorm);
+  *linf = h[2];
+  return 0;
+}
+
+// Run-time dispatch on (precision, D, P1).
+
+#define TWOLDG_DISPATCH_FN(name, impl)                                                     \
+  template <class T, int D, int P1>                                                        \
+  struct name {                                                                            \
+    template <class... A>                                                                  \
+    static int run(twoldg_problem* p, A... args) {                                         \
+      return impl<T, D, P1>(p, args...);                                                   \
+    }                                                                                      \
+  };
+TWOLDG_DISPATCH_FN(RhsF, rhs_impl)
+TWOLDG_DISPATCH_FN(Rk4F, rk4_steps)
+TWOLDG_DISPATCH_FN(TruncF, truncate_impl)
+TWOLDG_DISPATCH_FN(EnergyF, energy_impl)
+TWOLDG_DISPATCH_FN(NormsF, norms_impl)
+
+template <template <class, int, int> class F, class T, int D, class... A>
+int dispatch_p(twoldg_problem* p, A... args) {
+  switch (p->P1) {
+    case 2: return F<T, D, 2>::run(p, args...);
+    case 3: return F<T, D, 3>::run(p, args...);
+    case 4: return F<T, D, 4>::run(p, args...);
+    case 5: return F<T, D, 5>::run(p, args...);
+    case 6: return F<T, D, 6>::run(p, args...);
+    case 7: return F<T, D, 7>::run(p, args...);
+    case 8: return F<T, D, 8>::run(p, args...);
+    case 9: return F<T, D, 9>::run(p, args...);
+  }
+  return fail(TWOLDG_EINVAL, "unsupported P");
+}
+
+}  // namespace

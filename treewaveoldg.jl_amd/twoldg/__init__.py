@@ -1,0 +1,23 @@
+"""twoldg: the MI355X-native field-evolution path of TreeWaveOLDG.jl.
+
+The host side mirrors the reference's Julia API (src/TreeWaveOLDG.jl:53-85);
+all arithmetic of the hot path runs in libtwoldg.so (HIP, gfx950) behind the C
+ABI of include/twoldg.h.
+"""
+from .element import Element, order
+from .forest import Forest, wave_forest, nested_cubes, refinement_regions, check_layout
+from .problem import (WaveProblem, wave_rhs, truncate_state, truncate_stage, initial_state,
+                      fill_at_nodes, wave_energy, wave_norm, quadrature_norm, oldg_timestep,
+                      wave_integrator, Integrator, exchange_plan)
+from .drivers import (wave_omega, wave_exact, reflect_omega, reflect_exact, gaussian_pulse,
+                      PlaneWave, sevenpatch_exact, wave_errors, reflect_errors, sevenpatch_errors)
+from . import _lib
+
+__all__ = [
+    "Element", "order", "Forest", "wave_forest", "nested_cubes", "refinement_regions",
+    "check_layout", "WaveProblem", "wave_rhs", "truncate_state", "truncate_stage",
+    "initial_state", "fill_at_nodes", "wave_energy", "wave_norm", "quadrature_norm",
+    "oldg_timestep", "wave_integrator", "Integrator", "exchange_plan", "wave_omega",
+    "wave_exact", "reflect_omega", "reflect_exact", "gaussian_pulse", "PlaneWave",
+    "sevenpatch_exact", "wave_errors", "reflect_errors", "sevenpatch_errors",
+]
