@@ -1,0 +1,177 @@
+"""Benchmark: OLDG node-updates/s on one MI355X, with the HBM roofline of the
+stage kernel and the oracle timed on the host cores beside it.
+
+Workload (default): the 3D periodic uniform forest, P = 4, E = 4, roots = 16
+(4096 blocks of 20^3 nodes, 320^3 = 32.8M nodes per field), standing mode
+u = sin(2 pi x) sin(2 pi y) sin(2 pi z), v = 0, classic RK4 with the truncation
+as stage and step limiter, dt = 0.5 h / (P+1).  A node-update is one node of
+one field advanced by one full RK4 step (4 right-hand sides, 4 truncations and
+the stage arithmetic), the reference's "ns per node and step".
+
+Timed region: K RK4 steps with the state resident in HBM, bracketed by device
+synchronizations.  Printed: one JSON line on stdout; progress on stderr.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision f64|f32c]
+                  [--accuracy-steps S] [--no-cpu]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "treewaveoldg.jl_amd"))
+
+PEAK_HBM_GBS = 8000.0
+# algorithmic values moved per node-update by the fused three-register RK4:
+# stage 1 reads y_n (2), writes acc (2) and y_2 (2); stages 2 and 3 read y_s,
+# y_n, acc (6) and write acc, y_next (4); stage 4 reads y_4, y_n, acc (6) and
+# writes y_{n+1} (2): 34 values.  The compensated form adds the low words of
+# u and v, read and written at stage 4 (4 more values).
+VALUES_PER_NODE_UPDATE = {"f64": 34, "f32c": 38}
+BYTES_PER_VALUE = {"f64": 8, "f32c": 4}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_problem(precision, D=3, P=4, E=4, roots=16):
+    import twoldg as T
+    forest = T.wave_forest(D, P, E, roots=roots)
+    prob = T.WaveProblem(forest, T.Element(P), precision=precision)
+    return T, forest, prob
+
+
+def gpu_run(args, precision):
+    T, forest, prob = build_problem(precision)
+    nodes = forest.nblocks * forest.N ** 3
+    log(f"[bench] {precision}: {forest.nblocks} blocks, {nodes} nodes per field, "
+        f"device bytes {prob.info()['device_bytes'] / 2**30:.2f} GiB")
+    u0 = T.initial_state(prob, T.wave_exact(3, 1.0, 1, 0.0))
+    dt = T.oldg_timestep(prob, 0.5)
+    total = args.warmup + args.steps
+    integ = T.wave_integrator(prob, u0, 0.0, total * dt, dt)
+    integ.step(args.warmup)
+    T._lib.check(T._lib.lib().twoldg_synchronize(prob._h))
+    t0 = time.perf_counter()
+    ms, kernel_ms, launches = integ.timed(args.steps)
+    T._lib.check(T._lib.lib().twoldg_synchronize(prob._h))
+    wall = time.perf_counter() - t0
+    value = nodes * args.steps / wall
+    log(f"[bench] {precision}: {args.steps} steps in {wall * 1e3:.1f} ms (events {ms:.1f} ms), "
+        f"stage kernels {kernel_ms:.1f} ms over {launches} launches -> {value:.3e} node-updates/s")
+    alg_bytes = VALUES_PER_NODE_UPDATE[precision] * BYTES_PER_VALUE[precision] * nodes * args.steps
+    achieved = alg_bytes / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else None
+    res = dict(value=value, ms_per_step=wall * 1e3 / args.steps, kernel_ms=kernel_ms,
+               launches=launches, achieved_gbs=achieved,
+               bytes_per_launch=alg_bytes / max(launches, 1), nodes=nodes, dt=dt)
+    if args.accuracy_steps:
+        res.update(accuracy(T, prob, args.accuracy_steps, dt))
+    prob.close()
+    return res
+
+
+def accuracy(T, prob, nsteps, dt):
+    """L2 error against the exact standing mode after nsteps steps (config 2's
+    200), and the discrete energy drift."""
+    u0 = T.initial_state(prob, T.wave_exact(3, 1.0, 1, 0.0))
+    e0 = T.wave_energy(prob, u0)
+    integ = T.wave_integrator(prob, u0, 0.0, nsteps * dt, dt)
+    integ.solve()
+    e1 = integ.energy()
+    exact = T.fill_at_nodes(T.wave_exact(3, 1.0, 1, nsteps * dt), prob)
+    prob.upload(exact, T._lib.SLOT_AUX)
+    from twoldg.drivers import _errors
+    l2, linf = _errors(prob, (1, 2))
+    log(f"[bench] {prob.precision}: {nsteps} steps, l2 error {l2:.4e}, linf {linf:.4e}, "
+        f"energy drift {(e1 - e0) / e0:.3e}")
+    return dict(l2=l2, linf=linf, energy_drift=(e1 - e0) / e0, accuracy_steps=nsteps)
+
+
+def cpu_baseline(seconds=15.0):
+    """The oracle (the CPU FP64 restatement of the same algorithm) on this host's
+    cores: RK4 steps of the same workload until ~`seconds` of CPU work."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    threads = O.lib().orc_threads()
+    pr = O.Problem.brick(3, 4, 4, roots=16)
+    y = pr.truncate(pr.fill_exact(0, 0.0, (1.0, 1)))
+    dt = pr.timestep(0.5)
+    nodes = pr.nb * pr.nn
+    t0 = time.perf_counter()
+    y = pr.rk4(y, 0.0, dt, 1)  # first step: first touch of every work array
+    first = time.perf_counter() - t0
+    steps = max(1, min(20, int(seconds / max(first, 1e-3))))
+    t0 = time.perf_counter()
+    y = pr.rk4(y, dt, dt, steps)
+    el = time.perf_counter() - t0
+    value = nodes * steps / el
+    log(f"[bench] cpu oracle: {threads} threads, {steps} steps in {el:.2f} s "
+        f"-> {value:.3e} node-updates/s ({el / steps / nodes * 1e9:.2f} ns per node and step)")
+    return dict(value=value, unit="node-updates/s", cores=threads, kind="port",
+                sample=f"{steps} RK4 steps of the same 32.8M-node forest after one warm-up "
+                       f"step, FP64 C++/OpenMP restatement (oracle), {threads} threads")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--precision", default="f64", choices=["f64", "f32c"])
+    ap.add_argument("--accuracy-steps", type=int, default=200)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for "
+              f"--gpus {args.gpus})", file=sys.stderr)
+        sys.exit(2)
+    main_res = gpu_run(args, args.precision)
+    other = "f32c" if args.precision == "f64" else "f64"
+    other_res = gpu_run(args, other)
+    cpu = None if args.no_cpu else cpu_baseline()
+    prec = args.precision
+    launch_ms = main_res["kernel_ms"] / max(main_res["launches"], 1)
+    roofline = None
+    if main_res["achieved_gbs"]:
+        roofline = dict(bound="hbm", achieved=round(main_res["achieved_gbs"], 1),
+                        peak=PEAK_HBM_GBS, unit="GB/s",
+                        frac=round(main_res["achieved_gbs"] / PEAK_HBM_GBS, 4), traffic=None,
+                        kernel="fused RK4 stage (k_fused3)",
+                        avg_launch_ms=round(launch_ms, 4),
+                        alg_bytes_per_launch=round(main_res["bytes_per_launch"]))
+    out = {
+        "metric": "OLDG node-updates/s + % of HBM BW, FP32 batched vs FP64 at equal L2 error",
+        "value": main_res["value"],
+        "unit": "node-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": main_res["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": prec,
+        "data": "synthetic",
+        "config": {"workload": "3D periodic uniform forest, P=4, E=4, roots=16 "
+                               "(4096 blocks of 20^3, 32.8M nodes per field), standing sine "
+                               "mode, RK4 with truncation, dt = 0.5 h/(P+1)",
+                   "global_batch": main_res["nodes"], "parallelism": "single GPU"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "forms": {prec: {k: main_res.get(k) for k in ("value", "l2", "linf", "energy_drift")},
+                  other: {k: other_res.get(k) for k in ("value", "l2", "linf", "energy_drift")}},
+    }
+    if main_res.get("l2") and other_res.get("l2"):
+        f64 = main_res if prec == "f64" else other_res
+        f32 = other_res if prec == "f64" else main_res
+        out["forms"]["f32c_over_f64_l2"] = f32["l2"] / f64["l2"]
+        out["forms"]["f32c_over_f64_speed"] = f32["value"] / f64["value"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

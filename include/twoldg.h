@@ -87,7 +87,12 @@ typedef struct twoldg_desc {
   double wave_k[3];
   double wave_phi;
   double hspeed;                /* min over blocks of h_b / max logical speed (0: flat) */
+  int32_t flags;                /* TWOLDG_FLAG_* */
+  int32_t pad;
 } twoldg_desc;
+
+/* flags */
+#define TWOLDG_FLAG_GENERIC 1  /* never use the fused one-launch stages */
 
 typedef struct twoldg_problem twoldg_problem;
 
@@ -99,7 +104,10 @@ int twoldg_device_count(int* out);
 /* --- problem (WaveProblem, src/evolution.jl:1043-1108) ------------------- */
 int twoldg_problem_create(const twoldg_desc* desc, twoldg_problem** out);
 int twoldg_problem_destroy(twoldg_problem* p);
-int twoldg_problem_info(const twoldg_problem* p, int64_t* state_len, int64_t* device_bytes);
+/* state vector length, device bytes held, and whether RK4 runs the fused
+   one-launch stages (1) or the generic exchange + gradient + stage sequence (0) */
+int twoldg_problem_info(const twoldg_problem* p, int64_t* state_len, int64_t* device_bytes,
+                        int32_t* fused);
 
 /* --- state transfer (statevector / gather!; layout above) --------------- */
 int twoldg_state_upload(twoldg_problem* p, int slot, const double* host);
@@ -137,10 +145,11 @@ int twoldg_exchange_plan(const twoldg_desc* desc, int64_t* nrows, int64_t* nnz,
                          int32_t* rowptr, int32_t* cb, int32_t* cl, double* vu, double* vg);
 
 /* --- measurement ---------------------------------------------------------- */
-/* Run nsteps RK4 steps like twoldg_rk4 and report the summed HIP-event time of
-   every launch of the named kernel family on the problem's stream:
-   which = 0 whole step, 1 stage kernel.  out_ms = total elapsed, out_kernel_ms
-   = summed kernel time of `which`, out_launches = its launch count. */
+/* Run nsteps RK4 steps like twoldg_rk4, timed with HIP events on the problem's
+   stream: out_ms = the whole run, out_kernel_ms = the summed time of the stage
+   kernels (the Laplacian + RK combination + truncation launches), measured by
+   an event pair around each of them, out_launches = their count.  which is
+   reserved (0). */
 int twoldg_rk4_timed(twoldg_problem* p, double t0, double dt, int64_t nsteps, int which,
                      double* out_ms, double* out_kernel_ms, int64_t* out_launches);
 

@@ -136,3 +136,55 @@ def test_f32c_close_to_f64():
     r32 = T.wave_errors(2, 4, 2, roots=8, precision="f32c")
     assert r32["l2"] == pytest.approx(r64["l2"], rel=0.1)
     assert r32["energy"] == pytest.approx(r64["energy"], rel=1e-5)
+
+
+FUSED = [
+    dict(D=3, P=3, E=2, roots=2),
+    dict(D=3, P=2, E=2, roots=4, reflecting=True),
+    dict(D=3, P=4, E=4, roots=2),                       # config 2's 20^3 blocks
+    dict(D=3, P=4, E=2, roots=3),
+    dict(D=3, P=1, E=4, roots=2, reflecting=True),
+]
+
+
+@pytest.mark.parametrize("case", FUSED, ids=_id)
+def test_fused_stages_match_oracle(case):
+    """The one-launch fused stages (faces all same-level or mirror) against the
+    oracle's RK4, and against the generic exchange path."""
+    forest = T.wave_forest(case["D"], case["P"], case["E"], roots=case["roots"],
+                           reflecting=case.get("reflecting", False))
+    fused = T.WaveProblem(forest, T.Element(case["P"]))
+    generic = T.WaveProblem(forest, T.Element(case["P"]), generic=True)
+    assert fused.info()["fused"] and not generic.info()["fused"]
+    ref = O.Problem.brick(case["D"], case["P"], case["E"], roots=case["roots"],
+                          reflecting=case.get("reflecting", False))
+    refl = case.get("reflecting", False)
+    y0 = ref.truncate(ref.fill_exact(1 if refl else 0, 0.0, (1.0, 2 if refl else 1)))
+    # a rough perturbation so that every face and element mode is exercised
+    y0 = ref.truncate(y0 + 0.01 * np.random.default_rng(9).standard_normal(y0.size))
+    dt = ref.timestep(0.5)
+    n = 6
+    y = ref.rk4(y0, 0.0, dt, n)
+    for prob in (fused, generic):
+        integ = T.wave_integrator(prob, y0, 0.0, n * dt, dt)
+        integ.solve()
+        assert np.linalg.norm(integ.u - y) <= 1e-13 * np.linalg.norm(y)
+
+
+@pytest.mark.parametrize("case", FUSED[:3], ids=_id)
+def test_fused_f32c_close_to_f64(case):
+    forest = T.wave_forest(case["D"], case["P"], case["E"], roots=case["roots"],
+                           reflecting=case.get("reflecting", False))
+    p32 = T.WaveProblem(forest, T.Element(case["P"]), precision="f32c")
+    g32 = T.WaveProblem(forest, T.Element(case["P"]), precision="f32c", generic=True)
+    assert p32.info()["fused"]
+    ref = O.Problem.brick(case["D"], case["P"], case["E"], roots=case["roots"],
+                          reflecting=case.get("reflecting", False))
+    y0 = ref.truncate(ref.fill_exact(1 if case.get("reflecting") else 0, 0.0, (1.0, 1)))
+    dt = ref.timestep(0.5)
+    n = 6
+    y = ref.rk4(y0, 0.0, dt, n)
+    for prob in (p32, g32):
+        integ = T.wave_integrator(prob, y0, 0.0, n * dt, dt)
+        integ.solve()
+        assert np.linalg.norm(integ.u - y) <= 1e-5 * np.linalg.norm(y)

@@ -59,8 +59,10 @@ void destroy(twoldg_problem* p) {
                   p->grads, p->tables, p->coef, (void*)p->flat, (void*)p->tslot, p->tmodes,
                   p->tweights, p->tchol, (void*)p->spacing, (void*)p->rowptr, (void*)p->cb,
                   (void*)p->cl, p->cval_u, p->cval_g, (void*)p->dir_rows, (void*)p->dir_x,
-                  (void*)p->part, (void*)p->red})
+                  (void*)p->part, (void*)p->red, (void*)p->nbr6, p->uf[0], p->uf[1], p->pf[0],
+                  p->pf[1]})
     dfree(b);
+  for (hipEvent_t e : p->tev) (void)hipEventDestroy(e);
   if (p->ev0) (void)hipEventDestroy(p->ev0);
   if (p->ev1) (void)hipEventDestroy(p->ev1);
   if (p->stream) (void)hipStreamDestroy(p->stream);
@@ -167,6 +169,26 @@ int create_buffers(twoldg_problem* p, const twoldg_desc* d, const Csr& csr) {
     if ((rc = dalloc(p, &p->dir_x, p->ndir * D))) return rc;
     HIP_TRY(hipMemcpy(p->dir_x, x.data(), x.size() * 8, hipMemcpyHostToDevice));
   }
+  // fused one-launch stages: every face same-level or a mirror, flat, 3D
+  bool eligible = !p->full && fused_supported(D, p->P1, p->E);
+  for (int64_t i = 0; eligible && i < p->sh.nb * 2 * D; ++i)
+    if (d->faces[i].kind != TWOLDG_FACE_SAME && d->faces[i].kind != TWOLDG_FACE_MIRROR) eligible = false;
+  if (eligible && !p->no_fused) {
+    std::vector<int32_t> nbr(p->sh.nb * 6);
+    for (int64_t b = 0; b < p->sh.nb; ++b)
+      for (int f = 0; f < 6; ++f) {
+        const twoldg_face& F = d->faces[b * 6 + f];
+        nbr[b * 6 + f] = F.kind == TWOLDG_FACE_SAME ? (int32_t)F.nb[0] : -1;
+      }
+    if ((rc = dalloc(p, &p->nbr6, nbr.size()))) return rc;
+    HIP_TRY(hipMemcpy(p->nbr6, nbr.data(), nbr.size() * 4, hipMemcpyHostToDevice));
+    const size_t nf = (size_t)p->sh.nb * 6 * p->sh.Nt;
+    for (int k = 0; k < 2; ++k) {
+      if ((rc = dalloc_bytes(p, &p->uf[k], nf * sizeof(T)))) return rc;
+      if ((rc = dalloc_bytes(p, &p->pf[k], nf * sizeof(T)))) return rc;
+    }
+    p->fused = true;
+  }
   if ((rc = dalloc(p, &p->part, 3 * p->sh.nb))) return rc;
   if ((rc = dalloc(p, &p->red, 4))) return rc;
   return 0;
@@ -203,6 +225,7 @@ int twoldg_problem_create(const twoldg_desc* d, twoldg_problem** out) {
   p->N = d->E * (d->P + 1);
   p->precision = d->precision;
   p->truncate = d->truncate != 0;
+  p->no_fused = (d->flags & TWOLDG_FLAG_GENERIC) != 0;
   p->full = d->coef != nullptr;
   p->elt = d->precision == TWOLDG_F64 ? 8 : 4;
   p->sh.N = p->N;
@@ -248,10 +271,12 @@ int twoldg_problem_destroy(twoldg_problem* p) {
   return 0;
 }
 
-int twoldg_problem_info(const twoldg_problem* p, int64_t* state_len, int64_t* device_bytes) {
+int twoldg_problem_info(const twoldg_problem* p, int64_t* state_len, int64_t* device_bytes,
+                        int32_t* fused) {
   if (!p) return fail(TWOLDG_EINVAL, "no problem");
   if (state_len) *state_len = p->state_len;
   if (device_bytes) *device_bytes = p->device_bytes;
+  if (fused) *fused = p->fused ? 1 : 0;
   return 0;
 }
 
@@ -384,17 +409,27 @@ int twoldg_exchange_plan(const twoldg_desc* d, int64_t* nrows, int64_t* nnz, int
 int twoldg_rk4_timed(twoldg_problem* p, double t0, double dt, int64_t nsteps, int which,
                      double* out_ms, double* out_kernel_ms, int64_t* out_launches) {
   if (!p) return fail(TWOLDG_EINVAL, "no problem");
+  if (!p->uploaded[TWOLDG_SLOT_STATE]) return fail(TWOLDG_ESTATE, "no state uploaded");
+  (void)which;
+  p->timing = true;
+  p->ntev = 0;
   HIP_TRY(hipEventRecord(p->ev0, p->stream));
   int rc = TW_DISPATCH(rk4, p, t0, dt, nsteps);
+  p->timing = false;
   if (rc) return rc;
   HIP_TRY(hipEventRecord(p->ev1, p->stream));
   HIP_TRY(hipEventSynchronize(p->ev1));
   float ms = 0;
   HIP_TRY(hipEventElapsedTime(&ms, p->ev0, p->ev1));
+  double kms = 0;
+  for (size_t i = 0; i + 1 < p->ntev; i += 2) {
+    float e = 0;
+    HIP_TRY(hipEventElapsedTime(&e, p->tev[i], p->tev[i + 1]));
+    kms += e;
+  }
   if (out_ms) *out_ms = ms;
-  if (out_kernel_ms) *out_kernel_ms = ms;
-  if (out_launches) *out_launches = nsteps;
-  (void)which;
+  if (out_kernel_ms) *out_kernel_ms = kms;
+  if (out_launches) *out_launches = (int64_t)(p->ntev / 2);
   return 0;
 }
 

@@ -635,6 +635,8 @@ __global__ void k_to_double(int64_t n, const T* __restrict__ hi, const T* __rest
   dst[i] = x;
 }
 
+#include "fused3.h"
+
 }  // namespace
 
 // ===========================================================================
@@ -682,6 +684,16 @@ struct twoldg_problem {
   bool has_tables = false;
   int64_t device_bytes = 0;
   bool uploaded[3] = {false, false, false};
+  // fused one-launch stages (3D, faces all same-level or mirror, flat metric)
+  bool fused = false;
+  bool no_fused = false;    // the caller asked for the generic stages
+  int32_t* nbr6 = nullptr;  // (nb, 6)
+  void* uf[2] = {nullptr, nullptr};
+  void* pf[2] = {nullptr, nullptr};
+  // per-launch timing of the stage kernels (bench): event pairs
+  bool timing = false;
+  std::vector<hipEvent_t> tev;
+  size_t ntev = 0;
 };
 
 namespace {
@@ -884,6 +896,17 @@ int upload_tables(twoldg_problem* p) {
   return 0;
 }
 
+// Stage timing for the benchmark: an event pair around each fused stage launch.
+inline void tmark(twoldg_problem* p) {
+  if (!p->timing) return;
+  if (p->ntev >= p->tev.size()) {
+    hipEvent_t e;
+    (void)hipEventCreate(&e);
+    p->tev.push_back(e);
+  }
+  (void)hipEventRecord(p->tev[p->ntev++], p->stream);
+}
+
 // ---- launch helpers, dispatched on (D, P1) at run time ----------------------
 template <class T, int D, int P1>
 struct Launch {
@@ -923,6 +946,12 @@ struct Launch {
   template <int MODE>
   static void stage(twoldg_problem* p, const StageIO<T>& io) {
     const int64_t n = p->sh.nb * p->sh.nn;
+    tmark(p);
+    stage_launch<MODE>(p, io, n);
+    tmark(p);
+  }
+  template <int MODE>
+  static void stage_launch(twoldg_problem* p, const StageIO<T>& io, int64_t n) {
     if (!p->full)
       hipLaunchKernelGGL((k_stage_flat<T, D, P1, MODE>), dim3(grid_for(n, 256)), dim3(256), 0,
                          p->stream, p->sh, (const Tables<T>*)p->tables, p->spacing,
@@ -974,8 +1003,86 @@ struct Launch {
   }
 };
 
+template <class T, int P1, int E>
+int fused_steps(twoldg_problem* p, double dt, int64_t nsteps) {
+  constexpr int N = P1 * E;
+  constexpr int threads = (N * N + 63) / 64 * 64;
+  const int64_t nb = p->sh.nb;
+  T* yn = (T*)p->slot[TWOLDG_SLOT_STATE];
+  {
+    const int64_t n = nb * 6 * (int64_t)N * N;
+    hipLaunchKernelGGL((k_faces3<T, P1, E>), dim3(grid_for(n, 256)), dim3(256), 0, p->stream, nb,
+                       yn, (const Tables<T>*)p->tables, (T*)p->uf[0], (T*)p->pf[0]);
+  }
+  FusedArgs<T> A{};
+  A.yn = yn;
+  A.acc = (T*)p->acc;
+  A.lo = (T*)p->lo;
+  A.nbr = p->nbr6;
+  A.spacing = p->spacing;
+  A.tables = (const Tables<T>*)p->tables;
+  A.truncate = p->truncate ? 1 : 0;
+  auto io = [&](int in) {
+    A.uf_in = (const T*)p->uf[in];
+    A.pf_in = (const T*)p->pf[in];
+    A.uf_out = (T*)p->uf[1 - in];
+    A.pf_out = (T*)p->pf[1 - in];
+  };
+  for (int64_t s = 0; s < nsteps; ++s) {
+    A.ys = yn;
+    A.yo = (T*)p->ys;
+    A.a = (T)(dt / 2);
+    io(0);
+    tmark(p);
+    hipLaunchKernelGGL((k_fused3<T, P1, E, 1>), dim3((unsigned)nb), dim3(threads), 0, p->stream, A);
+    tmark(p);
+    A.ys = (T*)p->ys;
+    io(1);
+    tmark(p);
+    hipLaunchKernelGGL((k_fused3<T, P1, E, 2>), dim3((unsigned)nb), dim3(threads), 0, p->stream, A);
+    tmark(p);
+    A.a = (T)dt;
+    io(0);
+    tmark(p);
+    hipLaunchKernelGGL((k_fused3<T, P1, E, 2>), dim3((unsigned)nb), dim3(threads), 0, p->stream, A);
+    tmark(p);
+    A.a = (T)(dt / 6);
+    A.yo = yn;
+    io(1);
+    tmark(p);
+    if (p->precision == TWOLDG_F32C)
+      hipLaunchKernelGGL((k_fused3<T, P1, E, 4>), dim3((unsigned)nb), dim3(threads), 0, p->stream, A);
+    else
+      hipLaunchKernelGGL((k_fused3<T, P1, E, 3>), dim3((unsigned)nb), dim3(threads), 0, p->stream, A);
+    tmark(p);
+  }
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+// (P+1, E) pairs the fused stage is compiled for.
+template <class T, int D, int P1>
+int fused_dispatch(twoldg_problem* p, double dt, int64_t nsteps) {
+  if constexpr (D == 3) {
+    switch (p->E) {
+      case 2:
+        if constexpr (P1 * 2 <= 32) return fused_steps<T, P1, 2>(p, dt, nsteps);
+        break;
+      case 4:
+        if constexpr (P1 * 4 <= 24) return fused_steps<T, P1, 4>(p, dt, nsteps);
+        break;
+    }
+  }
+  return fail(TWOLDG_EINVAL, "no fused stage for this shape");
+}
+
+inline bool fused_supported(int D, int P1, int E) {
+  return D == 3 && ((E == 2 && P1 * 2 <= 32) || (E == 4 && P1 * 4 <= 24));
+}
+
 template <class T, int D, int P1>
 int rk4_steps(twoldg_problem* p, double t0, double dt, int64_t nsteps) {
+  if (p->fused) return fused_dispatch<T, D, P1>(p, dt, nsteps);
   using L = Launch<T, D, P1>;
   T* yn = (T*)p->slot[TWOLDG_SLOT_STATE];
   T* ys = (T*)p->ys;

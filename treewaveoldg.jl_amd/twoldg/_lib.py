@@ -14,6 +14,7 @@ ABI_VERSION = 1
 F64, F32C = 0, 1
 FACE_SAME, FACE_COARSER, FACE_FINER, FACE_MIRROR, FACE_GLUE, FACE_DIRICHLET = 1, 2, 3, 4, 5, 6
 SLOT_STATE, SLOT_DU, SLOT_AUX = 0, 1, 2
+FLAG_GENERIC = 1
 
 
 class Face(ctypes.Structure):
@@ -32,7 +33,8 @@ class Desc(ctypes.Structure):
                 ("tmodes", ctypes.c_void_p), ("tweights", ctypes.c_void_p),
                 ("tchol", ctypes.c_void_p), ("dirichlet_x", ctypes.c_void_p),
                 ("wave_k", ctypes.c_double * 3), ("wave_phi", ctypes.c_double),
-                ("hspeed", ctypes.c_double)]
+                ("hspeed", ctypes.c_double), ("flags", ctypes.c_int32),
+                ("pad", ctypes.c_int32)]
 
 
 class TwoldgError(RuntimeError):
@@ -59,7 +61,7 @@ def lib():
             "twoldg_device_count": ([_vp], ctypes.c_int),
             "twoldg_problem_create": ([_vp, _vp], ctypes.c_int),
             "twoldg_problem_destroy": ([_vp], ctypes.c_int),
-            "twoldg_problem_info": ([_vp, _vp, _vp], ctypes.c_int),
+            "twoldg_problem_info": ([_vp, _vp, _vp, _vp], ctypes.c_int),
             "twoldg_state_upload": ([_vp, ctypes.c_int, _vp], ctypes.c_int),
             "twoldg_state_download": ([_vp, ctypes.c_int, _vp], ctypes.c_int),
             "twoldg_state_copy": ([_vp, ctypes.c_int, ctypes.c_int], ctypes.c_int),

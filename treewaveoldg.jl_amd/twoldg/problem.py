@@ -48,7 +48,7 @@ class WaveProblem:
     """
 
     def __init__(self, forest, el=None, truncate=True, precision="f64", geometry=None,
-                 boundary=None):
+                 boundary=None, generic=False):
         if el is None:
             el = Element(forest.P)
         if el.P != forest.P:
@@ -88,6 +88,7 @@ class WaveProblem:
         desc.nblocks = self.nblocks
         desc.spacing = _ptr(self.spacings)
         desc.faces = ctypes.cast(self._faces, ctypes.c_void_p)
+        desc.flags = _lib.FLAG_GENERIC if generic else 0
         self.hspeed = 0.0
         if geometry is not None:
             m = geometry.metric(forest, el)
@@ -148,8 +149,10 @@ class WaveProblem:
     def info(self):
         n = ctypes.c_int64()
         b = ctypes.c_int64()
-        _lib.check(_lib.lib().twoldg_problem_info(self._h, ctypes.byref(n), ctypes.byref(b)))
-        return {"state_len": n.value, "device_bytes": b.value}
+        f = ctypes.c_int32()
+        _lib.check(_lib.lib().twoldg_problem_info(self._h, ctypes.byref(n), ctypes.byref(b),
+                                                  ctypes.byref(f)))
+        return {"state_len": n.value, "device_bytes": b.value, "fused": bool(f.value)}
 
     # -- positions -------------------------------------------------------------
     def node_coordinates(self):
@@ -296,7 +299,7 @@ class Integrator:
             raise ValueError("dt must be positive")
         self.p = p
         self.t0, self.t1 = float(t0), float(t1)
-        self.nsteps = max(0, math.ceil((self.t1 - self.t0) / dt - 1e-12))
+        self.nsteps = max(0, int(round((self.t1 - self.t0) / dt)))
         self.dt = float(dt)
         self.istep = 0
         p.upload(u0, _lib.SLOT_STATE)
@@ -308,6 +311,18 @@ class Integrator:
     @property
     def u(self):
         return self.p.download(_lib.SLOT_STATE)
+
+    def timed(self, n):
+        """n steps timed with HIP events on the problem's stream: (total ms,
+        summed stage-kernel ms, stage launches)."""
+        ms = ctypes.c_double()
+        kms = ctypes.c_double()
+        nl = ctypes.c_int64()
+        _lib.check(_lib.lib().twoldg_rk4_timed(self.p._h, self.t, self.dt, int(n), 0,
+                                               ctypes.byref(ms), ctypes.byref(kms),
+                                               ctypes.byref(nl)))
+        self.istep += n
+        return ms.value, kms.value, nl.value
 
     def step(self, n=1):
         n = min(n, self.nsteps - self.istep)
