@@ -153,6 +153,29 @@ int twoldg_exchange_plan(const twoldg_desc* desc, int64_t* nrows, int64_t* nnz,
 int twoldg_rk4_timed(twoldg_problem* p, double t0, double dt, int64_t nsteps, int which,
                      double* out_ms, double* out_kernel_ms, int64_t* out_launches);
 
+/* --- batches of independent problems --------------------------------------- */
+/* Many independent flat 2D problems (uniform: same-level or mirror faces, P <= 7,
+   (E(P+1))^2 <= 256 nodes per block), each with its own order, spacing and time
+   step, advanced together: one launch per RK4 stage over a device-resident work
+   list of (order, first block, blocks per workgroup), the four stage launches of
+   a step captured once in a hipGraph and replayed.  The blocks per workgroup of
+   each order are chosen at creation by timing the candidates.  All problems
+   share precision and truncation.  There is no batch in the reference: this is
+   its wave_errors driver run over many problems at once (sinewave.jl:228-268). */
+typedef struct twoldg_batch twoldg_batch;
+int twoldg_batch_create(const twoldg_desc* const* descs, int64_t n, const double* dt,
+                        twoldg_batch** out);
+int twoldg_batch_destroy(twoldg_batch* b);
+/* problem i's state, statearray layout of that problem */
+int twoldg_batch_upload(twoldg_batch* b, int64_t i, const double* host);
+int twoldg_batch_download(twoldg_batch* b, int64_t i, double* host);
+/* nsteps RK4 steps of every problem (graph replays); out_ms: HIP-event time */
+int twoldg_batch_rk4(twoldg_batch* b, int64_t nsteps, double* out_ms);
+/* nodes per field over all problems, workgroups per stage launch, and per order
+   P = 1..8 the chosen blocks per workgroup (0: order absent) and its timed stage ms */
+int twoldg_batch_info(const twoldg_batch* b, int64_t* nodes, int64_t* workgroups,
+                      int32_t* tiles, double* tune_ms);
+
 #ifdef __cplusplus
 }
 #endif

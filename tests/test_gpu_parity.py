@@ -188,3 +188,92 @@ def test_fused_f32c_close_to_f64(case):
         integ = T.wave_integrator(prob, y0, 0.0, n * dt, dt)
         integ.solve()
         assert np.linalg.norm(integ.u - y) <= 1e-5 * np.linalg.norm(y)
+
+
+def _ball(P=3, E=2, roots=(2, 2), precision="f64", truncate=True):
+    from twoldg.geometry import sevenpatch_forest
+    forest, geo = sevenpatch_forest(P, E, roots=roots)
+    wave = T.PlaneWave((1.1, -0.7, 0.5), 0.3)
+    prob = T.WaveProblem(forest, T.Element(P), truncate=truncate, precision=precision,
+                         geometry=geo, boundary=wave)
+    ref = O.Problem.ball(P, E, roots=roots, truncate=truncate)
+    return prob, ref
+
+
+@pytest.mark.parametrize("P", [2, 3])
+def test_ball_rhs_truncate_energy_match_oracle(P, rough):
+    """Full metric: Cartesian gradient, frame-weighted flux, glue ghosts and the
+    Dirichlet sphere at t (evolution.jl:136-220, 1110-1127); per-element
+    sqrt(g)-orthogonal truncation (evolution.jl:384-425); sqrt(g)-weighted energy."""
+    prob, ref = _ball(P=P)
+    u = rough(prob.state_len, seed=11)
+    du = np.empty_like(u)
+    T.wave_rhs(du, u, prob, 0.37)
+    want = ref.rhs(u, 0.37)
+    assert np.max(np.abs(du - want)) <= 1e-12 * np.max(np.abs(want))
+    got = T.truncate_state(u.copy(), prob)
+    want = ref.truncate(u)
+    assert np.max(np.abs(got - want)) <= 1e-12 * np.max(np.abs(want))
+    assert T.wave_energy(prob, u, t=0.37) == pytest.approx(ref.energy(u, 0.37), rel=1e-12)
+    l2 = T.wave_norm(prob, u)
+    assert l2 == pytest.approx(ref.norms(u)[0], rel=1e-13)
+
+
+def test_ball_truncation_is_projection(rough):
+    """sqrt(g)-orthogonal projection: idempotent (sevenpatch_tests.jl:128-158)."""
+    prob, _ = _ball()
+    u = rough(prob.state_len, seed=12)
+    once = T.truncate_state(u.copy(), prob)
+    twice = T.truncate_state(once.copy(), prob)
+    assert np.max(np.abs(twice - once)) <= 1e-12 * np.max(np.abs(once))
+    nn = prob.nn
+    assert np.array_equal(once.reshape(-1, 2, nn)[:, 0], u.reshape(-1, 2, nn)[:, 0])
+
+
+def test_sevenpatch_errors_published_and_oracle():
+    """sevenpatch_errors(P=3, E=2, roots=(2,2)), k=(1.1,-0.7,0.5), phi=0.3, t=1:
+    u 8.83e-3, v 1.14e-2, 1 - E1/E0 = 0.035 (CODE.md:4251), and the oracle's run."""
+    r = T.sevenpatch_errors(3, 2, roots=(2, 2))
+    o = O.sevenpatch_errors(3, 2, roots=(2, 2))
+    assert r["nsteps"] == o["nsteps"]
+    assert r["l2u"] == pytest.approx(8.83e-3, rel=5e-3)
+    assert r["l2v"] == pytest.approx(1.14e-2, rel=5e-3)
+    assert 1 - r["energy"] / r["energy0"] == pytest.approx(0.035, abs=5e-4)
+    assert r["l2"] == pytest.approx(o["l2"], rel=SQRT_EPS)
+    assert r["energy"] == pytest.approx(o["energy"], rel=SQRT_EPS)
+    u = r["integrator"].u
+    assert np.linalg.norm(u - o["y"]) <= SQRT_EPS * np.linalg.norm(o["y"])
+
+
+def test_sevenpatch_f32c():
+    """FP32 stencil with FP64-built coefficients stored as FP32 (config 5's form)."""
+    r64 = T.sevenpatch_errors(3, 2, roots=(2, 2))
+    r32 = T.sevenpatch_errors(3, 2, roots=(2, 2), precision="f32c")
+    assert r32["l2"] == pytest.approx(r64["l2"], rel=1e-3)
+
+
+def test_batch_matches_oracle_per_problem():
+    """Batched uneven problems (orders 2..6, roots 4/8/16, Gaussian pulses), one
+    launch per stage over the work list, graph-replayed steps: every problem
+    equals the oracle's own RK4 run of it."""
+    from twoldg.batch import gaussian_pulses, initial_states
+    probs = gaussian_pulses(n=12, seed=7, roots=(2, 4))
+    for prec, tol in (("f64", 1e-12), ("f32c", 2e-5)):
+        batch = T.Batch([p[:3] for p in probs], precision=prec)
+        u0s = initial_states(probs)
+        for i, u0 in enumerate(u0s):
+            batch.upload(i, u0)
+        n = 5
+        batch.rk4(2)
+        batch.rk4(n - 2)
+        info = batch.info()
+        assert set(info["tiles"]) == {p[3]["P"] for p in probs}
+        for i, p in enumerate(probs):
+            ref = O.Problem.brick(2, p[3]["P"], 2, roots=p[3]["roots"])
+            c, w = p[3]["c"], p[3]["w"]
+            y0 = ref.truncate(ref.fill_exact(3, 0.0, (c[0], c[1], w)))
+            assert np.max(np.abs(y0 - u0s[i])) <= 1e-15
+            y = ref.rk4(y0, 0.0, p[2], n)
+            got = batch.download(i)
+            assert np.linalg.norm(got - y) <= tol * np.linalg.norm(y), (prec, i)
+        batch.close()

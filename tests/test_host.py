@@ -102,3 +102,51 @@ def test_layout_refusals():
         T.wave_forest(2, 2, 2, roots=2, refined=True)
     with pytest.raises(ValueError, match="refined must be"):
         T.wave_forest(2, 2, 2, roots=4, refined="yes")
+
+
+@pytest.mark.parametrize("roots", [(2, 2), (4, 4)])
+def test_ball_geometry_and_glue_match_oracle(roots):
+    """Seven-patch ball: positions bitwise, the closed-form frame against the
+    oracle's dual-number Jacobian, and every glued ghost (identity and quarter
+    turns, geometry.jl:418-455) equal to the oracle's physical-coincidence
+    source, through the exchange plan."""
+    from twoldg.geometry import sevenpatch_forest
+    P, E = 3, 2
+    f, geo = sevenpatch_forest(P, E, roots=roots)
+    o = O.Problem.ball(P, E, roots=roots)
+    assert [(lf[0], lf[1], tuple(lf[2:5])) for lf in f.leaves] == [b[:3] for b in o.blocks()]
+    el = T.Element(P)
+    assert np.array_equal(geo.physical(f, f.node_logical(el.offsets)), o.coords())
+    m = geo.metric(f, el)
+    co, flo = o.coef()
+    assert np.max(np.abs(m["coef"] - co)) <= 1e-14 * np.max(np.abs(co))
+    assert np.array_equal(m["flat"].astype(bool), flo) and m["flat"].sum() == roots[0] ** 3
+    assert m["hspeed"] == pytest.approx(o.timestep(1.0) * (P + 1), rel=1e-14)
+    plan = T.exchange_plan(f)
+    y = np.random.default_rng(1).standard_normal(o.size())
+    ug, gg = o.ghosts(y, 0.0)
+    nn, Nt = (E * (P + 1)) ** 3, (E * (P + 1)) ** 2
+    rp, cb, cl = plan["rowptr"], plan["cb"].astype(np.int64), plan["cl"]
+    rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+    has = np.diff(rp) > 0
+    assert (~has).sum() == 6 * roots[0] ** 2 * Nt      # the sphere's ghosts take data
+    mu = np.zeros(len(rp) - 1)
+    np.add.at(mu, rows, plan["vu"] * y[cb * 2 * nn + cl])
+    assert np.array_equal(mu[has], ug[has])
+    g = o.gradients(y, 0.0).reshape(f.nblocks, 3, nn)
+    ggm = gg.reshape(-1, 3, Nt)
+    for i in range(3):
+        mg = np.zeros(len(rp) - 1)
+        np.add.at(mg, rows, plan["vu"] * g[cb, i, cl])
+        hf = has.reshape(-1, Nt)
+        assert np.array_equal(mg.reshape(-1, Nt)[hf], ggm[:, i, :][hf])
+
+
+def test_ball_refusals():
+    from twoldg.geometry import SevenPatchBall, sevenpatch_forest
+    with pytest.raises(ValueError, match="must exceed"):
+        SevenPatchBall(1.0, 1.5, (2, 2))
+    with pytest.raises(ValueError, match="reaches the cube's faces"):
+        sevenpatch_forest(2, 2, roots=(2, 2), refined=((0.0, 2.0),) * 3)
+    f, geo = sevenpatch_forest(2, 2, roots=(4, 2), refined=True)
+    assert sum(1 for lf in f.leaves if lf[1] == 1) == 64

@@ -1,0 +1,401 @@
+// batch.hip -- many independent 2D problems advanced together: one launch per
+// RK4 stage over a device-resident work list, a hipGraph per step, and a
+// blocks-per-workgroup tile per element order chosen by timing.
+#include "twoldg_impl.h"
+#include "fused2.h"
+
+struct twoldg_batch {
+  int precision = TWOLDG_F64;
+  bool truncate = true;
+  size_t elt = 8;
+  int64_t nprob = 0, nb = 0, total = 0, nfaces = 0;
+  std::vector<int64_t> prob_off, prob_len;  // state slice of each problem
+  std::vector<int32_t> P1s_host;
+  int tiles[10] = {0};                      // blocks per workgroup by P1
+  double tune_ms[10] = {0};
+  // device
+  int64_t *off = nullptr, *foff = nullptr;
+  int32_t *nbr = nullptr, *P1s = nullptr, *Ns = nullptr;
+  double *spacing = nullptr, *dt = nullptr;
+  void* tables = nullptr;
+  BatchItem* items = nullptr;
+  int64_t nitems = 0;
+  void *yn = nullptr, *ys = nullptr, *acc = nullptr, *lo = nullptr;
+  void *uf[2] = {nullptr, nullptr}, *pf[2] = {nullptr, nullptr};
+  hipStream_t stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int64_t device_bytes = 0;
+};
+
+namespace {
+
+int balloc(twoldg_batch* B, void** p, size_t bytes) {
+  hipError_t e = hipMalloc(p, bytes ? bytes : 8);
+  if (e != hipSuccess)
+    return fail(TWOLDG_ENOMEM, std::string("hipMalloc of ") + std::to_string(bytes) +
+                                   " bytes: " + hipGetErrorString(e));
+  B->device_bytes += (int64_t)bytes;
+  return 0;
+}
+
+template <class T>
+int batch_tables(twoldg_batch* B) {
+  std::vector<Tables<T>> tb(8);
+  std::memset(tb.data(), 0, tb.size() * sizeof(Tables<T>));
+  for (int P = 1; P <= 8; ++P) {
+    const twoldg::ElementTables& et = twoldg::kElementTables[P - 1];
+    for (int i = 0; i <= P; ++i) {
+      for (int j = 0; j < P + 3; ++j) tb[P - 1].W[i][j] = (T)et.stencil[i][j];
+      tb[P - 1].mode[i] = (T)et.topmode[i];
+      tb[P - 1].dual[i] = (T)et.topdual[i];
+      tb[P - 1].wq[i] = et.weights[i];
+    }
+  }
+  int rc = balloc(B, &B->tables, tb.size() * sizeof(Tables<T>));
+  if (rc) return rc;
+  HIP_TRY(hipMemcpy(B->tables, tb.data(), tb.size() * sizeof(Tables<T>), hipMemcpyHostToDevice));
+  return 0;
+}
+
+// Blocks per workgroup that fit the thread count and the LDS of the kernel.
+int max_tile(int N) {
+  const int NT = N * N, NP = N + 2;
+  const int per = NP * NP + 2 * N * NP + 3 * NT;
+  return std::max(1, std::min(kBatchThreads / NT, kBatchLds / per));
+}
+
+// The work list with tile[P1] blocks per entry; only orders in `only` (0: all).
+std::vector<BatchItem> make_items(const twoldg_batch* B, const std::vector<int32_t>& Nh,
+                                  const int* tile, int only) {
+  std::vector<BatchItem> items;
+  int64_t b = 0;
+  while (b < B->nb) {
+    const int P1 = B->P1s_host[b], N = Nh[b];
+    int64_t e = b;
+    while (e < B->nb && B->P1s_host[e] == P1 && Nh[e] == N) ++e;
+    if (only == 0 || only == P1) {
+      const int t = std::max(1, tile[P1]);
+      for (int64_t f = b; f < e; f += t)
+        items.push_back(BatchItem{P1, N, (int32_t)std::min<int64_t>(t, e - f), 0, f});
+    }
+    b = e;
+  }
+  return items;
+}
+
+template <class T, int MODE>
+void launch_stage(twoldg_batch* B, const BatchItem* items, int64_t nitems, const T* ys, T* yo,
+                  double coef, int in) {
+  BatchArgs<T> A{};
+  A.items = items;
+  A.off = B->off;
+  A.foff = B->foff;
+  A.nbr = B->nbr;
+  A.spacing = B->spacing;
+  A.dt = B->dt;
+  A.tables = (const Tables<T>*)B->tables;
+  A.ys = ys;
+  A.yn = (const T*)B->yn;
+  A.acc = (T*)B->acc;
+  A.yo = yo;
+  A.lo = (T*)B->lo;
+  A.uf_in = (const T*)B->uf[in];
+  A.pf_in = (const T*)B->pf[in];
+  A.uf_out = (T*)B->uf[1 - in];
+  A.pf_out = (T*)B->pf[1 - in];
+  A.coef = coef;
+  A.truncate = B->truncate ? 1 : 0;
+  hipLaunchKernelGGL((k_batch2<T, MODE>), dim3((unsigned)nitems), dim3(kBatchThreads), 0,
+                     B->stream, A);
+}
+
+template <class T>
+void enqueue_step(twoldg_batch* B) {
+  T* yn = (T*)B->yn;
+  T* ys = (T*)B->ys;
+  launch_stage<T, 1>(B, B->items, B->nitems, yn, ys, 0.5, 0);
+  launch_stage<T, 2>(B, B->items, B->nitems, ys, ys, 0.5, 1);
+  launch_stage<T, 2>(B, B->items, B->nitems, ys, ys, 1.0, 0);
+  if (B->precision == TWOLDG_F32C)
+    launch_stage<T, 4>(B, B->items, B->nitems, ys, yn, 1.0 / 6.0, 1);
+  else
+    launch_stage<T, 3>(B, B->items, B->nitems, ys, yn, 1.0 / 6.0, 1);
+}
+
+// Time the middle stage for each order and candidate tile; keep the fastest.
+template <class T>
+int autotune(twoldg_batch* B, const std::vector<int32_t>& Nh) {
+  std::vector<int> orders;
+  for (int64_t b = 0; b < B->nb; ++b)
+    if (std::find(orders.begin(), orders.end(), B->P1s_host[b]) == orders.end())
+      orders.push_back(B->P1s_host[b]);
+  BatchItem* scratch = nullptr;
+  int rc = balloc(B, (void**)&scratch, sizeof(BatchItem) * (size_t)B->nb);
+  if (rc) return rc;
+  for (int P1 : orders) {
+    int N = 0;
+    for (int64_t b = 0; b < B->nb; ++b)
+      if (B->P1s_host[b] == P1) N = std::max<int>(N, Nh[b]);
+    const int tmax = max_tile(N);
+    float best = 1e30f;
+    int besttile = 1;
+    for (int t = 1; t <= tmax; t = (t == tmax) ? tmax + 1 : std::min(2 * t, tmax)) {
+      int tile[10];
+      for (int i = 0; i < 10; ++i) tile[i] = t;
+      std::vector<BatchItem> it = make_items(B, Nh, tile, P1);
+      HIP_TRY(hipMemcpy(scratch, it.data(), it.size() * sizeof(BatchItem), hipMemcpyHostToDevice));
+      // stage 2 writes the accumulator and the stage buffer only
+      launch_stage<T, 2>(B, scratch, (int64_t)it.size(), (const T*)B->yn, (T*)B->ys, 0.5, 0);
+      HIP_TRY(hipEventRecord(B->ev0, B->stream));
+      for (int r = 0; r < 3; ++r)
+        launch_stage<T, 2>(B, scratch, (int64_t)it.size(), (const T*)B->yn, (T*)B->ys, 0.5, 0);
+      HIP_TRY(hipEventRecord(B->ev1, B->stream));
+      HIP_TRY(hipEventSynchronize(B->ev1));
+      float ms = 0;
+      HIP_TRY(hipEventElapsedTime(&ms, B->ev0, B->ev1));
+      if (ms < best) {
+        best = ms;
+        besttile = t;
+      }
+    }
+    B->tiles[P1] = besttile;
+    B->tune_ms[P1] = best / 3;
+  }
+  (void)hipFree(scratch);
+  return 0;
+}
+
+template <class T>
+int batch_build(twoldg_batch* B, const twoldg_desc* const* d, int64_t n, const double* dt) {
+  // order problems by element order (then as given): each order's blocks are
+  // one run of the block list, so one tile applies to the run
+  std::vector<int64_t> order(n);
+  for (int64_t i = 0; i < n; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int64_t a, int64_t b) { return d[a]->P < d[b]->P; });
+  B->nprob = n;
+  B->prob_off.assign(n, 0);
+  B->prob_len.assign(n, 0);
+  std::vector<int64_t> first(n, 0), offh, foffh;
+  std::vector<int32_t> nbrh, Nh;
+  std::vector<double> hh, dth;
+  int64_t nb = 0, total = 0, fsum = 0;
+  for (int64_t oi = 0; oi < n; ++oi) {
+    const int64_t i = order[oi];
+    const twoldg_desc* p = d[i];
+    const int N = p->E * (p->P + 1);
+    first[i] = nb;
+    B->prob_off[i] = total;
+    B->prob_len[i] = p->nblocks * 2 * (int64_t)N * N;
+    for (int64_t b = 0; b < p->nblocks; ++b) {
+      offh.push_back(total);
+      foffh.push_back(fsum);
+      total += 2 * (int64_t)N * N;
+      fsum += 4 * (int64_t)N;
+      B->P1s_host.push_back(p->P + 1);
+      Nh.push_back(N);
+      hh.push_back(p->spacing[b]);
+      dth.push_back(dt[i]);
+    }
+    nb += p->nblocks;
+  }
+  for (int64_t oi = 0; oi < n; ++oi) {
+    const int64_t i = order[oi];
+    const twoldg_desc* p = d[i];
+    for (int64_t b = 0; b < p->nblocks; ++b)
+      for (int f = 0; f < 4; ++f) {
+        const twoldg_face& F = p->faces[b * 4 + f];
+        nbrh.push_back(F.kind == TWOLDG_FACE_SAME ? (int32_t)(first[i] + F.nb[0]) : -1);
+      }
+  }
+  B->nb = nb;
+  B->total = total;
+  B->nfaces = fsum;
+  int rc;
+  if ((rc = balloc(B, (void**)&B->off, nb * 8)) || (rc = balloc(B, (void**)&B->foff, nb * 8)) ||
+      (rc = balloc(B, (void**)&B->nbr, nb * 16)) || (rc = balloc(B, (void**)&B->P1s, nb * 4)) ||
+      (rc = balloc(B, (void**)&B->Ns, nb * 4)) || (rc = balloc(B, (void**)&B->spacing, nb * 8)) ||
+      (rc = balloc(B, (void**)&B->dt, nb * 8)))
+    return rc;
+  HIP_TRY(hipMemcpy(B->off, offh.data(), nb * 8, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(B->foff, foffh.data(), nb * 8, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(B->nbr, nbrh.data(), nb * 16, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(B->P1s, B->P1s_host.data(), nb * 4, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(B->Ns, Nh.data(), nb * 4, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(B->spacing, hh.data(), nb * 8, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(B->dt, dth.data(), nb * 8, hipMemcpyHostToDevice));
+  if ((rc = batch_tables<T>(B))) return rc;
+  for (void** p : {&B->yn, &B->ys, &B->acc})
+    if ((rc = balloc(B, p, total * sizeof(T)))) return rc;
+  if (B->precision == TWOLDG_F32C && (rc = balloc(B, &B->lo, total * sizeof(T)))) return rc;
+  for (int k = 0; k < 2; ++k)
+    if ((rc = balloc(B, &B->uf[k], fsum * sizeof(T))) || (rc = balloc(B, &B->pf[k], fsum * sizeof(T))))
+      return rc;
+  for (void* p : {B->yn, B->ys, B->acc, B->lo, B->uf[0], B->uf[1], B->pf[0], B->pf[1]})
+    if (p) HIP_TRY(hipMemset(p, 0, (p == B->uf[0] || p == B->uf[1] || p == B->pf[0] || p == B->pf[1] ? fsum : total) * sizeof(T)));
+  // tiles per order, then the work list, then the step graph
+  if ((rc = autotune<T>(B, Nh))) return rc;
+  std::vector<BatchItem> it = make_items(B, Nh, B->tiles, 0);
+  B->nitems = (int64_t)it.size();
+  if ((rc = balloc(B, (void**)&B->items, it.size() * sizeof(BatchItem)))) return rc;
+  HIP_TRY(hipMemcpy(B->items, it.data(), it.size() * sizeof(BatchItem), hipMemcpyHostToDevice));
+  HIP_TRY(hipStreamSynchronize(B->stream));
+  HIP_TRY(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
+  enqueue_step<T>(B);
+  HIP_TRY(hipStreamEndCapture(B->stream, &B->graph));
+  HIP_TRY(hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0));
+  return 0;
+}
+
+void batch_destroy(twoldg_batch* B) {
+  if (!B) return;
+  if (B->stream) (void)hipStreamSynchronize(B->stream);
+  if (B->exec) (void)hipGraphExecDestroy(B->exec);
+  if (B->graph) (void)hipGraphDestroy(B->graph);
+  for (void* p : {(void*)B->off, (void*)B->foff, (void*)B->nbr, (void*)B->P1s, (void*)B->Ns,
+                  (void*)B->spacing, (void*)B->dt, B->tables, (void*)B->items, B->yn, B->ys,
+                  B->acc, B->lo, B->uf[0], B->uf[1], B->pf[0], B->pf[1]})
+    if (p) (void)hipFree(p);
+  if (B->ev0) (void)hipEventDestroy(B->ev0);
+  if (B->ev1) (void)hipEventDestroy(B->ev1);
+  if (B->stream) (void)hipStreamDestroy(B->stream);
+  delete B;
+}
+
+template <class T>
+int batch_steps(twoldg_batch* B, int64_t nsteps) {
+  hipLaunchKernelGGL((k_batch_faces<T>), dim3((unsigned)B->nb), dim3(64), 0, B->stream, B->nb,
+                     B->P1s, B->Ns, B->off, B->foff, (const Tables<T>*)B->tables,
+                     (const T*)B->yn, (T*)B->uf[0], (T*)B->pf[0]);
+  for (int64_t s = 0; s < nsteps; ++s) HIP_TRY(hipGraphLaunch(B->exec, B->stream));
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int twoldg_batch_create(const twoldg_desc* const* descs, int64_t n, const double* dt,
+                        twoldg_batch** out) {
+  if (!out) return fail(TWOLDG_EINVAL, "no output handle");
+  *out = nullptr;
+  if (!descs || n < 1 || !dt) return fail(TWOLDG_EINVAL, "a batch needs problems and time steps");
+  const int prec = descs[0]->precision;
+  const int trunc = descs[0]->truncate;
+  for (int64_t i = 0; i < n; ++i) {
+    const twoldg_desc* d = descs[i];
+    if (!d || d->abi_version != TWOLDG_ABI_VERSION)
+      return fail(TWOLDG_EINVAL, "problem " + std::to_string(i) + ": bad description");
+    if (d->D != 2 || d->coef)
+      return fail(TWOLDG_EINVAL, "a batch takes flat two-dimensional problems");
+    if (d->P < 1 || d->P > 7 || d->E < 2 || d->E % 2)
+      return fail(TWOLDG_EINVAL, "problem " + std::to_string(i) + ": P must be 1..7 and E even");
+    const int N = d->E * (d->P + 1);
+    if (N * N > kBatchThreads || max_tile(N) < 1)
+      return fail(TWOLDG_EINVAL, "problem " + std::to_string(i) + ": blocks of " +
+                                     std::to_string(N) + "^2 nodes exceed a workgroup");
+    if (d->precision != prec || d->truncate != trunc)
+      return fail(TWOLDG_EINVAL, "every problem of a batch shares its precision and truncation");
+    if (!(dt[i] > 0)) return fail(TWOLDG_EINVAL, "time steps must be positive");
+    for (int64_t f = 0; f < d->nblocks * 4; ++f) {
+      const int k = d->faces[f].kind;
+      if (k != TWOLDG_FACE_SAME && k != TWOLDG_FACE_MIRROR)
+        return fail(TWOLDG_EINVAL, "a batch takes uniform problems (same-level or mirror faces)");
+      if (k == TWOLDG_FACE_SAME && (d->faces[f].nb[0] < 0 || d->faces[f].nb[0] >= d->nblocks))
+        return fail(TWOLDG_EINVAL, "face neighbour out of range");
+    }
+  }
+  auto* B = new twoldg_batch();
+  B->precision = prec;
+  B->truncate = trunc != 0;
+  B->elt = prec == TWOLDG_F64 ? 8 : 4;
+  hipError_t e = hipStreamCreate(&B->stream);
+  if (e != hipSuccess) {
+    delete B;
+    return fail(TWOLDG_EDEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  }
+  (void)hipEventCreate(&B->ev0);
+  (void)hipEventCreate(&B->ev1);
+  int rc = prec == TWOLDG_F64 ? batch_build<double>(B, descs, n, dt)
+                              : batch_build<float>(B, descs, n, dt);
+  if (rc) {
+    batch_destroy(B);
+    return rc;
+  }
+  *out = B;
+  return 0;
+}
+
+int twoldg_batch_destroy(twoldg_batch* B) {
+  batch_destroy(B);
+  return 0;
+}
+
+int twoldg_batch_upload(twoldg_batch* B, int64_t i, const double* host) {
+  if (!B || !host || i < 0 || i >= B->nprob) return fail(TWOLDG_EINVAL, "bad batch upload");
+  const int64_t n = B->prob_len[i], o = B->prob_off[i];
+  double* tmp = nullptr;
+  HIP_TRY(hipMalloc((void**)&tmp, n * 8));
+  hipError_t e = hipMemcpyAsync(tmp, host, n * 8, hipMemcpyHostToDevice, B->stream);
+  if (e == hipSuccess) {
+    if (B->precision == TWOLDG_F64)
+      hipLaunchKernelGGL((k_from_double<double>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                         B->stream, n, tmp, (double*)B->yn + o, (double*)nullptr);
+    else
+      hipLaunchKernelGGL((k_from_double<float>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                         B->stream, n, tmp, (float*)B->yn + o, B->lo ? (float*)B->lo + o : nullptr);
+    e = hipStreamSynchronize(B->stream);
+  }
+  (void)hipFree(tmp);
+  if (e != hipSuccess) return fail(TWOLDG_EDEVICE, std::string("batch upload: ") + hipGetErrorString(e));
+  return 0;
+}
+
+int twoldg_batch_download(twoldg_batch* B, int64_t i, double* host) {
+  if (!B || !host || i < 0 || i >= B->nprob) return fail(TWOLDG_EINVAL, "bad batch download");
+  const int64_t n = B->prob_len[i], o = B->prob_off[i];
+  double* tmp = nullptr;
+  HIP_TRY(hipMalloc((void**)&tmp, n * 8));
+  if (B->precision == TWOLDG_F64)
+    hipLaunchKernelGGL((k_to_double<double>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       B->stream, n, (const double*)B->yn + o, (const double*)nullptr, tmp);
+  else
+    hipLaunchKernelGGL((k_to_double<float>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       B->stream, n, (const float*)B->yn + o,
+                       B->lo ? (const float*)B->lo + o : nullptr, tmp);
+  hipError_t e = hipMemcpyAsync(host, tmp, n * 8, hipMemcpyDeviceToHost, B->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(B->stream);
+  (void)hipFree(tmp);
+  if (e != hipSuccess) return fail(TWOLDG_EDEVICE, std::string("batch download: ") + hipGetErrorString(e));
+  return 0;
+}
+
+int twoldg_batch_rk4(twoldg_batch* B, int64_t nsteps, double* out_ms) {
+  if (!B || nsteps < 0) return fail(TWOLDG_EINVAL, "bad batch step call");
+  HIP_TRY(hipEventRecord(B->ev0, B->stream));
+  int rc = B->precision == TWOLDG_F64 ? batch_steps<double>(B, nsteps) : batch_steps<float>(B, nsteps);
+  if (rc) return rc;
+  HIP_TRY(hipEventRecord(B->ev1, B->stream));
+  HIP_TRY(hipEventSynchronize(B->ev1));
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, B->ev0, B->ev1));
+  if (out_ms) *out_ms = ms;
+  return 0;
+}
+
+int twoldg_batch_info(const twoldg_batch* B, int64_t* nodes, int64_t* workgroups, int32_t* tiles,
+                      double* tune_ms) {
+  if (!B) return fail(TWOLDG_EINVAL, "no batch");
+  if (nodes) *nodes = B->total / 2;
+  if (workgroups) *workgroups = B->nitems;
+  for (int P = 1; P <= 8; ++P) {
+    if (tiles) tiles[P - 1] = B->tiles[P + 1];
+    if (tune_ms) tune_ms[P - 1] = B->tune_ms[P + 1];
+  }
+  return 0;
+}
+
+}  // extern "C"

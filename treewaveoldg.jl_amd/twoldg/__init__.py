@@ -11,6 +11,7 @@ from .problem import (WaveProblem, wave_rhs, truncate_state, truncate_stage, ini
                       wave_integrator, Integrator, exchange_plan)
 from .drivers import (wave_omega, wave_exact, reflect_omega, reflect_exact, gaussian_pulse,
                       PlaneWave, sevenpatch_exact, wave_errors, reflect_errors, sevenpatch_errors)
+from .batch import Batch, gaussian_pulses
 from . import _lib
 
 __all__ = [
@@ -19,5 +20,6 @@ __all__ = [
     "initial_state", "fill_at_nodes", "wave_energy", "wave_norm", "quadrature_norm",
     "oldg_timestep", "wave_integrator", "Integrator", "exchange_plan", "wave_omega",
     "wave_exact", "reflect_omega", "reflect_exact", "gaussian_pulse", "PlaneWave",
-    "sevenpatch_exact", "wave_errors", "reflect_errors", "sevenpatch_errors",
+    "sevenpatch_exact", "wave_errors", "reflect_errors", "sevenpatch_errors", "Batch",
+    "gaussian_pulses",
 ]

@@ -76,6 +76,12 @@ def lib():
             "twoldg_exchange_plan": ([_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),
             "twoldg_rk4_timed": ([_vp, _dbl, _dbl, _i64, ctypes.c_int, _vp, _vp, _vp],
                                  ctypes.c_int),
+            "twoldg_batch_create": ([_vp, _i64, _vp, _vp], ctypes.c_int),
+            "twoldg_batch_destroy": ([_vp], ctypes.c_int),
+            "twoldg_batch_upload": ([_vp, _i64, _vp], ctypes.c_int),
+            "twoldg_batch_download": ([_vp, _i64, _vp], ctypes.c_int),
+            "twoldg_batch_rk4": ([_vp, _i64, _vp], ctypes.c_int),
+            "twoldg_batch_info": ([_vp, _vp, _vp, _vp, _vp], ctypes.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -104,4 +110,6 @@ def exported_symbols():
             "twoldg_problem_create", "twoldg_problem_destroy", "twoldg_problem_info",
             "twoldg_state_upload", "twoldg_state_download", "twoldg_state_copy", "twoldg_rhs",
             "twoldg_truncate", "twoldg_rk4", "twoldg_energy", "twoldg_norms", "twoldg_timestep",
-            "twoldg_synchronize", "twoldg_exchange_plan", "twoldg_rk4_timed"]
+            "twoldg_synchronize", "twoldg_exchange_plan", "twoldg_rk4_timed",
+            "twoldg_batch_create", "twoldg_batch_destroy", "twoldg_batch_upload",
+            "twoldg_batch_download", "twoldg_batch_rk4", "twoldg_batch_info"]

@@ -224,16 +224,24 @@ def truncate_stage(u, integrator, p, t):
     return None
 
 
-def fill_at_nodes(f, p):
-    """The state vector of f(x, var) at every owned node (var = 1 for u, 2 for
-    v); f takes x as an array (D, n) of positions and returns n values."""
-    x = p.node_coordinates()
+def _fill(x, f):
     nb, D, nn = x.shape
     u = np.empty((nb, 2, nn))
     xs = np.moveaxis(x, 1, 0).reshape(D, nb * nn)
     for var in (1, 2):
         u[:, var - 1, :] = np.asarray(f(xs, var), dtype=np.float64).reshape(nb, nn)
     return u.reshape(-1)
+
+
+def fill_at_nodes(f, p):
+    """The state vector of f(x, var) at every owned node (var = 1 for u, 2 for
+    v); f takes x as an array (D, n) of positions and returns n values."""
+    return _fill(p.node_coordinates(), f)
+
+
+def fill_at_nodes_forest(forest, f, el):
+    """fill_at_nodes on a flat forest, without a problem."""
+    return _fill(forest.node_logical(el.offsets), f)
 
 
 def initial_state(p, f):
