@@ -124,6 +124,7 @@ def main():
     ap.add_argument("--precision", default="f64", choices=["f64", "f32c"])
     ap.add_argument("--accuracy-steps", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--single", action="store_true", help="time only --precision's form")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for "
@@ -131,7 +132,7 @@ def main():
         sys.exit(2)
     main_res = gpu_run(args, args.precision)
     other = "f32c" if args.precision == "f64" else "f64"
-    other_res = gpu_run(args, other)
+    other_res = {} if args.single else gpu_run(args, other)
     cpu = None if args.no_cpu else cpu_baseline()
     prec = args.precision
     launch_ms = main_res["kernel_ms"] / max(main_res["launches"], 1)

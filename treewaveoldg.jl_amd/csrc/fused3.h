@@ -21,8 +21,11 @@
 
 // Waves per SIMD the fused stage is compiled for.  FP64 needs about 250 VGPRs
 // without spilling (two waves per SIMD); FP32 fits three.
+#ifndef TWOLDG_FUSED_WAVES_F32
+#define TWOLDG_FUSED_WAVES_F32 3
+#endif
 template <class T>
-constexpr int kFusedWavesPerEU = sizeof(T) == 8 ? 2 : 3;
+constexpr int kFusedWavesPerEU = sizeof(T) == 8 ? 2 : TWOLDG_FUSED_WAVES_F32;
 
 // A workgroup barrier that orders LDS only.  __syncthreads() also fences global
 // memory, which makes every wave wait for its outstanding global loads at each
@@ -63,14 +66,20 @@ k_fused3(FusedArgs<T> A) {
   constexpr int PW = P1 + 2;
   constexpr int P = P1 - 1;
   constexpr int NP = N + 2;
+  constexpr int SU = NP * NP, SG = N * NP;  // plane strides of U and of GX, GY
+  // One element layer (P1 planes) per phase.  R1 holds the u planes, later the
+  // values to truncate; R2 the x-gradients, later the x-truncated values.
   __shared__ T sW[P1][PW];
   __shared__ T sMode[P1], sDual[P1];
-  __shared__ T U[NP * NP];         // u plane with its x and y ghosts
-  __shared__ T GX[N * NP];         // g_x plane, x ghosts
-  __shared__ T GY[NP * N];         // g_y plane, y ghosts
-  __shared__ T VA[NT], VB[NT];     // truncation passes
-  __shared__ T UN[NT];             // next u plane (face parts)
+  __shared__ T R1[P1 * SU];
+  __shared__ T R2[P1 * SG];
+  __shared__ T GY[P1 * SG];
+  __shared__ T UN[P1 * NT];
   __shared__ T EG[8][NT];          // x/y neighbours' face values: u (xl xh yl yh), then p
+  T* U = R1;
+  T* VA = R1;
+  T* GX = R2;
+  T* VB = R2;
 
   const int64_t b = blockIdx.x;
   const int tid = threadIdx.x;
@@ -95,6 +104,21 @@ k_fused3(FusedArgs<T> A) {
   T* acc = A.acc + b * 2 * nn + col;
   T* yo = A.yo + b * 2 * nn + col;
   T* lo = A.lo ? A.lo + b * 2 * nn + col : nullptr;
+  const int nxl = nb6[0], nxh = nb6[1], nyl = nb6[2], nyh = nb6[3];
+
+  // the x and y neighbours' face planes, once per block
+  {
+    const int nbs[4] = {nxl, nxh, nyl, nyh};
+    const int opp[4] = {1, 0, 3, 2};  // the neighbour's face that touches ours
+    for (int idx = tid; idx < 8 * NT; idx += blockDim.x) {
+      const int q = idx / NT, t = idx % NT, f = q % 4;
+      const int nbq = nbs[f];
+      if (nbq >= 0) {
+        const T* src = q < 4 ? A.uf_in : A.pf_in;
+        EG[q][t] = src[((int64_t)nbq * 6 + opp[f]) * N2 + t];
+      }
+    }
+  }
   lds_barrier();
 
   auto load_layer = [&](int e, T* dst) {
@@ -135,8 +159,6 @@ k_fused3(FusedArgs<T> A) {
   }
   if (E > 1) load_layer(1, up);
 
-  // x and y neighbours
-  const int nxl = nb6[0], nxh = nb6[1], nyl = nb6[2], nyh = nb6[3];
   const T rxl = nxl >= 0 ? T(1) / (T)A.spacing[nxl] : rh, rxh = nxh >= 0 ? T(1) / (T)A.spacing[nxh] : rh;
   const T ryl = nyl >= 0 ? T(1) / (T)A.spacing[nyl] : rh, ryh = nyh >= 0 ? T(1) / (T)A.spacing[nyh] : rh;
   const int ix = x % P1, iy = y % P1;
@@ -145,50 +167,32 @@ k_fused3(FusedArgs<T> A) {
   T pz_lo = 0, pz_hi = 0;  // partial z-face gradients of the next u
   T uz0 = 0, uzN = 0;      // the next u on the z faces
 
-  // Every global operand of plane z: the stage arithmetic's, and on the block's
-  // x and y faces the neighbours' face values.  They are loaded one plane ahead
-  // so that a plane's loads are in flight while the previous plane's LDS phases
-  // run.
-  struct Ops {
-    T ku, yu, yv, au, av;
-  };
-  auto load_ops = [&](int z) {
-    Ops o{};
-    if (!act) return o;
-    const int64_t iu = (int64_t)z * N2, iv = iu + nn;
-    o.ku = vs[iu];
-    if (MODE != 4) {
-      o.yu = un_[iu];
-      o.yv = un_[iv];
-    }
-    if (MODE >= 2) {
-      o.au = acc[iu];
-      o.av = acc[iv];
-    }
-    return o;
-  };
-  // the x and y neighbours' face planes, once per block
-  {
-    const int nbs[4] = {nxl, nxh, nyl, nyh};
-    const int opp[4] = {1, 0, 3, 2};  // the neighbour's face that touches ours
-    for (int idx = tid; idx < 8 * NT; idx += blockDim.x) {
-      const int q = idx / NT, t = idx % NT, f = q % 4;
-      const int nbq = nbs[f];
-      if (nbq >= 0) {
-        const T* src = q < 4 ? A.uf_in : A.pf_in;
-        EG[q][t] = src[((int64_t)nbq * 6 + opp[f]) * N2 + t];
-      }
-    }
-  }
-  Ops cur = load_ops(0);
-  lds_barrier();
-
 #pragma unroll 1
   for (int e = 0; e < E; ++e) {
+    // ---- the layer's global operands: issued first, used in phase (3) ----
+    T ku[P1], yu[P1], yv[P1], au[P1], av[P1];
+#pragma unroll
+    for (int i = 0; i < P1; ++i) {
+      ku[i] = yu[i] = yv[i] = au[i] = av[i] = T(0);
+      if (act) {
+        const int64_t iu = (int64_t)(e * P1 + i) * N2, iv = iu + nn;
+        ku[i] = vs[iu];
+        yu[i] = un_[iu];
+        yv[i] = un_[iv];
+        if (MODE >= 2) {
+          au[i] = acc[iu];
+          av[i] = acc[iv];
+        }
+      }
+    }
+    // the column two layers on, needed when the window slides
+    T un2[P1];
+    if (e + 2 < E) load_layer(e + 2, un2);
+
     // ---- the layer's z-gradients (one node either side) and z-Laplacian ----
-    T gl[PW];
+    T lapl[P1];
     {
-      // node e*P1 - 1: row P of the previous layer, its ghost after = uc[0]
+      T gl[PW];
       if (e == 0) {
         gl[0] = gg_lo;
       } else {
@@ -206,7 +210,6 @@ k_fused3(FusedArgs<T> A) {
         }
         gl[i + 1] = s * rh;
       }
-      // node e*P1 + P1: row 0 of the next layer, its ghost before = uc[P]
       if (e == E - 1) {
         gl[PW - 1] = gg_hi;
       } else {
@@ -215,156 +218,174 @@ k_fused3(FusedArgs<T> A) {
         for (int j = 0; j < P1; ++j) s += sW[0][j + 1] * up[j];
         gl[PW - 1] = s * rh;
       }
-    }
-    T lapl[P1];
 #pragma unroll
-    for (int i = 0; i < P1; ++i) {
-      T s = 0;
+      for (int i = 0; i < P1; ++i) {
+        T s = 0;
 #pragma unroll
-      for (int k = 0; k < PW; ++k) {
-        if ((k == 0 && i != 0) || (k == PW - 1 && i != P)) continue;
-        s += sW[i][k] * gl[k];
+        for (int k = 0; k < PW; ++k) {
+          if ((k == 0 && i != 0) || (k == PW - 1 && i != P)) continue;
+          s += sW[i][k] * gl[k];
+        }
+        lapl[i] = s;
       }
-      lapl[i] = s;
     }
 
-    T wl[P1];  // the layer's values to truncate (v_next, or its increment)
+    // (1) the layer's u planes and their x/y ghosts
+    if (act) {
 #pragma unroll
-    for (int i = 0; i < P1; ++i) {
-      const int z = e * P1 + i;
-      const T uz = uc[i];
-      const int64_t iu = (int64_t)z * N2, iv = iu + nn;
-      // the next plane's operands go out first
-      Ops nxt{};
-      if (z + 1 < N) nxt = load_ops(z + 1);
-      const T ku = cur.ku, yu = cur.yu, yv = cur.yv, au = cur.au, av = cur.av;
-      // (1) the u plane and its x/y ghosts
-      if (act) {
-        U[(y + 1) * NP + x + 1] = uz;
-        if (x == 0) U[(y + 1) * NP] = nxl >= 0 ? EG[0][y + N * z] : uz;
-        if (x == N - 1) U[(y + 1) * NP + N + 1] = nxh >= 0 ? EG[1][y + N * z] : uz;
-        if (y == 0) U[x + 1] = nyl >= 0 ? EG[2][x + N * z] : uz;
-        if (y == N - 1) U[(N + 1) * NP + x + 1] = nyh >= 0 ? EG[3][x + N * z] : uz;
+      for (int i = 0; i < P1; ++i) {
+        const int z = e * P1 + i;
+        const T uz = uc[i];
+        T* Ui = U + i * SU;
+        Ui[(y + 1) * NP + x + 1] = uz;
+        if (x == 0) Ui[(y + 1) * NP] = nxl >= 0 ? EG[0][y + N * z] : uz;
+        if (x == N - 1) Ui[(y + 1) * NP + N + 1] = nxh >= 0 ? EG[1][y + N * z] : uz;
+        if (y == 0) Ui[x + 1] = nyl >= 0 ? EG[2][x + N * z] : uz;
+        if (y == N - 1) Ui[(N + 1) * NP + x + 1] = nyh >= 0 ? EG[3][x + N * z] : uz;
       }
-      lds_barrier();
-      // (2) g_x and g_y at the node, and the ghosts of the gradients
-      if (act) {
+    }
+    lds_barrier();
+    // (2) g_x and g_y, and the ghosts of the gradients
+    if (act) {
+#pragma unroll
+      for (int i = 0; i < P1; ++i) {
+        const int z = e * P1 + i;
+        const T uz = uc[i];
+        const T* Ui = U + i * SU;
         T gx = 0, gy = 0;
 #pragma unroll
         for (int k = 0; k < PW; ++k) {
-          gx += sW[ix][k] * U[(y + 1) * NP + (sx + k + 1)];
-          gy += sW[iy][k] * U[(sy + k + 1) * NP + x + 1];
+          gx += sW[ix][k] * Ui[(y + 1) * NP + (sx + k + 1)];
+          gy += sW[iy][k] * Ui[(sy + k + 1) * NP + x + 1];
         }
         gx *= rh;
         gy *= rh;
-        GX[y * NP + x + 1] = gx;
-        GY[(y + 1) * N + x] = gy;
-        if (x == 0)
-          GX[y * NP] = nxl >= 0 ? (EG[4][y + N * z] + Whi * uz) * rxl : -gx;
-        if (x == N - 1)
-          GX[y * NP + N + 1] = nxh >= 0 ? (EG[5][y + N * z] + Wlo * uz) * rxh : -gx;
-        if (y == 0)
-          GY[x] = nyl >= 0 ? (EG[6][x + N * z] + Whi * uz) * ryl : -gy;
-        if (y == N - 1)
-          GY[(N + 1) * N + x] = nyh >= 0 ? (EG[7][x + N * z] + Wlo * uz) * ryh : -gy;
+        T* GXi = GX + i * SG;
+        T* GYi = GY + i * SG;
+        GXi[y * NP + x + 1] = gx;
+        GYi[(y + 1) * N + x] = gy;
+        if (x == 0) GXi[y * NP] = nxl >= 0 ? (EG[4][y + N * z] + Whi * uz) * rxl : -gx;
+        if (x == N - 1) GXi[y * NP + N + 1] = nxh >= 0 ? (EG[5][y + N * z] + Wlo * uz) * rxh : -gx;
+        if (y == 0) GYi[x] = nyl >= 0 ? (EG[6][x + N * z] + Whi * uz) * ryl : -gy;
+        if (y == N - 1) GYi[(N + 1) * N + x] = nyh >= 0 ? (EG[7][x + N * z] + Wlo * uz) * ryh : -gy;
       }
-      lds_barrier();
-      // (3) the Laplacian and the stage combination
-      T unext = 0, w = 0;
+    }
+    lds_barrier();
+    // (3) the Laplacian and the stage combination
+    T w[P1];
+#pragma unroll
+    for (int i = 0; i < P1; ++i) {
+      const int z = e * P1 + i;
+      T unext = 0;
+      w[i] = 0;
       if (act) {
+        const T* GXi = GX + i * SG;
+        const T* GYi = GY + i * SG;
         T lx = 0, ly = 0;
 #pragma unroll
         for (int k = 0; k < PW; ++k) {
-          lx += sW[ix][k] * GX[y * NP + (sx + k + 1)];
-          ly += sW[iy][k] * GY[(sy + k + 1) * N + x];
+          lx += sW[ix][k] * GXi[y * NP + (sx + k + 1)];
+          ly += sW[iy][k] * GYi[(sy + k + 1) * N + x];
         }
         const T lap = ((T(0) + lx) + ly) + lapl[i];
         const T kv = lap * rh;
+        const int64_t iu = (int64_t)z * N2, iv = iu + nn;
         if (MODE == 1) {
-          acc[iu] = ku;
+          acc[iu] = ku[i];
           acc[iv] = kv;
-          unext = yu + A.a * ku;
-          w = yv + A.a * kv;
+          unext = yu[i] + A.a * ku[i];
+          w[i] = yv[i] + A.a * kv;
         } else if (MODE == 2) {
-          acc[iu] = au + T(2) * ku;
-          acc[iv] = av + T(2) * kv;
-          unext = yu + A.a * ku;
-          w = yv + A.a * kv;
+          acc[iu] = au[i] + T(2) * ku[i];
+          acc[iv] = av[i] + T(2) * kv;
+          unext = yu[i] + A.a * ku[i];
+          w[i] = yv[i] + A.a * kv;
         } else if (MODE == 3) {
-          unext = yu + A.a * (au + ku);
-          w = yv + A.a * (av + kv);
+          unext = yu[i] + A.a * (au[i] + ku[i]);
+          w[i] = yv[i] + A.a * (av[i] + kv);
         } else {  // compensated: u by Kahan now, v's increment truncated first
-          const T du = A.a * (au + ku);
-          const T yk = du - lo[iu], hu = un_[iu], tu = hu + yk;
+          const T du = A.a * (au[i] + ku[i]);
+          const T yk = du - lo[iu], hu = yu[i], tu = hu + yk;
           lo[iu] = (tu - hu) - yk;
           unext = tu;
-          w = A.a * (av + kv);
+          w[i] = A.a * (av[i] + kv);
         }
         yo[iu] = unext;
-        VA[tid] = w;
-        UN[tid] = unext;
+        UN[i * NT + tid] = unext;
       }
       // faces of the next u along z
       if (e == 0) pz_lo += sW[0][i + 1] * unext;
       if (e == E - 1) pz_hi += sW[P][i + 1] * unext;
       if (z == 0) uz0 = unext;
       if (z == N - 1) uzN = unext;
-      lds_barrier();
-      // (4) faces of the next u along x and y; truncation along x
-      if (act) {
+    }
+    lds_barrier();
+    // U is dead: the values to truncate go to VA (aliasing R1)
+    if (act) {
+#pragma unroll
+      for (int i = 0; i < P1; ++i) VA[i * NT + tid] = w[i];
+    }
+    lds_barrier();
+    // (4) faces of the next u along x and y; truncation along x (GX is dead: VB)
+    if (act) {
+#pragma unroll
+      for (int i = 0; i < P1; ++i) {
+        const int z = e * P1 + i;
+        const T* UNi = UN + i * NT;
         if (x == 0) {
           T s = 0;
 #pragma unroll
-          for (int j = 0; j < P1; ++j) s += sW[0][j + 1] * UN[y * N + j];
-          A.uf_out[(b * 6 + 0) * N2 + y + N * z] = unext;
+          for (int j = 0; j < P1; ++j) s += sW[0][j + 1] * UNi[y * N + j];
+          A.uf_out[(b * 6 + 0) * N2 + y + N * z] = UNi[tid];
           A.pf_out[(b * 6 + 0) * N2 + y + N * z] = s;
         }
         if (x == N - 1) {
           T s = 0;
 #pragma unroll
-          for (int j = 0; j < P1; ++j) s += sW[P][j + 1] * UN[y * N + N - P1 + j];
-          A.uf_out[(b * 6 + 1) * N2 + y + N * z] = unext;
+          for (int j = 0; j < P1; ++j) s += sW[P][j + 1] * UNi[y * N + N - P1 + j];
+          A.uf_out[(b * 6 + 1) * N2 + y + N * z] = UNi[tid];
           A.pf_out[(b * 6 + 1) * N2 + y + N * z] = s;
         }
         if (y == 0) {
           T s = 0;
 #pragma unroll
-          for (int j = 0; j < P1; ++j) s += sW[0][j + 1] * UN[j * N + x];
-          A.uf_out[(b * 6 + 2) * N2 + x + N * z] = unext;
+          for (int j = 0; j < P1; ++j) s += sW[0][j + 1] * UNi[j * N + x];
+          A.uf_out[(b * 6 + 2) * N2 + x + N * z] = UNi[tid];
           A.pf_out[(b * 6 + 2) * N2 + x + N * z] = s;
         }
         if (y == N - 1) {
           T s = 0;
 #pragma unroll
-          for (int j = 0; j < P1; ++j) s += sW[P][j + 1] * UN[(N - P1 + j) * N + x];
-          A.uf_out[(b * 6 + 3) * N2 + x + N * z] = unext;
+          for (int j = 0; j < P1; ++j) s += sW[P][j + 1] * UNi[(N - P1 + j) * N + x];
+          A.uf_out[(b * 6 + 3) * N2 + x + N * z] = UNi[tid];
           A.pf_out[(b * 6 + 3) * N2 + x + N * z] = s;
         }
+        const T* VAi = VA + i * NT;
         if (A.truncate) {
           T c = 0;
 #pragma unroll
-          for (int j = 0; j < P1; ++j) c += sDual[j] * VA[y * N + (x - ix) + j];
-          VB[tid] = w - sMode[ix] * c;
+          for (int j = 0; j < P1; ++j) c += sDual[j] * VAi[y * N + (x - ix) + j];
+          VB[i * NT + tid] = w[i] - sMode[ix] * c;
         } else {
-          VB[tid] = w;
+          VB[i * NT + tid] = w[i];
         }
       }
-      lds_barrier();
-      // (5) truncation along y: the layer's value
-      if (act) {
-        if (A.truncate) {
-          T c = 0;
-#pragma unroll
-          for (int j = 0; j < P1; ++j) c += sDual[j] * VB[((y - iy) + j) * N + x];
-          wl[i] = VB[tid] - sMode[iy] * c;
-        } else {
-          wl[i] = VB[tid];
-        }
-      }
-      cur = nxt;
     }
-    // end of the layer: truncation along z, then v out
+    lds_barrier();
+    // (5) truncation along y, then along z in registers; v out
     if (act) {
+      T wl[P1];
+#pragma unroll
+      for (int i = 0; i < P1; ++i) {
+        const T* VBi = VB + i * NT;
+        wl[i] = VBi[tid];
+        if (A.truncate) {
+          T c = 0;
+#pragma unroll
+          for (int j = 0; j < P1; ++j) c += sDual[j] * VBi[((y - iy) + j) * N + x];
+          wl[i] -= sMode[iy] * c;
+        }
+      }
       if (A.truncate) {
         T c = 0;
 #pragma unroll
@@ -376,7 +397,7 @@ k_fused3(FusedArgs<T> A) {
       for (int j = 0; j < P1; ++j) {
         const int64_t iv = (int64_t)(e * P1 + j) * N2 + nn;
         if (MODE == 4) {
-          const T yk = wl[j] - lo[iv], hv = un_[iv], tv = hv + yk;
+          const T yk = wl[j] - lo[iv], hv = yv[j], tv = hv + yk;
           lo[iv] = (tv - hv) - yk;
           yo[iv] = tv;
         } else {
@@ -390,8 +411,10 @@ k_fused3(FusedArgs<T> A) {
 #pragma unroll
     for (int j = 0; j < P1; ++j) prev_part += sW[P][j + 1] * uc[j];
 #pragma unroll
-    for (int j = 0; j < P1; ++j) uc[j] = up[j];
-    if (e + 2 < E) load_layer(e + 2, up);
+    for (int j = 0; j < P1; ++j) {
+      uc[j] = up[j];
+      up[j] = un2[j];
+    }
   }
   if (act) {
     A.uf_out[(b * 6 + 4) * N2 + col] = uz0;
