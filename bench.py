@@ -116,6 +116,117 @@ def cpu_baseline(seconds=15.0):
                        f"step, FP64 C++/OpenMP restatement (oracle), {threads} threads")
 
 
+def timed_integrator(T, prob, u0, dt, steps, warmup=1):
+    integ = T.wave_integrator(prob, u0, 0.0, (steps + warmup) * dt, dt)
+    integ.step(warmup)
+    T._lib.check(T._lib.lib().twoldg_synchronize(prob._h))
+    t0 = time.perf_counter()
+    ms, kms, nl = integ.timed(steps)
+    wall = time.perf_counter() - t0
+    return wall, kms, nl, integ
+
+
+def config1():
+    """wave_errors(Val(2); P=4, E=2, roots=8) end to end on the GPU (parity run)."""
+    import twoldg as T
+    t0 = time.perf_counter()
+    r = T.wave_errors(2, 4, 2, roots=8)
+    el = time.perf_counter() - t0
+    nodes = r["nblocks"] * 100
+    log(f"[bench] config 1: l2 {r['l2']:.4e} (u {r['l2u']:.3e}, v {r['l2v']:.3e}), "
+        f"E1/E0 {r['energy'] / r['energy0']:.12f}, {r['nsteps']} steps, {el:.2f} s wall")
+    return dict(l2=r["l2"], l2u=r["l2u"], l2v=r["l2v"], linf=r["linf"],
+                energy_ratio=r["energy"] / r["energy0"], nsteps=r["nsteps"], wall_s=el,
+                nodes=nodes, note="6400 nodes: launch-bound, a parity run")
+
+
+def config3(large=False, steps=20):
+    """Reflecting box with nested cubes (3 levels, finest half-width 1/8): P=3,
+    E=2, roots=8 (1408 blocks), or the large refined forest P=4, E=4, roots=16
+    (11264 blocks of 20^3) for the bandwidth figure; cos mode m=2, FP64 and
+    FP32c, generic exchange path (coarse-fine faces)."""
+    import twoldg as T
+    P, E, roots = (4, 4, 16) if large else (3, 2, 8)
+    out = {}
+    forest = T.wave_forest(3, P, E, roots=roots, reflecting=True,
+                           refined=T.nested_cubes(3, 1 / 8, 3))
+    nodes = forest.nblocks * forest.N ** 3
+    for prec in ("f64", "f32c"):
+        prob = T.WaveProblem(forest, T.Element(P), precision=prec)
+        u0 = T.initial_state(prob, T.reflect_exact(3, 1.0, 2, 0.0))
+        dt = T.oldg_timestep(prob, 0.5)
+        e0 = T.wave_energy(prob, u0)
+        wall, kms, nl, integ = timed_integrator(T, prob, u0, dt, steps)
+        e1 = integ.energy()
+        alg = VALUES_PER_NODE_UPDATE[prec] * BYTES_PER_VALUE[prec] * nodes * steps
+        out[prec] = dict(value=nodes * steps / wall, ms_per_step=wall * 1e3 / steps,
+                         alg_gbs_whole_step=alg / wall / 1e9, energy_drift=(e1 - e0) / e0,
+                         steps=steps + 1)
+        log(f"[bench] config 3{'-large' if large else ''} {prec}: {forest.nblocks} blocks, "
+            f"{nodes} nodes, {nodes * steps / wall:.3e} node-updates/s, "
+            f"{alg / wall / 1e9:.0f} GB/s algorithmic, energy drift {(e1 - e0) / e0:.2e}")
+        prob.close()
+    out["nodes"] = nodes
+    out["blocks"] = forest.nblocks
+    return out
+
+
+def config4(steps=20):
+    """512 independent 2D periodic problems, P in {2..6}, roots in {4, 8, 16},
+    E=2, seeded Gaussian pulses, one launch per stage, graph-replayed steps."""
+    import twoldg as T
+    from twoldg.batch import gaussian_pulses, initial_states
+    probs = gaussian_pulses()
+    u0s = initial_states(probs)
+    out = {}
+    for prec in ("f32c", "f64"):
+        t0 = time.perf_counter()
+        batch = T.Batch([p[:3] for p in probs], precision=prec)
+        setup = time.perf_counter() - t0
+        for i, u in enumerate(u0s):
+            batch.upload(i, u)
+        batch.rk4(2)
+        ms = batch.rk4(steps)
+        info = batch.info()
+        nodes = info["nodes"]
+        alg = VALUES_PER_NODE_UPDATE[prec] * BYTES_PER_VALUE[prec] * nodes * steps
+        out[prec] = dict(value=nodes * steps / (ms * 1e-3), ms_per_step=ms / steps,
+                         alg_gbs=alg / (ms * 1e-3) / 1e9, tiles=info["tiles"],
+                         workgroups_per_launch=info["workgroups"], setup_s=setup)
+        log(f"[bench] config 4 {prec}: {len(probs)} problems, {nodes} nodes, "
+            f"{nodes * steps / (ms * 1e-3):.3e} node-updates/s ({ms / steps:.3f} ms/step, "
+            f"{alg / (ms * 1e-3) / 1e9:.0f} GB/s), tiles {info['tiles']}")
+        batch.close()
+    out["nodes"] = nodes
+    out["problems"] = len(probs)
+    out["seed"] = 20261203
+    return out
+
+
+def config5(steps=20):
+    """Seven-patch ball, P=3, E=2, roots=(4,4) (448 blocks), plane wave
+    k = 2 pi (1,1,1)/sqrt(3), phi = 0, Dirichlet sphere; FP64 and FP32c; l2 at
+    t = 1 for both forms."""
+    import twoldg as T
+    k = tuple(2 * math.pi / math.sqrt(3) for _ in range(3))
+    out = {}
+    for prec in ("f64", "f32c"):
+        r = T.sevenpatch_errors(3, 2, roots=(4, 4), k=k, phi=0.0, precision=prec)
+        prob = r["problem"]
+        nodes = prob.nblocks * prob.nn
+        u0 = T.initial_state(prob, T.sevenpatch_exact(T.PlaneWave(k, 0.0), 0.0))
+        wall, kms, nl, integ = timed_integrator(T, prob, u0, r["dt"], steps)
+        out[prec] = dict(l2=r["l2"], l2u=r["l2u"], l2v=r["l2v"], linf=r["linf"],
+                         energy_ratio=r["energy"] / r["energy0"], nsteps=r["nsteps"],
+                         value=nodes * steps / wall, ms_per_step=wall * 1e3 / steps)
+        log(f"[bench] config 5 {prec}: {prob.nblocks} blocks, l2 {r['l2']:.4e} at t=1 "
+            f"({r['nsteps']} steps), E1/E0 {r['energy'] / r['energy0']:.6f}, "
+            f"{nodes * steps / wall:.3e} node-updates/s")
+        prob.close()
+    out["f32c_over_f64_l2"] = out["f32c"]["l2"] / out["f64"]["l2"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -125,6 +236,8 @@ def main():
     ap.add_argument("--accuracy-steps", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--single", action="store_true", help="time only --precision's form")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip configs 1, 3, 4 and 5 (reported beside the main line)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for "
@@ -133,6 +246,13 @@ def main():
     main_res = gpu_run(args, args.precision)
     other = "f32c" if args.precision == "f64" else "f64"
     other_res = {} if args.single else gpu_run(args, other)
+    configs = {}
+    if not args.no_configs:
+        configs["config1"] = config1()
+        configs["config3"] = config3()
+        configs["config3_large"] = config3(large=True, steps=10)
+        configs["config4"] = config4()
+        configs["config5"] = config5()
     cpu = None if args.no_cpu else cpu_baseline()
     prec = args.precision
     launch_ms = main_res["kernel_ms"] / max(main_res["launches"], 1)
@@ -166,6 +286,8 @@ def main():
         "forms": {prec: {k: main_res.get(k) for k in ("value", "l2", "linf", "energy_drift")},
                   other: {k: other_res.get(k) for k in ("value", "l2", "linf", "energy_drift")}},
     }
+    if configs:
+        out["configs"] = configs
     if main_res.get("l2") and other_res.get("l2"):
         f64 = main_res if prec == "f64" else other_res
         f32 = other_res if prec == "f64" else main_res

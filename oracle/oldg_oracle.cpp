@@ -1712,6 +1712,28 @@ void orc_ghosts(void* p, const double* y, double t, double* ug, double* gg) {
       }
 }
 
+// D_d g with g exchanged as the gradient's component d (odd under a mirror in
+// d): the Laplacian pass with every other gradient zero (refined_tests.jl:54-68).
+void orc_divergence(void* p, const double* g, int d, double* out) {
+  Problem* pr = (Problem*)p;
+  const int D = pr->D, N = pr->N;
+  for (int e = 0; e < D; ++e) std::fill(pr->grads[e].begin(), pr->grads[e].end(), 0.0);
+  int64_t gn[3] = {1, 1, 1};
+  for (int e = 0; e < D; ++e) gn[e] = N + (e == d ? 2 : 0);
+  for (int64_t b = 0; b < pr->nb; ++b)
+    for (int64_t l = 0; l < pr->nn; ++l) {
+      int64_t I[3] = {l % N, (l / N) % N, l / ((int64_t)N * N)};
+      int64_t G[3] = {I[0], I[1], I[2]};
+      G[d] += 1;
+      pr->grads[d][b * pr->grad_n[d] + hidx(*pr, G, gn)] = g[b * pr->nn + l];
+    }
+  exchange_grads(*pr);
+  std::vector<double> y(pr->nb * 2 * pr->nn, 0.0), du(pr->nb * 2 * pr->nn);
+  laplacian_flat(*pr, y.data(), du.data());
+  for (int64_t b = 0; b < pr->nb; ++b)
+    for (int64_t l = 0; l < pr->nn; ++l) out[b * pr->nn + l] = du[b * 2 * pr->nn + pr->nn + l];
+}
+
 void orc_norms(void* p, const double* err, int vars_mask, int pnorm, double* lp, double* linf) {
   norms(*(Problem*)p, err, vars_mask, pnorm, lp, linf);
 }

@@ -52,6 +52,7 @@ def lib():
         L.orc_energy.restype = ctypes.c_double
         L.orc_gradients.argtypes = [_dp, _dp, ctypes.c_double, _dp]
         L.orc_ghosts.argtypes = [_dp, _dp, ctypes.c_double, _dp, _dp]
+        L.orc_divergence.argtypes = [_dp, _dp, ctypes.c_int, _dp]
         L.orc_norms.argtypes = [_dp, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp]
         L.orc_min_spacing.argtypes = [_dp]
         L.orc_min_spacing.restype = ctypes.c_double
@@ -188,6 +189,14 @@ class Problem:
         gg = np.zeros(rows * (self.D if self.geometry else 1))
         lib().orc_ghosts(self.h, _ptr(y), float(t), _ptr(ug), _ptr(gg))
         return ug, gg
+
+    def divergence(self, g, d):
+        """D_d g with g exchanged as the gradient's component d (odd under a
+        mirror in d); g and the result are (nblocks, N^D)."""
+        g = np.ascontiguousarray(g, dtype=np.float64)
+        out = np.zeros(self.nb * self.nn)
+        lib().orc_divergence(self.h, _ptr(g), int(d), _ptr(out))
+        return out.reshape(self.nb, self.nn)
 
     def norms(self, err, vars=(1, 2), p=2):
         err = np.ascontiguousarray(err, dtype=np.float64)

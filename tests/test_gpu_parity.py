@@ -277,3 +277,37 @@ def test_batch_matches_oracle_per_problem():
             got = batch.download(i)
             assert np.linalg.norm(got - y) <= tol * np.linalg.norm(y), (prec, i)
         batch.close()
+
+
+def test_golden_fixtures():
+    """The HIP path against the committed fixtures (tests/golden): RHS of a
+    seeded state in 2D and on the ball (Dirichlet at t = 0.25), the ball's
+    per-element truncation, and the drivers' results."""
+    import json
+    import os
+    from twoldg.geometry import sevenpatch_forest
+    G = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    f = np.load(os.path.join(G, "rhs_2d_p4.npz"))
+    prob = T.WaveProblem(T.wave_forest(2, 4, 2, roots=2), T.Element(4))
+    du = np.empty_like(f["y"])
+    T.wave_rhs(du, f["y"], prob, 0.0)
+    assert np.max(np.abs(du - f["du"])) <= 1e-12 * np.max(np.abs(f["du"]))
+    f = np.load(os.path.join(G, "rhs_ball_p1.npz"))
+    forest, geo = sevenpatch_forest(1, 2, roots=(2, 2))
+    pb = T.WaveProblem(forest, T.Element(1), geometry=geo,
+                       boundary=T.PlaneWave((1.1, -0.7, 0.5), 0.3))
+    du = np.empty_like(f["y"])
+    T.wave_rhs(du, f["y"], pb, 0.25)
+    assert np.max(np.abs(du - f["du"])) <= 1e-12 * np.max(np.abs(f["du"]))
+    tr = T.truncate_state(f["y"].copy(), pb)
+    assert np.max(np.abs(tr - f["ytrunc"])) <= 1e-12 * np.max(np.abs(f["ytrunc"]))
+    res = json.load(open(os.path.join(G, "results.json")))
+    runs = {"config1": T.wave_errors(2, 4, 2, roots=8),
+            "reflect_d2_p2_r4": T.reflect_errors(2, 2, 2, roots=4),
+            "twolevel_d2_p3_r4": T.wave_errors(2, 3, 2, roots=4, refined=True),
+            "sevenpatch_p3_s1": T.sevenpatch_errors(3, 2, roots=(2, 2))}
+    for name, r in runs.items():
+        want = res[name]
+        assert r["nsteps"] == want["nsteps"], name
+        for k in ("l2", "l2u", "l2v", "linf", "energy0", "energy"):
+            assert r[k] == pytest.approx(want[k], rel=SQRT_EPS), (name, k)

@@ -150,3 +150,23 @@ def test_ball_refusals():
         sevenpatch_forest(2, 2, roots=(2, 2), refined=((0.0, 2.0),) * 3)
     f, geo = sevenpatch_forest(2, 2, roots=(4, 2), refined=True)
     assert sum(1 for lf in f.leaves if lf[1] == 1) == 64
+
+
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def test_oracle_reproduces_golden_fixtures():
+    """The committed fixtures (tests/golden/gen_golden.py) are the oracle's
+    output; the oracle is deterministic, so it reproduces them exactly."""
+    import json
+    f = np.load(os.path.join(GOLDEN, "rhs_2d_p4.npz"))
+    pr = O.Problem.brick(2, 4, 2, roots=2)
+    assert np.array_equal(pr.rhs(f["y"]), f["du"])
+    f = np.load(os.path.join(GOLDEN, "rhs_ball_p1.npz"))
+    pb = O.Problem.ball(1, 2, roots=(2, 2))
+    assert np.array_equal(pb.rhs(f["y"], 0.25), f["du"])
+    assert np.array_equal(pb.truncate(f["y"]), f["ytrunc"])
+    res = json.load(open(os.path.join(GOLDEN, "results.json")))
+    r = O.wave_errors(2, 4, 2, roots=8)
+    for k, v in res["config1"].items():
+        assert r[k] == pytest.approx(v, rel=1e-14)
