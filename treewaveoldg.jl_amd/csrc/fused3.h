@@ -19,13 +19,31 @@
 
 #pragma once
 
-// Waves per SIMD the fused stage is compiled for.  FP64 needs about 250 VGPRs
-// without spilling (two waves per SIMD); FP32 fits three.
-#ifndef TWOLDG_FUSED_WAVES_F32
-#define TWOLDG_FUSED_WAVES_F32 3
+// Waves per SIMD the fused stage is compiled for: four (128 VGPRs) lets two
+// workgroups of the 20^3-node blocks share a CU, which hides the latency of
+// one's barriers and loads behind the other's work.  FP64 then keeps the stage
+// operands one plane at a time (NB 0).
+#ifndef TWOLDG_FUSED_WAVES_F64
+#define TWOLDG_FUSED_WAVES_F64 4
 #endif
-template <class T>
-constexpr int kFusedWavesPerEU = sizeof(T) == 8 ? 2 : TWOLDG_FUSED_WAVES_F32;
+#ifndef TWOLDG_FUSED_WAVES_F32
+#define TWOLDG_FUSED_WAVES_F32 4
+#endif
+#ifndef TWOLDG_FUSED_NB_F64
+#define TWOLDG_FUSED_NB_F64 0
+#endif
+// Waves per SIMD that the workgroups the LDS admits on one CU can bring (the
+// most loaded SIMD), capped by the above.
+template <class T, int P1, int E>
+constexpr int fused_waves_per_eu() {
+  constexpr int N = P1 * E, NP = N + 2;
+  constexpr long lds = (long)sizeof(T) *
+      (P1 * (NP * NP + 2 * N * NP + N * N) + 4 * P1 * N + P1 * (P1 + 2) + P1);
+  constexpr long wgs = (160L * 1024) / lds;
+  constexpr long by_lds = (wgs * ((N * N + 63) / 64) + 3) / 4;
+  constexpr int want = sizeof(T) == 8 ? TWOLDG_FUSED_WAVES_F64 : TWOLDG_FUSED_WAVES_F32;
+  return by_lds < 1 ? 1 : (by_lds < want ? (int)by_lds : want);
+}
 
 // A workgroup barrier that orders LDS only.  __syncthreads() also fences global
 // memory, which makes every wave wait for its outstanding global loads at each
@@ -58,24 +76,37 @@ struct FusedArgs {
 // Face index f = 2d + s; tangential index t of node (x, y, z):
 // d = 0: y + N z; d = 1: x + N z; d = 2: x + N y.
 
+// The element's operators by value: a kernel argument lives in the constant
+// kernarg segment, so the uniform reads of the z-direction go to scalar
+// registers instead of occupying vector registers.
+template <class T, int P1>
+struct FusedTab {
+  T W[P1][P1 + 2];
+  T mode[P1];
+  T dual[P1];
+};
+
 template <class T, int P1, int E, int MODE>
-__global__ void __launch_bounds__(((P1 * E) * (P1 * E) + 63) / 64 * 64, kFusedWavesPerEU<T>)
-k_fused3(FusedArgs<T> A) {
+__global__ void __launch_bounds__(((P1 * E) * (P1 * E) + 63) / 64 * 64, (fused_waves_per_eu<T, P1, E>()))
+k_fused3(FusedArgs<T> A, FusedTab<T, P1> tb) {
   constexpr int N = P1 * E;
   constexpr int NT = N * N;
+  constexpr int NTH = (NT + 63) / 64 * 64;
   constexpr int PW = P1 + 2;
   constexpr int P = P1 - 1;
   constexpr int NP = N + 2;
   constexpr int SU = NP * NP, SG = N * NP;  // plane strides of U and of GX, GY
+  constexpr int NG = 8 * P1 * N;             // x/y ghost values of one layer
+  constexpr int GPT = (NG + NTH - 1) / NTH;  // per thread
   // One element layer (P1 planes) per phase.  R1 holds the u planes, later the
   // values to truncate; R2 the x-gradients, later the x-truncated values.
-  __shared__ T sW[P1][PW];
-  __shared__ T sMode[P1], sDual[P1];
+  __shared__ T sW[P1 * PW];
+  __shared__ T sMode[P1];
   __shared__ T R1[P1 * SU];
   __shared__ T R2[P1 * SG];
   __shared__ T GY[P1 * SG];
   __shared__ T UN[P1 * NT];
-  __shared__ T EG[8][NT];          // x/y neighbours' face values: u (xl xh yl yh), then p
+  __shared__ T EP[4][P1 * N];  // the x/y neighbours' partial face gradients
   T* U = R1;
   T* VA = R1;
   T* GX = R2;
@@ -85,178 +116,173 @@ k_fused3(FusedArgs<T> A) {
   const int tid = threadIdx.x;
   const bool act = tid < NT;
   const int x = act ? tid % N : 0, y = act ? tid / N : 0;
-  const int64_t nn = (int64_t)N * N * N;
-  const int64_t N2 = (int64_t)N * N;
+  constexpr int N2 = N * N, nn = N2 * N;
 
-  if (tid < P1 * PW) sW[tid / PW][tid % PW] = A.tables->W[tid / PW][tid % PW];
-  if (tid < P1) {
-    sMode[tid] = A.tables->mode[tid];
-    sDual[tid] = A.tables->dual[tid];
-  }
+  if (tid < P1 * PW) sW[tid] = tb.W[tid / PW][tid % PW];
+  if (tid < P1) sMode[tid] = tb.mode[tid];
 
   const T h = (T)A.spacing[b];
   const T rh = T(1) / h;
   const int32_t* nb6 = A.nbr + b * 6;
-  const int64_t col = (int64_t)y * N + x;
-  const T* us = A.ys + b * 2 * nn + col;          // this column's u (stride N2)
+  // the block's arrays: uniform bases, per-thread 32-bit offsets from this
+  // column (stride N2 along z; v at +nn)
+  const int col = y * N + x;
+  const T* us = A.ys + b * 2 * nn;
   const T* vs = us + nn;
-  const T* un_ = A.yn + b * 2 * nn + col;
-  T* acc = A.acc + b * 2 * nn + col;
-  T* yo = A.yo + b * 2 * nn + col;
-  T* lo = A.lo ? A.lo + b * 2 * nn + col : nullptr;
+  const T* un_ = A.yn + b * 2 * nn;
+  T* acc = A.acc + b * 2 * nn;
+  T* yo = A.yo + b * 2 * nn;
+  T* lo = A.lo ? A.lo + b * 2 * nn : nullptr;
+  T* ufo = A.uf_out + b * 6 * N2;
+  T* pfo = A.pf_out + b * 6 * N2;
   const int nxl = nb6[0], nxh = nb6[1], nyl = nb6[2], nyh = nb6[3];
-
-  // the x and y neighbours' face planes, once per block
-  {
-    const int nbs[4] = {nxl, nxh, nyl, nyh};
-    const int opp[4] = {1, 0, 3, 2};  // the neighbour's face that touches ours
-    for (int idx = tid; idx < 8 * NT; idx += blockDim.x) {
-      const int q = idx / NT, t = idx % NT, f = q % 4;
-      const int nbq = nbs[f];
-      if (nbq >= 0) {
-        const T* src = q < 4 ? A.uf_in : A.pf_in;
-        EG[q][t] = src[((int64_t)nbq * 6 + opp[f]) * N2 + t];
-      }
-    }
-  }
-  lds_barrier();
 
   auto load_layer = [&](int e, T* dst) {
 #pragma unroll
-    for (int j = 0; j < P1; ++j) dst[j] = act ? us[(int64_t)(e * P1 + j) * N2] : T(0);
+    for (int j = 0; j < P1; ++j) dst[j] = act ? us[col + (e * P1 + j) * N2] : T(0);
   };
   // the ghost terms of rows 0 and P
-  const T Wlo = sW[0][0], Whi = sW[P][P1 + 1];
+  const T Wlo = tb.W[0][0], Whi = tb.W[P][P1 + 1];
 
-  // ---- z ghosts: u and the gradient on both z faces -------------------------
-  T uc[P1], up[P1];
+  T uc[P1];
   T prev_last = 0, prev_part = 0;  // previous layer: last u, row-P partial sum
   load_layer(0, uc);
-  T ug_lo, ug_hi, gg_lo, gg_hi;
-  {
-    const int nlo = nb6[4], nhi = nb6[5];
-    T ul[P1];
-    load_layer(E - 1, ul);
-    ug_lo = (act && nlo >= 0) ? A.uf_in[((int64_t)nlo * 6 + 5) * N2 + col] : uc[0];
-    ug_hi = (act && nhi >= 0) ? A.uf_in[((int64_t)nhi * 6 + 4) * N2 + col] : ul[P];
-    if (act && nlo >= 0) {
-      gg_lo = (A.pf_in[((int64_t)nlo * 6 + 5) * N2 + col] + Whi * uc[0]) * (T(1) / (T)A.spacing[nlo]);
-    } else {  // mirror: minus the own face gradient (row 0 with ghost = own face)
-      T g0 = Wlo * ug_lo;
-#pragma unroll
-      for (int j = 0; j < P1; ++j) g0 += sW[0][j + 1] * uc[j];
-      gg_lo = -(g0 * rh);
-    }
-    if (act && nhi >= 0) {
-      gg_hi = (A.pf_in[((int64_t)nhi * 6 + 4) * N2 + col] + Wlo * ul[P]) * (T(1) / (T)A.spacing[nhi]);
-    } else {
-      T gN = 0;
-#pragma unroll
-      for (int j = 0; j < P1; ++j) gN += sW[P][j + 1] * ul[j];
-      gN += Whi * ug_hi;
-      gg_hi = -(gN * rh);
-    }
-  }
-  if (E > 1) load_layer(1, up);
 
-  const T rxl = nxl >= 0 ? T(1) / (T)A.spacing[nxl] : rh, rxh = nxh >= 0 ? T(1) / (T)A.spacing[nxh] : rh;
-  const T ryl = nyl >= 0 ? T(1) / (T)A.spacing[nyl] : rh, ryh = nyh >= 0 ? T(1) / (T)A.spacing[nyh] : rh;
   const int ix = x % P1, iy = y % P1;
   const int sx = x - ix - 1, sy = y - iy - 1;  // window starts (owned index)
 
-  T pz_lo = 0, pz_hi = 0;  // partial z-face gradients of the next u
-  T uz0 = 0, uzN = 0;      // the next u on the z faces
-
 #pragma unroll 1
   for (int e = 0; e < E; ++e) {
-    // ---- the layer's global operands: issued first, used in phase (3) ----
-    T ku[P1], yu[P1], yv[P1], au[P1], av[P1];
+    // ---- the layer's x/y ghosts: the neighbours' face values, issued first ----
+    T gv[GPT];
 #pragma unroll
-    for (int i = 0; i < P1; ++i) {
-      ku[i] = yu[i] = yv[i] = au[i] = av[i] = T(0);
-      if (act) {
-        const int64_t iu = (int64_t)(e * P1 + i) * N2, iv = iu + nn;
-        ku[i] = vs[iu];
-        yu[i] = un_[iu];
-        yv[i] = un_[iv];
-        if (MODE >= 2) {
-          au[i] = acc[iu];
-          av[i] = acc[iv];
-        }
+    for (int k = 0; k < GPT; ++k) {
+      const int idx = tid + k * NTH;
+      gv[k] = T(0);
+      if (idx < NG) {
+        const int q = idx / (4 * P1 * N), r = idx % (4 * P1 * N), f = r / (P1 * N),
+                  t = r % (P1 * N);  // t = (plane in layer) * N + tangential index
+        const int nbq = f == 0 ? nxl : (f == 1 ? nxh : (f == 2 ? nyl : nyh));
+        if (nbq >= 0)
+          gv[k] = (q ? A.pf_in : A.uf_in)[((int64_t)nbq * 6 + (f ^ 1)) * N2 + e * P1 * N + t];
       }
     }
-    // the column two layers on, needed when the window slides
-    T un2[P1];
-    if (e + 2 < E) load_layer(e + 2, un2);
 
     // ---- the layer's z-gradients (one node either side) and z-Laplacian ----
     T lapl[P1];
     {
-      T gl[PW];
+      // the z ghosts: the next layer's first u and its row-0 partial sum, or
+      // on the block's z faces the neighbour's face u and face gradient (a
+      // mirror's ghost u is the own face u, its gradient minus the own face
+      // gradient)
+      T before = prev_last, after, gbefore = 0, gafter = 0, next_part = 0;
       if (e == 0) {
-        gl[0] = gg_lo;
-      } else {
-        gl[0] = (prev_part + sW[P][P1 + 1] * uc[0]) * rh;
+        const int nlo = nb6[4];
+        before = (act && nlo >= 0) ? A.uf_in[((int64_t)nlo * 6 + 5) * N2 + col] : uc[0];
+        if (act && nlo >= 0) {
+          gbefore = (A.pf_in[((int64_t)nlo * 6 + 5) * N2 + col] + Whi * uc[0]) * (T(1) / (T)A.spacing[nlo]);
+        } else {
+          T g0 = Wlo * before;
+#pragma unroll
+          for (int j = 0; j < P1; ++j) g0 += tb.W[0][j + 1] * uc[j];
+          gbefore = -(g0 * rh);
+        }
       }
-      const T before = e == 0 ? ug_lo : prev_last;
-      const T after = e == E - 1 ? ug_hi : up[0];
+      if (e < E - 1) {
+        T un2[P1];
+        load_layer(e + 1, un2);
+        after = un2[0];
+#pragma unroll
+        for (int j = 0; j < P1; ++j) next_part += tb.W[0][j + 1] * un2[j];
+      } else {
+        const int nhi = nb6[5];
+        after = (act && nhi >= 0) ? A.uf_in[((int64_t)nhi * 6 + 4) * N2 + col] : uc[P];
+        if (act && nhi >= 0) {
+          gafter = (A.pf_in[((int64_t)nhi * 6 + 4) * N2 + col] + Wlo * uc[P]) * (T(1) / (T)A.spacing[nhi]);
+        } else {
+          T gN = Whi * after;
+#pragma unroll
+          for (int j = 0; j < P1; ++j) gN += tb.W[P][j + 1] * uc[j];
+          gafter = -(gN * rh);
+        }
+      }
+      T gl[PW];
+      gl[0] = e == 0 ? gbefore : (prev_part + Whi * uc[0]) * rh;
 #pragma unroll
       for (int i = 0; i < P1; ++i) {
         T s = 0;
 #pragma unroll
         for (int k = 0; k < PW; ++k) {
           if ((k == 0 && i != 0) || (k == PW - 1 && i != P)) continue;
-          s += sW[i][k] * (k == 0 ? before : (k == PW - 1 ? after : uc[k - 1]));
+          s += tb.W[i][k] * (k == 0 ? before : (k == PW - 1 ? after : uc[k - 1]));
         }
         gl[i + 1] = s * rh;
       }
-      if (e == E - 1) {
-        gl[PW - 1] = gg_hi;
-      } else {
-        T s = sW[0][0] * uc[P];
-#pragma unroll
-        for (int j = 0; j < P1; ++j) s += sW[0][j + 1] * up[j];
-        gl[PW - 1] = s * rh;
-      }
+      gl[PW - 1] = e == E - 1 ? gafter : (Wlo * uc[P] + next_part) * rh;
 #pragma unroll
       for (int i = 0; i < P1; ++i) {
         T s = 0;
 #pragma unroll
         for (int k = 0; k < PW; ++k) {
           if ((k == 0 && i != 0) || (k == PW - 1 && i != P)) continue;
-          s += sW[i][k] * gl[k];
+          s += tb.W[i][k] * gl[k];
         }
         lapl[i] = s;
       }
     }
+    prev_last = uc[P];
+    prev_part = 0;
+#pragma unroll
+    for (int j = 0; j < P1; ++j) prev_part += tb.W[P][j + 1] * uc[j];
 
-    // (1) the layer's u planes and their x/y ghosts
+    // (1) the layer's u planes, and the x/y ghosts: neighbours' u into the
+    // ghost ring of U, their partial gradients into EP; a mirror's ghost u is
+    // the own face value
+#pragma unroll
+    for (int k = 0; k < GPT; ++k) {
+      const int idx = tid + k * NTH;
+      if (idx < NG) {
+        const int q = idx / (4 * P1 * N), r = idx % (4 * P1 * N), f = r / (P1 * N),
+                  t = r % (P1 * N), i = t / N, c = t % N;
+        const int nbq = f == 0 ? nxl : (f == 1 ? nxh : (f == 2 ? nyl : nyh));
+        if (nbq >= 0) {
+          if (q) {
+            EP[f][t] = gv[k];
+          } else {
+            T* Ui = U + i * SU;
+            const int at = f == 0 ? (c + 1) * NP : f == 1 ? (c + 1) * NP + N + 1
+                         : f == 2 ? c + 1 : (N + 1) * NP + c + 1;
+            Ui[at] = gv[k];
+          }
+        }
+      }
+    }
     if (act) {
 #pragma unroll
       for (int i = 0; i < P1; ++i) {
-        const int z = e * P1 + i;
         const T uz = uc[i];
         T* Ui = U + i * SU;
         Ui[(y + 1) * NP + x + 1] = uz;
-        if (x == 0) Ui[(y + 1) * NP] = nxl >= 0 ? EG[0][y + N * z] : uz;
-        if (x == N - 1) Ui[(y + 1) * NP + N + 1] = nxh >= 0 ? EG[1][y + N * z] : uz;
-        if (y == 0) Ui[x + 1] = nyl >= 0 ? EG[2][x + N * z] : uz;
-        if (y == N - 1) Ui[(N + 1) * NP + x + 1] = nyh >= 0 ? EG[3][x + N * z] : uz;
+        if (x == 0 && nxl < 0) Ui[(y + 1) * NP] = uz;
+        if (x == N - 1 && nxh < 0) Ui[(y + 1) * NP + N + 1] = uz;
+        if (y == 0 && nyl < 0) Ui[x + 1] = uz;
+        if (y == N - 1 && nyh < 0) Ui[(N + 1) * NP + x + 1] = uz;
       }
     }
     lds_barrier();
     // (2) g_x and g_y, and the ghosts of the gradients
     if (act) {
-#pragma unroll
+      const T* wx = sW + ix * PW;
+      const T* wy = sW + iy * PW;
+#pragma unroll 1
       for (int i = 0; i < P1; ++i) {
-        const int z = e * P1 + i;
-        const T uz = uc[i];
         const T* Ui = U + i * SU;
+        const T uz = Ui[(y + 1) * NP + x + 1];
         T gx = 0, gy = 0;
 #pragma unroll
         for (int k = 0; k < PW; ++k) {
-          gx += sW[ix][k] * Ui[(y + 1) * NP + (sx + k + 1)];
-          gy += sW[iy][k] * Ui[(sy + k + 1) * NP + x + 1];
+          gx += wx[k] * Ui[(y + 1) * NP + (sx + k + 1)];
+          gy += wy[k] * Ui[(sy + k + 1) * NP + x + 1];
         }
         gx *= rh;
         gy *= rh;
@@ -264,110 +290,137 @@ k_fused3(FusedArgs<T> A) {
         T* GYi = GY + i * SG;
         GXi[y * NP + x + 1] = gx;
         GYi[(y + 1) * N + x] = gy;
-        if (x == 0) GXi[y * NP] = nxl >= 0 ? (EG[4][y + N * z] + Whi * uz) * rxl : -gx;
-        if (x == N - 1) GXi[y * NP + N + 1] = nxh >= 0 ? (EG[5][y + N * z] + Wlo * uz) * rxh : -gx;
-        if (y == 0) GYi[x] = nyl >= 0 ? (EG[6][x + N * z] + Whi * uz) * ryl : -gy;
-        if (y == N - 1) GYi[(N + 1) * N + x] = nyh >= 0 ? (EG[7][x + N * z] + Wlo * uz) * ryh : -gy;
+        if (x == 0)
+          GXi[y * NP] = nxl >= 0 ? (EP[0][i * N + y] + Whi * uz) * (T(1) / (T)A.spacing[nxl]) : -gx;
+        if (x == N - 1)
+          GXi[y * NP + N + 1] = nxh >= 0 ? (EP[1][i * N + y] + Wlo * uz) * (T(1) / (T)A.spacing[nxh]) : -gx;
+        if (y == 0)
+          GYi[x] = nyl >= 0 ? (EP[2][i * N + x] + Whi * uz) * (T(1) / (T)A.spacing[nyl]) : -gy;
+        if (y == N - 1)
+          GYi[(N + 1) * N + x] = nyh >= 0 ? (EP[3][i * N + x] + Wlo * uz) * (T(1) / (T)A.spacing[nyh]) : -gy;
       }
     }
     lds_barrier();
-    // (3) the Laplacian and the stage combination
-    T w[P1];
-#pragma unroll
-    for (int i = 0; i < P1; ++i) {
-      const int z = e * P1 + i;
-      T unext = 0;
-      w[i] = 0;
-      if (act) {
-        const T* GXi = GX + i * SG;
-        const T* GYi = GY + i * SG;
-        T lx = 0, ly = 0;
-#pragma unroll
-        for (int k = 0; k < PW; ++k) {
-          lx += sW[ix][k] * GXi[y * NP + (sx + k + 1)];
-          ly += sW[iy][k] * GYi[(sy + k + 1) * N + x];
+    // (3) the Laplacian and the stage combination; U is dead, VA takes its place
+    {
+      // the stage operands: the whole layer at once where registers allow
+      // (FP32), plane by plane otherwise
+      constexpr int NB = sizeof(T) == 8 ? TWOLDG_FUSED_NB_F64 : P1;
+      T ku[P1], yu[P1], yv[P1], au[P1], av[P1];
+      auto load_ops = [&](int i) {
+        ku[i] = yu[i] = yv[i] = au[i] = av[i] = T(0);
+        if (act) {
+          const int iu = col + (e * P1 + i) * N2, iv = iu + nn;
+          ku[i] = vs[iu];
+          if (MODE != 4) {
+            yu[i] = un_[iu];
+            yv[i] = un_[iv];
+          }
+          if (MODE >= 2) {
+            au[i] = acc[iu];
+            av[i] = acc[iv];
+          }
         }
-        const T lap = ((T(0) + lx) + ly) + lapl[i];
-        const T kv = lap * rh;
-        const int64_t iu = (int64_t)z * N2, iv = iu + nn;
-        if (MODE == 1) {
-          acc[iu] = ku[i];
-          acc[iv] = kv;
-          unext = yu[i] + A.a * ku[i];
-          w[i] = yv[i] + A.a * kv;
-        } else if (MODE == 2) {
-          acc[iu] = au[i] + T(2) * ku[i];
-          acc[iv] = av[i] + T(2) * kv;
-          unext = yu[i] + A.a * ku[i];
-          w[i] = yv[i] + A.a * kv;
-        } else if (MODE == 3) {
-          unext = yu[i] + A.a * (au[i] + ku[i]);
-          w[i] = yv[i] + A.a * (av[i] + kv);
-        } else {  // compensated: u by Kahan now, v's increment truncated first
-          const T du = A.a * (au[i] + ku[i]);
-          const T yk = du - lo[iu], hu = yu[i], tu = hu + yk;
-          lo[iu] = (tu - hu) - yk;
-          unext = tu;
-          w[i] = A.a * (av[i] + kv);
+      };
+#pragma unroll
+      for (int i = 0; i < NB; ++i) load_ops(i);
+      T pzl = 0, pzh = 0;  // partial z-face gradients of the next u
+      const T* wx = sW + ix * PW;
+      const T* wy = sW + iy * PW;
+#pragma unroll
+      for (int i = 0; i < P1; ++i) {
+        const int z = e * P1 + i;
+        if (i + NB < P1) load_ops(i + NB);
+        T unext = 0;
+        if (act) {
+          const int iu = col + z * N2, iv = iu + nn;
+          const T* GXi = GX + i * SG;
+          const T* GYi = GY + i * SG;
+          T lx = 0, ly = 0;
+#pragma unroll
+          for (int k = 0; k < PW; ++k) {
+            lx += wx[k] * GXi[y * NP + (sx + k + 1)];
+            ly += wy[k] * GYi[(sy + k + 1) * N + x];
+          }
+          const T lap = ((T(0) + lx) + ly) + lapl[i];
+          const T kv = lap * rh;
+          T w;
+          if (MODE == 1) {
+            acc[iu] = ku[i];
+            acc[iv] = kv;
+            unext = yu[i] + A.a * ku[i];
+            w = yv[i] + A.a * kv;
+          } else if (MODE == 2) {
+            acc[iu] = au[i] + T(2) * ku[i];
+            acc[iv] = av[i] + T(2) * kv;
+            unext = yu[i] + A.a * ku[i];
+            w = yv[i] + A.a * kv;
+          } else if (MODE == 3) {
+            unext = yu[i] + A.a * (au[i] + ku[i]);
+            w = yv[i] + A.a * (av[i] + kv);
+          } else {  // compensated: u by Kahan now, v's increment truncated first
+            const T du = A.a * (au[i] + ku[i]);
+            const T yk = du - lo[iu], hu = un_[iu], tu = hu + yk;
+            lo[iu] = (tu - hu) - yk;
+            unext = tu;
+            w = A.a * (av[i] + kv);
+          }
+          yo[iu] = unext;
+          UN[i * NT + tid] = unext;
+          VA[i * NT + tid] = w;
         }
-        yo[iu] = unext;
-        UN[i * NT + tid] = unext;
+        // faces of the next u along z
+        if (e == 0) pzl += tb.W[0][i + 1] * unext;
+        if (e == E - 1) pzh += tb.W[P][i + 1] * unext;
+        if (act && z == 0) ufo[4 * N2 + col] = unext;
+        if (act && z == N - 1) ufo[5 * N2 + col] = unext;
       }
-      // faces of the next u along z
-      if (e == 0) pz_lo += sW[0][i + 1] * unext;
-      if (e == E - 1) pz_hi += sW[P][i + 1] * unext;
-      if (z == 0) uz0 = unext;
-      if (z == N - 1) uzN = unext;
-    }
-    lds_barrier();
-    // U is dead: the values to truncate go to VA (aliasing R1)
-    if (act) {
-#pragma unroll
-      for (int i = 0; i < P1; ++i) VA[i * NT + tid] = w[i];
+      if (act && e == 0) pfo[4 * N2 + col] = pzl;
+      if (act && e == E - 1) pfo[5 * N2 + col] = pzh;
     }
     lds_barrier();
     // (4) faces of the next u along x and y; truncation along x (GX is dead: VB)
     if (act) {
-#pragma unroll
+#pragma unroll 1
       for (int i = 0; i < P1; ++i) {
         const int z = e * P1 + i;
         const T* UNi = UN + i * NT;
         if (x == 0) {
           T s = 0;
 #pragma unroll
-          for (int j = 0; j < P1; ++j) s += sW[0][j + 1] * UNi[y * N + j];
-          A.uf_out[(b * 6 + 0) * N2 + y + N * z] = UNi[tid];
-          A.pf_out[(b * 6 + 0) * N2 + y + N * z] = s;
+          for (int j = 0; j < P1; ++j) s += tb.W[0][j + 1] * UNi[y * N + j];
+          ufo[0 * N2 + y + N * z] = UNi[tid];
+          pfo[0 * N2 + y + N * z] = s;
         }
         if (x == N - 1) {
           T s = 0;
 #pragma unroll
-          for (int j = 0; j < P1; ++j) s += sW[P][j + 1] * UNi[y * N + N - P1 + j];
-          A.uf_out[(b * 6 + 1) * N2 + y + N * z] = UNi[tid];
-          A.pf_out[(b * 6 + 1) * N2 + y + N * z] = s;
+          for (int j = 0; j < P1; ++j) s += tb.W[P][j + 1] * UNi[y * N + N - P1 + j];
+          ufo[1 * N2 + y + N * z] = UNi[tid];
+          pfo[1 * N2 + y + N * z] = s;
         }
         if (y == 0) {
           T s = 0;
 #pragma unroll
-          for (int j = 0; j < P1; ++j) s += sW[0][j + 1] * UNi[j * N + x];
-          A.uf_out[(b * 6 + 2) * N2 + x + N * z] = UNi[tid];
-          A.pf_out[(b * 6 + 2) * N2 + x + N * z] = s;
+          for (int j = 0; j < P1; ++j) s += tb.W[0][j + 1] * UNi[j * N + x];
+          ufo[2 * N2 + x + N * z] = UNi[tid];
+          pfo[2 * N2 + x + N * z] = s;
         }
         if (y == N - 1) {
           T s = 0;
 #pragma unroll
-          for (int j = 0; j < P1; ++j) s += sW[P][j + 1] * UNi[(N - P1 + j) * N + x];
-          A.uf_out[(b * 6 + 3) * N2 + x + N * z] = UNi[tid];
-          A.pf_out[(b * 6 + 3) * N2 + x + N * z] = s;
+          for (int j = 0; j < P1; ++j) s += tb.W[P][j + 1] * UNi[(N - P1 + j) * N + x];
+          ufo[3 * N2 + x + N * z] = UNi[tid];
+          pfo[3 * N2 + x + N * z] = s;
         }
         const T* VAi = VA + i * NT;
         if (A.truncate) {
           T c = 0;
 #pragma unroll
-          for (int j = 0; j < P1; ++j) c += sDual[j] * VAi[y * N + (x - ix) + j];
-          VB[i * NT + tid] = w[i] - sMode[ix] * c;
+          for (int j = 0; j < P1; ++j) c += tb.dual[j] * VAi[y * N + (x - ix) + j];
+          VB[i * NT + tid] = VAi[tid] - sMode[ix] * c;
         } else {
-          VB[i * NT + tid] = w[i];
+          VB[i * NT + tid] = VAi[tid];
         }
       }
     }
@@ -382,22 +435,22 @@ k_fused3(FusedArgs<T> A) {
         if (A.truncate) {
           T c = 0;
 #pragma unroll
-          for (int j = 0; j < P1; ++j) c += sDual[j] * VBi[((y - iy) + j) * N + x];
+          for (int j = 0; j < P1; ++j) c += tb.dual[j] * VBi[((y - iy) + j) * N + x];
           wl[i] -= sMode[iy] * c;
         }
       }
       if (A.truncate) {
         T c = 0;
 #pragma unroll
-        for (int j = 0; j < P1; ++j) c += sDual[j] * wl[j];
+        for (int j = 0; j < P1; ++j) c += tb.dual[j] * wl[j];
 #pragma unroll
-        for (int j = 0; j < P1; ++j) wl[j] -= sMode[j] * c;
+        for (int j = 0; j < P1; ++j) wl[j] -= tb.mode[j] * c;
       }
 #pragma unroll
       for (int j = 0; j < P1; ++j) {
-        const int64_t iv = (int64_t)(e * P1 + j) * N2 + nn;
+        const int iv = col + (e * P1 + j) * N2 + nn;
         if (MODE == 4) {
-          const T yk = wl[j] - lo[iv], hv = yv[j], tv = hv + yk;
+          const T yk = wl[j] - lo[iv], hv = un_[iv], tv = hv + yk;
           lo[iv] = (tv - hv) - yk;
           yo[iv] = tv;
         } else {
@@ -406,21 +459,7 @@ k_fused3(FusedArgs<T> A) {
       }
     }
     // slide the column window
-    prev_last = uc[P];
-    prev_part = 0;
-#pragma unroll
-    for (int j = 0; j < P1; ++j) prev_part += sW[P][j + 1] * uc[j];
-#pragma unroll
-    for (int j = 0; j < P1; ++j) {
-      uc[j] = up[j];
-      up[j] = un2[j];
-    }
-  }
-  if (act) {
-    A.uf_out[(b * 6 + 4) * N2 + col] = uz0;
-    A.uf_out[(b * 6 + 5) * N2 + col] = uzN;
-    A.pf_out[(b * 6 + 4) * N2 + col] = pz_lo;
-    A.pf_out[(b * 6 + 5) * N2 + col] = pz_hi;
+    if (e + 1 < E) load_layer(e + 1, uc);
   }
 }
 
