@@ -37,8 +37,7 @@
 template <class T, int P1, int E>
 constexpr int fused_waves_per_eu() {
   constexpr int N = P1 * E, NP = N + 2;
-  constexpr long lds = (long)sizeof(T) *
-      (P1 * (NP * NP + 2 * N * NP + N * N) + 4 * P1 * N + P1 * (P1 + 2) + P1);
+  constexpr long lds = (long)sizeof(T) * (P1 * (NP * NP + 3 * N * N) + 4 * P1 * N);
   constexpr long wgs = (160L * 1024) / lds;
   constexpr long by_lds = (wgs * ((N * N + 63) / 64) + 3) / 4;
   constexpr int want = sizeof(T) == 8 ? TWOLDG_FUSED_WAVES_F64 : TWOLDG_FUSED_WAVES_F32;
@@ -76,56 +75,72 @@ struct FusedArgs {
 // Face index f = 2d + s; tangential index t of node (x, y, z):
 // d = 0: y + N z; d = 1: x + N z; d = 2: x + N y.
 
-// The element's operators by value: a kernel argument lives in the constant
-// kernarg segment, so the uniform reads of the z-direction go to scalar
-// registers instead of occupying vector registers.
+// The element's operators as compile-time constants: after unrolling, every
+// coefficient is an immediate that the compiler materializes in scalar
+// registers where it is used (and skips where it is zero), so the tables take
+// neither vector registers nor LDS.
 template <class T, int P1>
 struct FusedTab {
   T W[P1][P1 + 2];
   T mode[P1];
   T dual[P1];
 };
+template <class T, int P1>
+constexpr FusedTab<T, P1> fused_tab() {
+  FusedTab<T, P1> r{};
+  const twoldg::ElementTables& et = twoldg::kElementTables[P1 - 2];
+  for (int j = 0; j < P1; ++j) {
+    for (int k = 0; k < P1 + 2; ++k) r.W[j][k] = (T)et.stencil[j][k];
+    r.mode[j] = (T)et.topmode[j];
+    r.dual[j] = (T)et.topdual[j];
+  }
+  return r;
+}
 
 template <class T, int P1, int E, int MODE>
 __global__ void __launch_bounds__(((P1 * E) * (P1 * E) + 63) / 64 * 64, (fused_waves_per_eu<T, P1, E>()))
-k_fused3(FusedArgs<T> A, FusedTab<T, P1> tb) {
+k_fused3(FusedArgs<T> A) {
+  constexpr FusedTab<T, P1> tb = fused_tab<T, P1>();
   constexpr int N = P1 * E;
-  constexpr int NT = N * N;
+  constexpr int NT = N * N;  // = E * N * P1: one element line per thread and role
   constexpr int NTH = (NT + 63) / 64 * 64;
   constexpr int PW = P1 + 2;
   constexpr int P = P1 - 1;
   constexpr int NP = N + 2;
-  constexpr int SU = NP * NP, SG = N * NP;  // plane strides of U and of GX, GY
+  constexpr int SU = NP * NP;                // plane stride of U (with its ghost ring)
   constexpr int NG = 8 * P1 * N;             // x/y ghost values of one layer
   constexpr int GPT = (NG + NTH - 1) / NTH;  // per thread
-  // One element layer (P1 planes) per phase.  R1 holds the u planes, later the
-  // values to truncate; R2 the x-gradients, later the x-truncated values.
-  __shared__ T sW[P1 * PW];
-  __shared__ T sMode[P1];
-  __shared__ T R1[P1 * SU];
-  __shared__ T R2[P1 * SG];
-  __shared__ T GY[P1 * SG];
-  __shared__ T UN[P1 * NT];
+  constexpr int N2 = N * N, nn = N2 * N;
+  // One element layer (P1 planes) at a time.  RA holds the u planes with their
+  // ghost ring, later the next u; LZ the z-Laplacian sums, LX the x-Laplacian
+  // sums, VB the values to truncate.
+  __shared__ T RA[P1 * SU];
+  __shared__ T LZ[P1 * NT];
+  __shared__ T LX[P1 * NT];
+  __shared__ T VB[P1 * NT];
   __shared__ T EP[4][P1 * N];  // the x/y neighbours' partial face gradients
-  T* U = R1;
-  T* VA = R1;
-  T* GX = R2;
-  T* VB = R2;
+  T* U = RA;
+  T* UN = RA;
 
   const int64_t b = blockIdx.x;
   const int tid = threadIdx.x;
   const bool act = tid < NT;
+  // column role: (x, y); x-line role: element ex of row yx in plane px;
+  // y-line role: element ey of column xy in plane py
   const int x = act ? tid % N : 0, y = act ? tid / N : 0;
-  constexpr int N2 = N * N, nn = N2 * N;
-
-  if (tid < P1 * PW) sW[tid] = tb.W[tid / PW][tid % PW];
-  if (tid < P1) sMode[tid] = tb.mode[tid];
+  // The line roles' indices are formed in each phase from an opaque copy of
+  // the thread index: hoisted out of the layer loop they would hold registers
+  // for its whole length.
+  auto opaque = [](int v) {
+    asm volatile("" : "+v"(v));
+    return v;
+  };
 
   const T h = (T)A.spacing[b];
   const T rh = T(1) / h;
   const int32_t* nb6 = A.nbr + b * 6;
-  // the block's arrays: uniform bases, per-thread 32-bit offsets from this
-  // column (stride N2 along z; v at +nn)
+  // the block's arrays: uniform bases, per-thread 32-bit offsets (z stride N2,
+  // v at +nn)
   const int col = y * N + x;
   const T* us = A.ys + b * 2 * nn;
   const T* vs = us + nn;
@@ -144,12 +159,67 @@ k_fused3(FusedArgs<T> A, FusedTab<T, P1> tb) {
   // the ghost terms of rows 0 and P
   const T Wlo = tb.W[0][0], Whi = tb.W[P][P1 + 1];
 
+  // The derivative along a line of one element (x or y): the gradient at the
+  // element's P1 nodes and at the two nodes beyond it (the neighbour elements'
+  // face nodes, or the block face's ghost gradient), then the Laplacian sums.
+  // rd(c) reads u at ring index c (node c - 1) along the line.  gface_lo/hi
+  // give the ghost gradient at a block face from the own face value and
+  // gradient.
+  auto line_laplacian = [&](int el, auto rd, auto gface_lo, auto gface_hi, T* lap) {
+    T uw[PW];
+#pragma unroll
+    for (int k = 0; k < PW; ++k) uw[k] = rd(el * P1 + k);
+    T g[PW];
+#pragma unroll
+    for (int j = 0; j < P1; ++j) {
+      T s = 0;
+#pragma unroll
+      for (int k = 0; k < PW; ++k) {
+        if ((k == 0 && j != 0) || (k == PW - 1 && j != P)) continue;
+        s += tb.W[j][k] * uw[k];
+      }
+      g[j + 1] = s * rh;
+    }
+    if (el > 0) {  // row P of the element below: its nodes, then our first node
+      T s = 0;
+#pragma unroll
+      for (int k = 1; k < PW - 1; ++k) s += tb.W[P][k] * rd((el - 1) * P1 + k);
+      s += tb.W[P][PW - 1] * uw[1];
+      g[0] = s * rh;
+    } else {
+      g[0] = gface_lo(uw[1], g[1]);
+    }
+    if (el < E - 1) {  // row 0 of the element above: our last node, then its nodes
+      T s = tb.W[0][0] * uw[P1];
+#pragma unroll
+      for (int k = 1; k < PW - 1; ++k) s += tb.W[0][k] * rd((el + 1) * P1 + k);
+      g[PW - 1] = s * rh;
+    } else {
+      g[PW - 1] = gface_hi(uw[P1], g[P1]);
+    }
+#pragma unroll
+    for (int j = 0; j < P1; ++j) {
+      T s = 0;
+#pragma unroll
+      for (int k = 0; k < PW; ++k) {
+        if ((k == 0 && j != 0) || (k == PW - 1 && j != P)) continue;
+        s += tb.W[j][k] * g[k];
+      }
+      lap[j] = s;
+    }
+  };
+  // one element line's truncation (evolution.jl:238-264 along one dimension)
+  auto truncate_line = [&](T* v) {
+    T c = 0;
+#pragma unroll
+    for (int j = 0; j < P1; ++j) c += tb.dual[j] * v[j];
+#pragma unroll
+    for (int j = 0; j < P1; ++j) v[j] -= tb.mode[j] * c;
+  };
+
   T uc[P1];
   T prev_last = 0, prev_part = 0;  // previous layer: last u, row-P partial sum
   load_layer(0, uc);
-
-  const int ix = x % P1, iy = y % P1;
-  const int sx = x - ix - 1, sy = y - iy - 1;  // window starts (owned index)
 
 #pragma unroll 1
   for (int e = 0; e < E; ++e) {
@@ -168,8 +238,7 @@ k_fused3(FusedArgs<T> A, FusedTab<T, P1> tb) {
       }
     }
 
-    // ---- the layer's z-gradients (one node either side) and z-Laplacian ----
-    T lapl[P1];
+    // ---- (1) columns: z-gradients (one node either side) and z-Laplacian ----
     {
       // the z ghosts: the next layer's first u and its row-0 partial sum, or
       // on the block's z faces the neighbour's face u and face gradient (a
@@ -219,15 +288,17 @@ k_fused3(FusedArgs<T> A, FusedTab<T, P1> tb) {
         gl[i + 1] = s * rh;
       }
       gl[PW - 1] = e == E - 1 ? gafter : (Wlo * uc[P] + next_part) * rh;
+      if (act) {
 #pragma unroll
-      for (int i = 0; i < P1; ++i) {
-        T s = 0;
+        for (int i = 0; i < P1; ++i) {
+          T s = 0;
 #pragma unroll
-        for (int k = 0; k < PW; ++k) {
-          if ((k == 0 && i != 0) || (k == PW - 1 && i != P)) continue;
-          s += tb.W[i][k] * gl[k];
+          for (int k = 0; k < PW; ++k) {
+            if ((k == 0 && i != 0) || (k == PW - 1 && i != P)) continue;
+            s += tb.W[i][k] * gl[k];
+          }
+          LZ[i * NT + col] = s;
         }
-        lapl[i] = s;
       }
     }
     prev_last = uc[P];
@@ -235,9 +306,9 @@ k_fused3(FusedArgs<T> A, FusedTab<T, P1> tb) {
 #pragma unroll
     for (int j = 0; j < P1; ++j) prev_part += tb.W[P][j + 1] * uc[j];
 
-    // (1) the layer's u planes, and the x/y ghosts: neighbours' u into the
-    // ghost ring of U, their partial gradients into EP; a mirror's ghost u is
-    // the own face value
+    // the layer's u planes, and the x/y ghosts: neighbours' u into the ghost
+    // ring of U, their partial gradients into EP; a mirror's ghost u is the
+    // own face value
 #pragma unroll
     for (int k = 0; k < GPT; ++k) {
       const int idx = tid + k * NTH;
@@ -270,188 +341,170 @@ k_fused3(FusedArgs<T> A, FusedTab<T, P1> tb) {
       }
     }
     lds_barrier();
-    // (2) g_x and g_y, and the ghosts of the gradients
+    // ---- (2) element lines: x-Laplacian sums to LX; y-Laplacian sums kept ----
+    T ly[P1];
     if (act) {
-      const T* wx = sW + ix * PW;
-      const T* wy = sW + iy * PW;
-#pragma unroll 1
-      for (int i = 0; i < P1; ++i) {
-        const T* Ui = U + i * SU;
-        const T uz = Ui[(y + 1) * NP + x + 1];
-        T gx = 0, gy = 0;
+      const int t2 = opaque(tid);
+      const int ex = t2 % E, yx = (t2 / E) % N, px = t2 / (E * N);
+      const int xy = t2 % N, ey = (t2 / N) % E, py = t2 / (N * E);
+      {
+        const T* Ur = U + px * SU + (yx + 1) * NP;
+        T lx[P1];
+        line_laplacian(
+            ex, [&](int c) { return Ur[c]; },
+            [&](T uf, T gf) { return nxl >= 0 ? (EP[0][px * N + yx] + Whi * uf) * (T(1) / (T)A.spacing[nxl]) : -gf; },
+            [&](T uf, T gf) { return nxh >= 0 ? (EP[1][px * N + yx] + Wlo * uf) * (T(1) / (T)A.spacing[nxh]) : -gf; },
+            lx);
 #pragma unroll
-        for (int k = 0; k < PW; ++k) {
-          gx += wx[k] * Ui[(y + 1) * NP + (sx + k + 1)];
-          gy += wy[k] * Ui[(sy + k + 1) * NP + x + 1];
-        }
-        gx *= rh;
-        gy *= rh;
-        T* GXi = GX + i * SG;
-        T* GYi = GY + i * SG;
-        GXi[y * NP + x + 1] = gx;
-        GYi[(y + 1) * N + x] = gy;
-        if (x == 0)
-          GXi[y * NP] = nxl >= 0 ? (EP[0][i * N + y] + Whi * uz) * (T(1) / (T)A.spacing[nxl]) : -gx;
-        if (x == N - 1)
-          GXi[y * NP + N + 1] = nxh >= 0 ? (EP[1][i * N + y] + Wlo * uz) * (T(1) / (T)A.spacing[nxh]) : -gx;
-        if (y == 0)
-          GYi[x] = nyl >= 0 ? (EP[2][i * N + x] + Whi * uz) * (T(1) / (T)A.spacing[nyl]) : -gy;
-        if (y == N - 1)
-          GYi[(N + 1) * N + x] = nyh >= 0 ? (EP[3][i * N + x] + Wlo * uz) * (T(1) / (T)A.spacing[nyh]) : -gy;
+        for (int j = 0; j < P1; ++j) LX[px * NT + yx * N + ex * P1 + j] = lx[j];
       }
+      const T* Uc = U + py * SU + xy + 1;
+      line_laplacian(
+          ey, [&](int c) { return Uc[c * NP]; },
+          [&](T uf, T gf) { return nyl >= 0 ? (EP[2][py * N + xy] + Whi * uf) * (T(1) / (T)A.spacing[nyl]) : -gf; },
+          [&](T uf, T gf) { return nyh >= 0 ? (EP[3][py * N + xy] + Wlo * uf) * (T(1) / (T)A.spacing[nyh]) : -gf; },
+          ly);
     }
     lds_barrier();
-    // (3) the Laplacian and the stage combination; U is dead, VA takes its place
-    {
-      // the stage operands: the whole layer at once where registers allow
-      // (FP32), plane by plane otherwise
-      constexpr int NB = sizeof(T) == 8 ? TWOLDG_FUSED_NB_F64 : P1;
-      T ku[P1], yu[P1], yv[P1], au[P1], av[P1];
-      auto load_ops = [&](int i) {
-        ku[i] = yu[i] = yv[i] = au[i] = av[i] = T(0);
-        if (act) {
-          const int iu = col + (e * P1 + i) * N2, iv = iu + nn;
-          ku[i] = vs[iu];
-          if (MODE != 4) {
-            yu[i] = un_[iu];
-            yv[i] = un_[iv];
-          }
-          if (MODE >= 2) {
-            au[i] = acc[iu];
-            av[i] = acc[iv];
-          }
-        }
+    // ---- (3) y lines: the Laplacian, the stage combination, the next u, and
+    // v's truncation along y; U is dead, UN takes its place ----
+    if (act) {
+      const int t3 = opaque(tid);
+      const int xy = t3 % N, ey = (t3 / N) % E, py = t3 / (N * E);
+      const int z = e * P1 + py;
+      const int base = xy + ey * P1 * N + z * N2;  // node j of the line at base + j N
+      T w[P1];
+      T plo = 0, phi = 0;  // partial y-face gradients of the next u
+      auto ops = [&](int j, T& ku, T& yu, T& yv, T& au, T& av) {
+        const int iu = base + j * N, iv = iu + nn;
+        ku = vs[iu];
+        yu = un_[iu];                          // compensated form: the high word
+        yv = MODE == 4 ? lo[iu] : un_[iv];     // compensated form: the low word
+        au = MODE >= 2 ? acc[iu] : T(0);
+        av = MODE >= 2 ? acc[iv] : T(0);
       };
+      // the stage operands one node ahead where registers allow
+      constexpr bool ahead = sizeof(T) == 4 || TWOLDG_FUSED_NB_F64 > 0;
+      T ku, yu, yv, au, av;
+      if (ahead) ops(0, ku, yu, yv, au, av);
 #pragma unroll
-      for (int i = 0; i < NB; ++i) load_ops(i);
-      T pzl = 0, pzh = 0;  // partial z-face gradients of the next u
-      const T* wx = sW + ix * PW;
-      const T* wy = sW + iy * PW;
-#pragma unroll
-      for (int i = 0; i < P1; ++i) {
-        const int z = e * P1 + i;
-        if (i + NB < P1) load_ops(i + NB);
-        T unext = 0;
-        if (act) {
-          const int iu = col + z * N2, iv = iu + nn;
-          const T* GXi = GX + i * SG;
-          const T* GYi = GY + i * SG;
-          T lx = 0, ly = 0;
-#pragma unroll
-          for (int k = 0; k < PW; ++k) {
-            lx += wx[k] * GXi[y * NP + (sx + k + 1)];
-            ly += wy[k] * GYi[(sy + k + 1) * N + x];
-          }
-          const T lap = ((T(0) + lx) + ly) + lapl[i];
-          const T kv = lap * rh;
-          T w;
-          if (MODE == 1) {
-            acc[iu] = ku[i];
-            acc[iv] = kv;
-            unext = yu[i] + A.a * ku[i];
-            w = yv[i] + A.a * kv;
-          } else if (MODE == 2) {
-            acc[iu] = au[i] + T(2) * ku[i];
-            acc[iv] = av[i] + T(2) * kv;
-            unext = yu[i] + A.a * ku[i];
-            w = yv[i] + A.a * kv;
-          } else if (MODE == 3) {
-            unext = yu[i] + A.a * (au[i] + ku[i]);
-            w = yv[i] + A.a * (av[i] + kv);
-          } else {  // compensated: u by Kahan now, v's increment truncated first
-            const T du = A.a * (au[i] + ku[i]);
-            const T yk = du - lo[iu], hu = un_[iu], tu = hu + yk;
-            lo[iu] = (tu - hu) - yk;
-            unext = tu;
-            w = A.a * (av[i] + kv);
-          }
-          yo[iu] = unext;
-          UN[i * NT + tid] = unext;
-          VA[i * NT + tid] = w;
+      for (int j = 0; j < P1; ++j) {
+        T ku1 = 0, yu1 = 0, yv1 = 0, au1 = 0, av1 = 0;
+        if (!ahead) ops(j, ku, yu, yv, au, av);
+        if (ahead && j + 1 < P1) ops(j + 1, ku1, yu1, yv1, au1, av1);
+        const int iu = base + j * N, iv = iu + nn;
+        const int l = py * NT + (ey * P1 + j) * N + xy;
+        const T lap = ((T(0) + LX[l]) + ly[j]) + LZ[l];
+        const T kv = lap * rh;
+        T unext;
+        if (MODE == 1) {
+          acc[iu] = ku;
+          acc[iv] = kv;
+          unext = yu + A.a * ku;
+          w[j] = yv + A.a * kv;
+        } else if (MODE == 2) {
+          acc[iu] = au + T(2) * ku;
+          acc[iv] = av + T(2) * kv;
+          unext = yu + A.a * ku;
+          w[j] = yv + A.a * kv;
+        } else if (MODE == 3) {
+          unext = yu + A.a * (au + ku);
+          w[j] = yv + A.a * (av + kv);
+        } else {  // compensated: u by Kahan now, v's increment truncated first
+          const T du = A.a * (au + ku);
+          const T yk = du - yv, tu = yu + yk;
+          lo[iu] = (tu - yu) - yk;
+          unext = tu;
+          w[j] = A.a * (av + kv);
         }
-        // faces of the next u along z
-        if (e == 0) pzl += tb.W[0][i + 1] * unext;
-        if (e == E - 1) pzh += tb.W[P][i + 1] * unext;
-        if (act && z == 0) ufo[4 * N2 + col] = unext;
-        if (act && z == N - 1) ufo[5 * N2 + col] = unext;
+        yo[iu] = unext;
+        UN[l] = unext;
+        // faces of the next u along y
+        if (ey == 0) {
+          plo += tb.W[0][j + 1] * unext;
+          if (j == 0) ufo[2 * N2 + xy + N * z] = unext;
+        }
+        if (ey == E - 1) {
+          phi += tb.W[P][j + 1] * unext;
+          if (j == P) ufo[3 * N2 + xy + N * z] = unext;
+        }
+        if (ahead) {
+          ku = ku1; yu = yu1; yv = yv1; au = au1; av = av1;
+        }
       }
-      if (act && e == 0) pfo[4 * N2 + col] = pzl;
-      if (act && e == E - 1) pfo[5 * N2 + col] = pzh;
+      if (ey == 0) pfo[2 * N2 + xy + N * z] = plo;
+      if (ey == E - 1) pfo[3 * N2 + xy + N * z] = phi;
+      if (A.truncate) truncate_line(w);
+#pragma unroll
+      for (int j = 0; j < P1; ++j) VB[py * NT + (ey * P1 + j) * N + xy] = w[j];
     }
     lds_barrier();
-    // (4) faces of the next u along x and y; truncation along x (GX is dead: VB)
+    // ---- (4) x lines: v's truncation along x, faces of the next u along x;
+    // columns: faces of the next u along z ----
+    T lov[P1], hv[P1];  // the compensated form's v operands
     if (act) {
-#pragma unroll 1
-      for (int i = 0; i < P1; ++i) {
-        const int z = e * P1 + i;
-        const T* UNi = UN + i * NT;
-        if (x == 0) {
-          T s = 0;
+      const int t4 = opaque(tid);
+      const int ex = t4 % E, yx = (t4 / E) % N, px = t4 / (E * N);
+      const int z = e * P1 + px;
+      const int l = px * NT + yx * N + ex * P1;
+      if (A.truncate) {
+        T w[P1];
 #pragma unroll
-          for (int j = 0; j < P1; ++j) s += tb.W[0][j + 1] * UNi[y * N + j];
-          ufo[0 * N2 + y + N * z] = UNi[tid];
-          pfo[0 * N2 + y + N * z] = s;
-        }
-        if (x == N - 1) {
-          T s = 0;
+        for (int j = 0; j < P1; ++j) w[j] = VB[l + j];
+        truncate_line(w);
 #pragma unroll
-          for (int j = 0; j < P1; ++j) s += tb.W[P][j + 1] * UNi[y * N + N - P1 + j];
-          ufo[1 * N2 + y + N * z] = UNi[tid];
-          pfo[1 * N2 + y + N * z] = s;
-        }
-        if (y == 0) {
-          T s = 0;
+        for (int j = 0; j < P1; ++j) VB[l + j] = w[j];
+      }
+      if (ex == 0) {
+        T s = 0;
 #pragma unroll
-          for (int j = 0; j < P1; ++j) s += tb.W[0][j + 1] * UNi[j * N + x];
-          ufo[2 * N2 + x + N * z] = UNi[tid];
-          pfo[2 * N2 + x + N * z] = s;
-        }
-        if (y == N - 1) {
-          T s = 0;
+        for (int j = 0; j < P1; ++j) s += tb.W[0][j + 1] * UN[l + j];
+        ufo[0 * N2 + yx + N * z] = UN[l];
+        pfo[0 * N2 + yx + N * z] = s;
+      }
+      if (ex == E - 1) {
+        T s = 0;
 #pragma unroll
-          for (int j = 0; j < P1; ++j) s += tb.W[P][j + 1] * UNi[(N - P1 + j) * N + x];
-          ufo[3 * N2 + x + N * z] = UNi[tid];
-          pfo[3 * N2 + x + N * z] = s;
-        }
-        const T* VAi = VA + i * NT;
-        if (A.truncate) {
-          T c = 0;
+        for (int j = 0; j < P1; ++j) s += tb.W[P][j + 1] * UN[l + j];
+        ufo[1 * N2 + yx + N * z] = UN[l + P];
+        pfo[1 * N2 + yx + N * z] = s;
+      }
+      if (e == 0) {
+        T s = 0;
 #pragma unroll
-          for (int j = 0; j < P1; ++j) c += tb.dual[j] * VAi[y * N + (x - ix) + j];
-          VB[i * NT + tid] = VAi[tid] - sMode[ix] * c;
-        } else {
-          VB[i * NT + tid] = VAi[tid];
+        for (int i = 0; i < P1; ++i) s += tb.W[0][i + 1] * UN[i * NT + col];
+        ufo[4 * N2 + col] = UN[col];
+        pfo[4 * N2 + col] = s;
+      }
+      if (e == E - 1) {
+        T s = 0;
+#pragma unroll
+        for (int i = 0; i < P1; ++i) s += tb.W[P][i + 1] * UN[i * NT + col];
+        ufo[5 * N2 + col] = UN[P * NT + col];
+        pfo[5 * N2 + col] = s;
+      }
+      if (MODE == 4) {
+#pragma unroll
+        for (int j = 0; j < P1; ++j) {
+          const int iv = col + (e * P1 + j) * N2 + nn;
+          lov[j] = lo[iv];
+          hv[j] = un_[iv];
         }
       }
     }
     lds_barrier();
-    // (5) truncation along y, then along z in registers; v out
+    // ---- (5) columns: truncation along z in registers; v out ----
     if (act) {
       T wl[P1];
 #pragma unroll
-      for (int i = 0; i < P1; ++i) {
-        const T* VBi = VB + i * NT;
-        wl[i] = VBi[tid];
-        if (A.truncate) {
-          T c = 0;
-#pragma unroll
-          for (int j = 0; j < P1; ++j) c += tb.dual[j] * VBi[((y - iy) + j) * N + x];
-          wl[i] -= sMode[iy] * c;
-        }
-      }
-      if (A.truncate) {
-        T c = 0;
-#pragma unroll
-        for (int j = 0; j < P1; ++j) c += tb.dual[j] * wl[j];
-#pragma unroll
-        for (int j = 0; j < P1; ++j) wl[j] -= tb.mode[j] * c;
-      }
+      for (int i = 0; i < P1; ++i) wl[i] = VB[i * NT + col];
+      if (A.truncate) truncate_line(wl);
 #pragma unroll
       for (int j = 0; j < P1; ++j) {
         const int iv = col + (e * P1 + j) * N2 + nn;
         if (MODE == 4) {
-          const T yk = wl[j] - lo[iv], hv = un_[iv], tv = hv + yk;
-          lo[iv] = (tv - hv) - yk;
+          const T yk = wl[j] - lov[j], tv = hv[j] + yk;
+          lo[iv] = (tv - hv[j]) - yk;
           yo[iv] = tv;
         } else {
           yo[iv] = wl[j];

@@ -1022,15 +1022,6 @@ int fused_steps(twoldg_problem* p, double dt, int64_t nsteps) {
   A.spacing = p->spacing;
   A.tables = (const Tables<T>*)p->tables;
   A.truncate = p->truncate ? 1 : 0;
-  FusedTab<T, P1> tb;
-  {
-    const twoldg::ElementTables& et = twoldg::kElementTables[P1 - 2];
-    for (int i = 0; i < P1; ++i) {
-      for (int j = 0; j < P1 + 2; ++j) tb.W[i][j] = (T)et.stencil[i][j];
-      tb.mode[i] = (T)et.topmode[i];
-      tb.dual[i] = (T)et.topdual[i];
-    }
-  }
   auto io = [&](int in) {
     A.uf_in = (const T*)p->uf[in];
     A.pf_in = (const T*)p->pf[in];
@@ -1043,26 +1034,26 @@ int fused_steps(twoldg_problem* p, double dt, int64_t nsteps) {
     A.a = (T)(dt / 2);
     io(0);
     tmark(p);
-    hipLaunchKernelGGL((k_fused3<T, P1, E, 1>), dim3((unsigned)nb), dim3(threads), 0, p->stream, A, tb);
+    hipLaunchKernelGGL((k_fused3<T, P1, E, 1>), dim3((unsigned)nb), dim3(threads), 0, p->stream, A);
     tmark(p);
     A.ys = (T*)p->ys;
     io(1);
     tmark(p);
-    hipLaunchKernelGGL((k_fused3<T, P1, E, 2>), dim3((unsigned)nb), dim3(threads), 0, p->stream, A, tb);
+    hipLaunchKernelGGL((k_fused3<T, P1, E, 2>), dim3((unsigned)nb), dim3(threads), 0, p->stream, A);
     tmark(p);
     A.a = (T)dt;
     io(0);
     tmark(p);
-    hipLaunchKernelGGL((k_fused3<T, P1, E, 2>), dim3((unsigned)nb), dim3(threads), 0, p->stream, A, tb);
+    hipLaunchKernelGGL((k_fused3<T, P1, E, 2>), dim3((unsigned)nb), dim3(threads), 0, p->stream, A);
     tmark(p);
     A.a = (T)(dt / 6);
     A.yo = yn;
     io(1);
     tmark(p);
     if (p->precision == TWOLDG_F32C)
-      hipLaunchKernelGGL((k_fused3<T, P1, E, 4>), dim3((unsigned)nb), dim3(threads), 0, p->stream, A, tb);
+      hipLaunchKernelGGL((k_fused3<T, P1, E, 4>), dim3((unsigned)nb), dim3(threads), 0, p->stream, A);
     else
-      hipLaunchKernelGGL((k_fused3<T, P1, E, 3>), dim3((unsigned)nb), dim3(threads), 0, p->stream, A, tb);
+      hipLaunchKernelGGL((k_fused3<T, P1, E, 3>), dim3((unsigned)nb), dim3(threads), 0, p->stream, A);
     tmark(p);
   }
   HIP_TRY(hipGetLastError());
