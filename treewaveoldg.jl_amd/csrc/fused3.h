@@ -36,12 +36,39 @@
 // most loaded SIMD), capped by the above.
 template <class T, int P1, int E>
 constexpr int fused_waves_per_eu() {
-  constexpr int N = P1 * E, NP = N + 2;
-  constexpr long lds = (long)sizeof(T) * (P1 * (NP * NP + 3 * N * N) + 4 * P1 * N);
+  constexpr int N = P1 * E;
+  constexpr long lds = (long)sizeof(T) * (4 * P1 * N * N + 8 * P1 * N);
   constexpr long wgs = (160L * 1024) / lds;
   constexpr long by_lds = (wgs * ((N * N + 63) / 64) + 3) / 4;
   constexpr int want = sizeof(T) == 8 ? TWOLDG_FUSED_WAVES_F64 : TWOLDG_FUSED_WAVES_F32;
   return by_lds < 1 ? 1 : (by_lds < want ? (int)by_lds : want);
+}
+
+// Whether the next element layer's column stays in registers from its first
+// read (FP32) or is read again (FP64, whose registers are spent).
+#ifndef TWOLDG_FUSED_CARRY_F64
+#define TWOLDG_FUSED_CARRY_F64 0
+#endif
+#ifndef TWOLDG_FUSED_CARRY_F32
+#define TWOLDG_FUSED_CARRY_F32 1
+#endif
+#ifndef TWOLDG_FUSED_NT
+#define TWOLDG_FUSED_NT 0
+#endif
+// Streaming access for the values a stage touches once (the stage operands and
+// its outputs): marked nontemporal, they leave the L2 to the data that is read
+// twice (the next element layer of u, the face buffers).
+template <class T>
+__device__ __forceinline__ T ld_once(const T* p) {
+  if (TWOLDG_FUSED_NT) return __builtin_nontemporal_load(p);
+  return *p;
+}
+template <class T>
+__device__ __forceinline__ void st_once(T* p, T v) {
+  if (TWOLDG_FUSED_NT)
+    __builtin_nontemporal_store(v, p);
+  else
+    *p = v;
 }
 
 // A workgroup barrier that orders LDS only.  __syncthreads() also fences global
@@ -106,19 +133,20 @@ k_fused3(FusedArgs<T> A) {
   constexpr int NTH = (NT + 63) / 64 * 64;
   constexpr int PW = P1 + 2;
   constexpr int P = P1 - 1;
-  constexpr int NP = N + 2;
-  constexpr int SU = NP * NP;                // plane stride of U (with its ghost ring)
   constexpr int NG = 8 * P1 * N;             // x/y ghost values of one layer
   constexpr int GPT = (NG + NTH - 1) / NTH;  // per thread
   constexpr int N2 = N * N, nn = N2 * N;
-  // One element layer (P1 planes) at a time.  RA holds the u planes with their
-  // ghost ring, later the next u; LZ the z-Laplacian sums, LX the x-Laplacian
-  // sums, VB the values to truncate.
-  __shared__ T RA[P1 * SU];
+  // One element layer (P1 planes) at a time, each plane row-major with row
+  // stride N (conflict-free banks for the line roles).  RA holds the u planes,
+  // later the next u; LZ the z-Laplacian sums, LX the x-Laplacian sums, VB the
+  // values to truncate; GH the x/y ghosts of the layer: per block face (x low,
+  // x high, y low, y high), the ghost u, then the neighbour's partial face
+  // gradient, at [plane * N + tangential index].
+  __shared__ T RA[P1 * NT];
   __shared__ T LZ[P1 * NT];
   __shared__ T LX[P1 * NT];
   __shared__ T VB[P1 * NT];
-  __shared__ T EP[4][P1 * N];  // the x/y neighbours' partial face gradients
+  __shared__ T GH[8][P1 * N];
   T* U = RA;
   T* UN = RA;
 
@@ -162,13 +190,16 @@ k_fused3(FusedArgs<T> A) {
   // The derivative along a line of one element (x or y): the gradient at the
   // element's P1 nodes and at the two nodes beyond it (the neighbour elements'
   // face nodes, or the block face's ghost gradient), then the Laplacian sums.
-  // rd(c) reads u at ring index c (node c - 1) along the line.  gface_lo/hi
-  // give the ghost gradient at a block face from the own face value and
-  // gradient.
-  auto line_laplacian = [&](int el, auto rd, auto gface_lo, auto gface_hi, T* lap) {
+  // rd(c) reads u at node c - 1 along the line (c = 1..N); ulo/uhi give the
+  // ghost u beyond the block's faces, gface_lo/hi the ghost gradient there from
+  // the own face value and gradient.
+  auto line_laplacian = [&](int el, auto rd, auto ulo, auto uhi, auto gface_lo, auto gface_hi,
+                            T* lap) {
     T uw[PW];
+    uw[0] = el == 0 ? ulo() : rd(el * P1);
 #pragma unroll
-    for (int k = 0; k < PW; ++k) uw[k] = rd(el * P1 + k);
+    for (int k = 1; k < PW - 1; ++k) uw[k] = rd(el * P1 + k);
+    uw[PW - 1] = el == E - 1 ? uhi() : rd(el * P1 + PW - 1);
     T g[PW];
 #pragma unroll
     for (int j = 0; j < P1; ++j) {
@@ -239,6 +270,7 @@ k_fused3(FusedArgs<T> A) {
     }
 
     // ---- (1) columns: z-gradients (one node either side) and z-Laplacian ----
+    T un2[P1];  // the next layer's column
     {
       // the z ghosts: the next layer's first u and its row-0 partial sum, or
       // on the block's z faces the neighbour's face u and face gradient (a
@@ -258,7 +290,6 @@ k_fused3(FusedArgs<T> A) {
         }
       }
       if (e < E - 1) {
-        T un2[P1];
         load_layer(e + 1, un2);
         after = un2[0];
 #pragma unroll
@@ -306,38 +337,27 @@ k_fused3(FusedArgs<T> A) {
 #pragma unroll
     for (int j = 0; j < P1; ++j) prev_part += tb.W[P][j + 1] * uc[j];
 
-    // the layer's u planes, and the x/y ghosts: neighbours' u into the ghost
-    // ring of U, their partial gradients into EP; a mirror's ghost u is the
-    // own face value
+    // the layer's u planes, and the x/y ghosts: the neighbours' face values
+    // into GH; a mirror's ghost u is the own face value
 #pragma unroll
     for (int k = 0; k < GPT; ++k) {
       const int idx = tid + k * NTH;
       if (idx < NG) {
         const int q = idx / (4 * P1 * N), r = idx % (4 * P1 * N), f = r / (P1 * N),
-                  t = r % (P1 * N), i = t / N, c = t % N;
+                  t = r % (P1 * N);
         const int nbq = f == 0 ? nxl : (f == 1 ? nxh : (f == 2 ? nyl : nyh));
-        if (nbq >= 0) {
-          if (q) {
-            EP[f][t] = gv[k];
-          } else {
-            T* Ui = U + i * SU;
-            const int at = f == 0 ? (c + 1) * NP : f == 1 ? (c + 1) * NP + N + 1
-                         : f == 2 ? c + 1 : (N + 1) * NP + c + 1;
-            Ui[at] = gv[k];
-          }
-        }
+        if (nbq >= 0) GH[4 * q + f][t] = gv[k];
       }
     }
     if (act) {
 #pragma unroll
       for (int i = 0; i < P1; ++i) {
         const T uz = uc[i];
-        T* Ui = U + i * SU;
-        Ui[(y + 1) * NP + x + 1] = uz;
-        if (x == 0 && nxl < 0) Ui[(y + 1) * NP] = uz;
-        if (x == N - 1 && nxh < 0) Ui[(y + 1) * NP + N + 1] = uz;
-        if (y == 0 && nyl < 0) Ui[x + 1] = uz;
-        if (y == N - 1 && nyh < 0) Ui[(N + 1) * NP + x + 1] = uz;
+        U[i * NT + col] = uz;
+        if (x == 0 && nxl < 0) GH[0][i * N + y] = uz;
+        if (x == N - 1 && nxh < 0) GH[1][i * N + y] = uz;
+        if (y == 0 && nyl < 0) GH[2][i * N + x] = uz;
+        if (y == N - 1 && nyh < 0) GH[3][i * N + x] = uz;
       }
     }
     lds_barrier();
@@ -348,21 +368,23 @@ k_fused3(FusedArgs<T> A) {
       const int ex = t2 % E, yx = (t2 / E) % N, px = t2 / (E * N);
       const int xy = t2 % N, ey = (t2 / N) % E, py = t2 / (N * E);
       {
-        const T* Ur = U + px * SU + (yx + 1) * NP;
+        const T* Ur = U + px * NT + yx * N - 1;
+        const int g = px * N + yx;
         T lx[P1];
         line_laplacian(
-            ex, [&](int c) { return Ur[c]; },
-            [&](T uf, T gf) { return nxl >= 0 ? (EP[0][px * N + yx] + Whi * uf) * (T(1) / (T)A.spacing[nxl]) : -gf; },
-            [&](T uf, T gf) { return nxh >= 0 ? (EP[1][px * N + yx] + Wlo * uf) * (T(1) / (T)A.spacing[nxh]) : -gf; },
+            ex, [&](int c) { return Ur[c]; }, [&]() { return GH[0][g]; }, [&]() { return GH[1][g]; },
+            [&](T uf, T gf) { return nxl >= 0 ? (GH[4][g] + Whi * uf) * (T(1) / (T)A.spacing[nxl]) : -gf; },
+            [&](T uf, T gf) { return nxh >= 0 ? (GH[5][g] + Wlo * uf) * (T(1) / (T)A.spacing[nxh]) : -gf; },
             lx);
 #pragma unroll
         for (int j = 0; j < P1; ++j) LX[px * NT + yx * N + ex * P1 + j] = lx[j];
       }
-      const T* Uc = U + py * SU + xy + 1;
+      const T* Uc = U + py * NT - N + xy;
+      const int g = py * N + xy;
       line_laplacian(
-          ey, [&](int c) { return Uc[c * NP]; },
-          [&](T uf, T gf) { return nyl >= 0 ? (EP[2][py * N + xy] + Whi * uf) * (T(1) / (T)A.spacing[nyl]) : -gf; },
-          [&](T uf, T gf) { return nyh >= 0 ? (EP[3][py * N + xy] + Wlo * uf) * (T(1) / (T)A.spacing[nyh]) : -gf; },
+          ey, [&](int c) { return Uc[c * N]; }, [&]() { return GH[2][g]; }, [&]() { return GH[3][g]; },
+          [&](T uf, T gf) { return nyl >= 0 ? (GH[6][g] + Whi * uf) * (T(1) / (T)A.spacing[nyl]) : -gf; },
+          [&](T uf, T gf) { return nyh >= 0 ? (GH[7][g] + Wlo * uf) * (T(1) / (T)A.spacing[nyh]) : -gf; },
           ly);
     }
     lds_barrier();
@@ -377,11 +399,11 @@ k_fused3(FusedArgs<T> A) {
       T plo = 0, phi = 0;  // partial y-face gradients of the next u
       auto ops = [&](int j, T& ku, T& yu, T& yv, T& au, T& av) {
         const int iu = base + j * N, iv = iu + nn;
-        ku = vs[iu];
-        yu = un_[iu];                          // compensated form: the high word
-        yv = MODE == 4 ? lo[iu] : un_[iv];     // compensated form: the low word
-        au = MODE >= 2 ? acc[iu] : T(0);
-        av = MODE >= 2 ? acc[iv] : T(0);
+        ku = ld_once(vs + iu);
+        yu = ld_once(un_ + iu);                          // compensated form: the high word
+        yv = MODE == 4 ? ld_once(lo + iu) : ld_once(un_ + iv);  // compensated form: the low word
+        au = MODE >= 2 ? ld_once(acc + iu) : T(0);
+        av = MODE >= 2 ? ld_once(acc + iv) : T(0);
       };
       // the stage operands one node ahead where registers allow
       constexpr bool ahead = sizeof(T) == 4 || TWOLDG_FUSED_NB_F64 > 0;
@@ -398,13 +420,13 @@ k_fused3(FusedArgs<T> A) {
         const T kv = lap * rh;
         T unext;
         if (MODE == 1) {
-          acc[iu] = ku;
-          acc[iv] = kv;
+          st_once(acc + iu, ku);
+          st_once(acc + iv, kv);
           unext = yu + A.a * ku;
           w[j] = yv + A.a * kv;
         } else if (MODE == 2) {
-          acc[iu] = au + T(2) * ku;
-          acc[iv] = av + T(2) * kv;
+          st_once(acc + iu, au + T(2) * ku);
+          st_once(acc + iv, av + T(2) * kv);
           unext = yu + A.a * ku;
           w[j] = yv + A.a * kv;
         } else if (MODE == 3) {
@@ -413,11 +435,11 @@ k_fused3(FusedArgs<T> A) {
         } else {  // compensated: u by Kahan now, v's increment truncated first
           const T du = A.a * (au + ku);
           const T yk = du - yv, tu = yu + yk;
-          lo[iu] = (tu - yu) - yk;
+          st_once(lo + iu, (tu - yu) - yk);
           unext = tu;
           w[j] = A.a * (av + kv);
         }
-        yo[iu] = unext;
+        st_once(yo + iu, unext);
         UN[l] = unext;
         // faces of the next u along y
         if (ey == 0) {
@@ -487,8 +509,8 @@ k_fused3(FusedArgs<T> A) {
 #pragma unroll
         for (int j = 0; j < P1; ++j) {
           const int iv = col + (e * P1 + j) * N2 + nn;
-          lov[j] = lo[iv];
-          hv[j] = un_[iv];
+          lov[j] = ld_once(lo + iv);
+          hv[j] = ld_once(un_ + iv);
         }
       }
     }
@@ -504,15 +526,22 @@ k_fused3(FusedArgs<T> A) {
         const int iv = col + (e * P1 + j) * N2 + nn;
         if (MODE == 4) {
           const T yk = wl[j] - lov[j], tv = hv[j] + yk;
-          lo[iv] = (tv - hv[j]) - yk;
-          yo[iv] = tv;
+          st_once(lo + iv, (tv - hv[j]) - yk);
+          st_once(yo + iv, tv);
         } else {
-          yo[iv] = wl[j];
+          st_once(yo + iv, wl[j]);
         }
       }
     }
     // slide the column window
-    if (e + 1 < E) load_layer(e + 1, uc);
+    if (e + 1 < E) {
+      if (sizeof(T) == 8 ? TWOLDG_FUSED_CARRY_F64 : TWOLDG_FUSED_CARRY_F32) {
+#pragma unroll
+        for (int j = 0; j < P1; ++j) uc[j] = un2[j];
+      } else {
+        load_layer(e + 1, uc);
+      }
+    }
   }
 }
 
