@@ -34,6 +34,24 @@ VALUES_PER_NODE_UPDATE = {"f64": 34, "f32c": 38}
 BYTES_PER_VALUE = {"f64": 8, "f32c": 4}
 
 
+# HBM bytes per launch of the stage kernel on the default workload, measured with
+# rocprofv3 (FETCH_SIZE and WRITE_SIZE, scaled by a device copy's reading of the
+# same counters); the summary holding them sits beside this file.
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_fused3_periodic320_p4e4.txt")
+
+
+def measured_traffic(precision):
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                w = line.split()
+                if len(w) == 3 and w[0] == "traffic_bytes_per_launch" and w[1] == precision:
+                    return int(w[2])
+    except OSError:
+        pass
+    return None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -260,7 +278,8 @@ def main():
     if main_res["achieved_gbs"]:
         roofline = dict(bound="hbm", achieved=round(main_res["achieved_gbs"], 1),
                         peak=PEAK_HBM_GBS, unit="GB/s",
-                        frac=round(main_res["achieved_gbs"] / PEAK_HBM_GBS, 4), traffic=None,
+                        frac=round(main_res["achieved_gbs"] / PEAK_HBM_GBS, 4),
+                        traffic=measured_traffic(prec),
                         kernel="fused RK4 stage (k_fused3)",
                         avg_launch_ms=round(launch_ms, 4),
                         alg_bytes_per_launch=round(main_res["bytes_per_launch"]))
