@@ -190,6 +190,74 @@ def test_fused_f32c_close_to_f64(case):
         assert np.linalg.norm(integ.u - y) <= 1e-5 * np.linalg.norm(y)
 
 
+FUSED_REFINED = [
+    dict(D=3, P=3, E=2, roots=4, regions=[((0.25, 0.75),) * 3]),
+    dict(D=3, P=2, E=2, roots=4, reflecting=True, regions=[((0.0, 0.5),) * 3]),
+    dict(D=3, P=4, E=4, roots=2, regions=[((0.5, 1.0),) * 3]),
+    dict(D=3, P=3, E=2, roots=8, reflecting=True, regions=O.nested_cubes(3, 1 / 8, 3)),
+]
+
+
+@pytest.mark.parametrize("case", FUSED_REFINED, ids=_id)
+def test_fused_refined_matches_oracle(case):
+    """The ghosted fused stages (coarse-fine faces: face ghosts formed by two
+    face-sized launches from the face buffers) against the oracle's RK4 and the
+    generic exchange path."""
+    prob, ref = _pair(case["D"], case["P"], case["E"], case["roots"],
+                      reflecting=case.get("reflecting", False), regions=case["regions"])
+    forest = T.wave_forest(case["D"], case["P"], case["E"], roots=case["roots"],
+                           reflecting=case.get("reflecting", False), refined=list(case["regions"]))
+    generic = T.WaveProblem(forest, T.Element(case["P"]), generic=True)
+    assert prob.info()["fused"] and not generic.info()["fused"]
+    refl = case.get("reflecting", False)
+    y0 = ref.truncate(ref.fill_exact(1 if refl else 0, 0.0, (1.0, 2 if refl else 1)))
+    y0 = ref.truncate(y0 + 0.01 * np.random.default_rng(11).standard_normal(y0.size))
+    dt = ref.timestep(0.5)
+    n = 5
+    y = ref.rk4(y0, 0.0, dt, n)
+    for p in (prob, generic):
+        integ = T.wave_integrator(p, y0, 0.0, n * dt, dt)
+        integ.solve()
+        assert np.linalg.norm(integ.u - y) <= 1e-13 * np.linalg.norm(y)
+
+
+def test_fused_large_forests_each_first_problem():
+    """Large forests of small blocks (one wave per workgroup, several
+    workgroups per CU), each forest's first problem and a rebuilt one, one step
+    against the oracle.  Guards the table uploads: tables copied by the DMA
+    path into reused device memory were read stale by the first problem."""
+    for P, E, roots, regions, refl in [(3, 2, 11, (), False),
+                                       (3, 2, 8, O.nested_cubes(3, 1 / 8, 3), True)]:
+        forest = T.wave_forest(3, P, E, roots=roots, reflecting=refl,
+                               refined=list(regions) if regions else False)
+        ref = O.Problem.brick(3, P, E, roots=roots, reflecting=refl, regions=regions)
+        y0 = ref.truncate(ref.fill_exact(1 if refl else 0, 0.0, (1.0, 2 if refl else 1)))
+        dt = ref.timestep(0.5)
+        y = ref.rk4(y0, 0.0, dt, 1)
+        for rep in range(2):
+            prob = T.WaveProblem(forest, T.Element(P))
+            assert prob.info()["fused"]
+            integ = T.wave_integrator(prob, y0, 0.0, dt, dt)
+            integ.solve()
+            assert np.linalg.norm(integ.u - y) <= 1e-13 * np.linalg.norm(y), (roots, rep)
+            prob.close()
+
+
+@pytest.mark.parametrize("case", FUSED_REFINED[:2], ids=_id)
+def test_fused_refined_f32c_close_to_f64(case):
+    prob, ref = _pair(case["D"], case["P"], case["E"], case["roots"],
+                      reflecting=case.get("reflecting", False), regions=case["regions"],
+                      precision="f32c")
+    assert prob.info()["fused"]
+    y0 = ref.truncate(ref.fill_exact(1 if case.get("reflecting") else 0, 0.0, (1.0, 1)))
+    dt = ref.timestep(0.5)
+    n = 5
+    y = ref.rk4(y0, 0.0, dt, n)
+    integ = T.wave_integrator(prob, y0, 0.0, n * dt, dt)
+    integ.solve()
+    assert np.linalg.norm(integ.u - y) <= 1e-5 * np.linalg.norm(y)
+
+
 def _ball(P=3, E=2, roots=(2, 2), precision="f64", truncate=True):
     from twoldg.geometry import sevenpatch_forest
     forest, geo = sevenpatch_forest(P, E, roots=roots)

@@ -40,6 +40,24 @@ int balloc(twoldg_batch* B, void** p, size_t bytes) {
   return 0;
 }
 
+// Host tables into device memory through pinned staging and a copy kernel on
+// the batch's stream (the copy path left stale values visible to the stage
+// kernels; see h2d in twoldg_impl.h).
+int bh2d(twoldg_batch* B, void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return 0;
+  const size_t words = (bytes + 3) / 4;
+  void* pinned = nullptr;
+  HIP_TRY(hipHostMalloc(&pinned, words * 4, hipHostMallocDefault));
+  std::memcpy(pinned, src, bytes);
+  if (bytes % 4) std::memset((char*)pinned + bytes, 0, words * 4 - bytes);
+  hipLaunchKernelGGL(k_copy_words, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, B->stream,
+                     (int64_t)words, (const uint32_t*)pinned, (uint32_t*)dst);
+  hipError_t e = hipStreamSynchronize(B->stream);
+  (void)hipHostFree(pinned);
+  if (e != hipSuccess) return fail(TWOLDG_EDEVICE, std::string("batch table upload: ") + hipGetErrorString(e));
+  return 0;
+}
+
 template <class T>
 int batch_tables(twoldg_batch* B) {
   std::vector<Tables<T>> tb(8);
@@ -55,7 +73,7 @@ int batch_tables(twoldg_batch* B) {
   }
   int rc = balloc(B, &B->tables, tb.size() * sizeof(Tables<T>));
   if (rc) return rc;
-  HIP_TRY(hipMemcpy(B->tables, tb.data(), tb.size() * sizeof(Tables<T>), hipMemcpyHostToDevice));
+  if ((rc = bh2d(B, B->tables, tb.data(), tb.size() * sizeof(Tables<T>)))) return rc;
   return 0;
 }
 
@@ -145,7 +163,7 @@ int autotune(twoldg_batch* B, const std::vector<int32_t>& Nh) {
       int tile[10];
       for (int i = 0; i < 10; ++i) tile[i] = t;
       std::vector<BatchItem> it = make_items(B, Nh, tile, P1);
-      HIP_TRY(hipMemcpy(scratch, it.data(), it.size() * sizeof(BatchItem), hipMemcpyHostToDevice));
+      if ((rc = bh2d(B, scratch, it.data(), it.size() * sizeof(BatchItem)))) return rc;
       // stage 2 writes the accumulator and the stage buffer only
       launch_stage<T, 2>(B, scratch, (int64_t)it.size(), (const T*)B->yn, (T*)B->ys, 0.5, 0);
       HIP_TRY(hipEventRecord(B->ev0, B->stream));
@@ -219,13 +237,13 @@ int batch_build(twoldg_batch* B, const twoldg_desc* const* d, int64_t n, const d
       (rc = balloc(B, (void**)&B->Ns, nb * 4)) || (rc = balloc(B, (void**)&B->spacing, nb * 8)) ||
       (rc = balloc(B, (void**)&B->dt, nb * 8)))
     return rc;
-  HIP_TRY(hipMemcpy(B->off, offh.data(), nb * 8, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(B->foff, foffh.data(), nb * 8, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(B->nbr, nbrh.data(), nb * 16, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(B->P1s, B->P1s_host.data(), nb * 4, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(B->Ns, Nh.data(), nb * 4, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(B->spacing, hh.data(), nb * 8, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(B->dt, dth.data(), nb * 8, hipMemcpyHostToDevice));
+  if ((rc = bh2d(B, B->off, offh.data(), nb * 8))) return rc;
+  if ((rc = bh2d(B, B->foff, foffh.data(), nb * 8))) return rc;
+  if ((rc = bh2d(B, B->nbr, nbrh.data(), nb * 16))) return rc;
+  if ((rc = bh2d(B, B->P1s, B->P1s_host.data(), nb * 4))) return rc;
+  if ((rc = bh2d(B, B->Ns, Nh.data(), nb * 4))) return rc;
+  if ((rc = bh2d(B, B->spacing, hh.data(), nb * 8))) return rc;
+  if ((rc = bh2d(B, B->dt, dth.data(), nb * 8))) return rc;
   if ((rc = batch_tables<T>(B))) return rc;
   for (void** p : {&B->yn, &B->ys, &B->acc})
     if ((rc = balloc(B, p, total * sizeof(T)))) return rc;
@@ -234,13 +252,14 @@ int batch_build(twoldg_batch* B, const twoldg_desc* const* d, int64_t n, const d
     if ((rc = balloc(B, &B->uf[k], fsum * sizeof(T))) || (rc = balloc(B, &B->pf[k], fsum * sizeof(T))))
       return rc;
   for (void* p : {B->yn, B->ys, B->acc, B->lo, B->uf[0], B->uf[1], B->pf[0], B->pf[1]})
-    if (p) HIP_TRY(hipMemset(p, 0, (p == B->uf[0] || p == B->uf[1] || p == B->pf[0] || p == B->pf[1] ? fsum : total) * sizeof(T)));
+    if (p)
+      HIP_TRY(hipMemsetAsync(p, 0, (p == B->uf[0] || p == B->uf[1] || p == B->pf[0] || p == B->pf[1] ? fsum : total) * sizeof(T), B->stream));
   // tiles per order, then the work list, then the step graph
   if ((rc = autotune<T>(B, Nh))) return rc;
   std::vector<BatchItem> it = make_items(B, Nh, B->tiles, 0);
   B->nitems = (int64_t)it.size();
   if ((rc = balloc(B, (void**)&B->items, it.size() * sizeof(BatchItem)))) return rc;
-  HIP_TRY(hipMemcpy(B->items, it.data(), it.size() * sizeof(BatchItem), hipMemcpyHostToDevice));
+  if ((rc = bh2d(B, B->items, it.data(), it.size() * sizeof(BatchItem)))) return rc;
   HIP_TRY(hipStreamSynchronize(B->stream));
   HIP_TRY(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
   enqueue_step<T>(B);
@@ -337,19 +356,19 @@ int twoldg_batch_destroy(twoldg_batch* B) {
 int twoldg_batch_upload(twoldg_batch* B, int64_t i, const double* host) {
   if (!B || !host || i < 0 || i >= B->nprob) return fail(TWOLDG_EINVAL, "bad batch upload");
   const int64_t n = B->prob_len[i], o = B->prob_off[i];
+  // staged in pinned host memory and converted by a kernel that reads it there
   double* tmp = nullptr;
-  HIP_TRY(hipMalloc((void**)&tmp, n * 8));
-  hipError_t e = hipMemcpyAsync(tmp, host, n * 8, hipMemcpyHostToDevice, B->stream);
-  if (e == hipSuccess) {
-    if (B->precision == TWOLDG_F64)
-      hipLaunchKernelGGL((k_from_double<double>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
-                         B->stream, n, tmp, (double*)B->yn + o, (double*)nullptr);
-    else
-      hipLaunchKernelGGL((k_from_double<float>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
-                         B->stream, n, tmp, (float*)B->yn + o, B->lo ? (float*)B->lo + o : nullptr);
-    e = hipStreamSynchronize(B->stream);
-  }
-  (void)hipFree(tmp);
+  HIP_TRY(hipHostMalloc((void**)&tmp, n * 8, hipHostMallocDefault));
+  std::memcpy(tmp, host, n * 8);
+  if (B->precision == TWOLDG_F64)
+    hipLaunchKernelGGL((k_from_double<double>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       B->stream, n, (const double*)tmp, (double*)B->yn + o, (double*)nullptr);
+  else
+    hipLaunchKernelGGL((k_from_double<float>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       B->stream, n, (const double*)tmp, (float*)B->yn + o,
+                       B->lo ? (float*)B->lo + o : nullptr);
+  hipError_t e = hipStreamSynchronize(B->stream);
+  (void)hipHostFree(tmp);
   if (e != hipSuccess) return fail(TWOLDG_EDEVICE, std::string("batch upload: ") + hipGetErrorString(e));
   return 0;
 }

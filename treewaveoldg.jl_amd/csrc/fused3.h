@@ -92,7 +92,9 @@ struct FusedArgs {
   const T* pf_in;    //              block-local part of the face gradient
   T* uf_out;         // faces of yo
   T* pf_out;
-  const int32_t* nbr;       // (nb, 6) same-level neighbour, or -1 for a mirror
+  const int32_t* nbr;       // (nb, 6) same-level neighbour, -1 a mirror, -2 coarse-fine
+  const T* gu;              // complete ghosts of the coarse-fine faces (u, gradient),
+  const T* gg;              //   (nb, 6, N^2), written by k_face_ghost_u/_g
   const double* spacing;    // (nb)
   const Tables<T>* tables;
   T a;               // stage coefficient (dt/2, dt or dt/6)
@@ -124,7 +126,10 @@ constexpr FusedTab<T, P1> fused_tab() {
   return r;
 }
 
-template <class T, int P1, int E, int MODE>
+// Face ghosts are formed here from a same-level neighbour's face buffers (nbr
+// >= 0) or from the own face on a mirror (-1); with GHOST, a coarse-fine face
+// (-2) takes them complete from k_face_ghost_u/_g.
+template <class T, int P1, int E, int MODE, bool GHOST>
 __global__ void __launch_bounds__(((P1 * E) * (P1 * E) + 63) / 64 * 64, (fused_waves_per_eu<T, P1, E>()))
 k_fused3(FusedArgs<T> A) {
   constexpr FusedTab<T, P1> tb = fused_tab<T, P1>();
@@ -179,6 +184,7 @@ k_fused3(FusedArgs<T> A) {
   T* ufo = A.uf_out + b * 6 * N2;
   T* pfo = A.pf_out + b * 6 * N2;
   const int nxl = nb6[0], nxh = nb6[1], nyl = nb6[2], nyh = nb6[3];
+  constexpr int CF = -2;  // a coarse-fine face
 
   auto load_layer = [&](int e, T* dst) {
 #pragma unroll
@@ -266,6 +272,8 @@ k_fused3(FusedArgs<T> A) {
         const int nbq = f == 0 ? nxl : (f == 1 ? nxh : (f == 2 ? nyl : nyh));
         if (nbq >= 0)
           gv[k] = (q ? A.pf_in : A.uf_in)[((int64_t)nbq * 6 + (f ^ 1)) * N2 + e * P1 * N + t];
+        else if (GHOST && nbq == CF)
+          gv[k] = (q ? A.gg : A.gu)[((int64_t)b * 6 + f) * N2 + e * P1 * N + t];
       }
     }
 
@@ -277,7 +285,10 @@ k_fused3(FusedArgs<T> A) {
       // mirror's ghost u is the own face u, its gradient minus the own face
       // gradient)
       T before = prev_last, after, gbefore = 0, gafter = 0, next_part = 0;
-      if (e == 0) {
+      if (GHOST && e == 0 && nb6[4] == CF) {
+        before = act ? A.gu[((int64_t)b * 6 + 4) * N2 + col] : T(0);
+        gbefore = act ? A.gg[((int64_t)b * 6 + 4) * N2 + col] : T(0);
+      } else if (e == 0) {
         const int nlo = nb6[4];
         before = (act && nlo >= 0) ? A.uf_in[((int64_t)nlo * 6 + 5) * N2 + col] : uc[0];
         if (act && nlo >= 0) {
@@ -294,6 +305,9 @@ k_fused3(FusedArgs<T> A) {
         after = un2[0];
 #pragma unroll
         for (int j = 0; j < P1; ++j) next_part += tb.W[0][j + 1] * un2[j];
+      } else if (GHOST && nb6[5] == CF) {
+        after = act ? A.gu[((int64_t)b * 6 + 5) * N2 + col] : T(0);
+        gafter = act ? A.gg[((int64_t)b * 6 + 5) * N2 + col] : T(0);
       } else {
         const int nhi = nb6[5];
         after = (act && nhi >= 0) ? A.uf_in[((int64_t)nhi * 6 + 4) * N2 + col] : uc[P];
@@ -346,7 +360,7 @@ k_fused3(FusedArgs<T> A) {
         const int q = idx / (4 * P1 * N), r = idx % (4 * P1 * N), f = r / (P1 * N),
                   t = r % (P1 * N);
         const int nbq = f == 0 ? nxl : (f == 1 ? nxh : (f == 2 ? nyl : nyh));
-        if (nbq >= 0) GH[4 * q + f][t] = gv[k];
+        if (nbq >= 0 || (GHOST && nbq == CF)) GH[4 * q + f][t] = gv[k];
       }
     }
     if (act) {
@@ -354,10 +368,10 @@ k_fused3(FusedArgs<T> A) {
       for (int i = 0; i < P1; ++i) {
         const T uz = uc[i];
         U[i * NT + col] = uz;
-        if (x == 0 && nxl < 0) GH[0][i * N + y] = uz;
-        if (x == N - 1 && nxh < 0) GH[1][i * N + y] = uz;
-        if (y == 0 && nyl < 0) GH[2][i * N + x] = uz;
-        if (y == N - 1 && nyh < 0) GH[3][i * N + x] = uz;
+        if (x == 0 && nxl == -1) GH[0][i * N + y] = uz;
+        if (x == N - 1 && nxh == -1) GH[1][i * N + y] = uz;
+        if (y == 0 && nyl == -1) GH[2][i * N + x] = uz;
+        if (y == N - 1 && nyh == -1) GH[3][i * N + x] = uz;
       }
     }
     lds_barrier();
@@ -373,8 +387,14 @@ k_fused3(FusedArgs<T> A) {
         T lx[P1];
         line_laplacian(
             ex, [&](int c) { return Ur[c]; }, [&]() { return GH[0][g]; }, [&]() { return GH[1][g]; },
-            [&](T uf, T gf) { return nxl >= 0 ? (GH[4][g] + Whi * uf) * (T(1) / (T)A.spacing[nxl]) : -gf; },
-            [&](T uf, T gf) { return nxh >= 0 ? (GH[5][g] + Wlo * uf) * (T(1) / (T)A.spacing[nxh]) : -gf; },
+            [&](T uf, T gf) {
+              if (GHOST && nxl == CF) return GH[4][g];
+              return nxl >= 0 ? (GH[4][g] + Whi * uf) * (T(1) / (T)A.spacing[nxl]) : -gf;
+            },
+            [&](T uf, T gf) {
+              if (GHOST && nxh == CF) return GH[5][g];
+              return nxh >= 0 ? (GH[5][g] + Wlo * uf) * (T(1) / (T)A.spacing[nxh]) : -gf;
+            },
             lx);
 #pragma unroll
         for (int j = 0; j < P1; ++j) LX[px * NT + yx * N + ex * P1 + j] = lx[j];
@@ -383,8 +403,14 @@ k_fused3(FusedArgs<T> A) {
       const int g = py * N + xy;
       line_laplacian(
           ey, [&](int c) { return Uc[c * N]; }, [&]() { return GH[2][g]; }, [&]() { return GH[3][g]; },
-          [&](T uf, T gf) { return nyl >= 0 ? (GH[6][g] + Whi * uf) * (T(1) / (T)A.spacing[nyl]) : -gf; },
-          [&](T uf, T gf) { return nyh >= 0 ? (GH[7][g] + Wlo * uf) * (T(1) / (T)A.spacing[nyh]) : -gf; },
+          [&](T uf, T gf) {
+            if (GHOST && nyl == CF) return GH[6][g];
+            return nyl >= 0 ? (GH[6][g] + Whi * uf) * (T(1) / (T)A.spacing[nyl]) : -gf;
+          },
+          [&](T uf, T gf) {
+            if (GHOST && nyh == CF) return GH[7][g];
+            return nyh >= 0 ? (GH[7][g] + Wlo * uf) * (T(1) / (T)A.spacing[nyh]) : -gf;
+          },
           ly);
     }
     lds_barrier();
@@ -543,6 +569,50 @@ k_fused3(FusedArgs<T> A) {
       }
     }
   }
+}
+
+// The complete ghosts of the coarse-fine block faces, from the face buffers the
+// previous stage wrote.  One group of rows with `count` entries each: rows[i]
+// is an exchange row (block, face, tangential node) of such a face; its entry j
+// (stored at j * nrows + i) reads face-buffer value src (source block, its
+// face, tangential node) with weight wu (u) or wg (gradient): the tangential
+// interpolation from a coarser neighbour, or the restriction from the finer
+// ones, injection along the normal.
+template <class T>
+__global__ void k_face_ghost_u(int64_t nrows, int count, const int32_t* __restrict__ rows,
+                               const int32_t* __restrict__ src, const T* __restrict__ wu,
+                               const T* __restrict__ uf, T* __restrict__ gu) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nrows) return;
+  T acc = 0;
+#pragma unroll 8
+  for (int j = 0; j < count; ++j) acc += wu[j * nrows + i] * uf[src[j * nrows + i]];
+  gu[rows[i]] = acc;
+}
+
+// The ghost gradient: the weighted source faces' complete face gradients, each
+// the block-local part plus the ghost term with the source's own ghost u (a
+// coarse-fine face too), divided by the source block's spacing
+// (evolution.jl:72-110 at a face node).
+template <class T, int P1>
+__global__ void k_face_ghost_g(int64_t nrows, int count, int64_t N2, const int32_t* __restrict__ rows,
+                               const int32_t* __restrict__ src, const T* __restrict__ wg,
+                               const T* __restrict__ pf, const T* __restrict__ gu,
+                               const double* __restrict__ spacing, T* __restrict__ gg) {
+  constexpr FusedTab<T, P1> tb = fused_tab<T, P1>();
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nrows) return;
+  T acc = 0;
+#pragma unroll 8
+  for (int j = 0; j < count; ++j) {
+    const int64_t s = src[j * nrows + i];
+    const T w = wg[j * nrows + i];
+    const int64_t face = s / N2;  // source block * 6 + its face
+    const T wc = (face % 2) ? tb.W[P1 - 1][P1 + 1] : tb.W[0][0];
+    const T fgrad = (pf[s] + wc * gu[s]) * (T(1) / (T)spacing[face / 6]);
+    acc += w * fgrad;
+  }
+  gg[rows[i]] = acc;
 }
 
 // The face buffers of a state (before the first fused stage): per block face,

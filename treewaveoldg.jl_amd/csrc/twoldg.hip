@@ -1,6 +1,7 @@
 // twoldg.hip -- MI355X (gfx950) OLDG field-evolution library: host side and C ABI.
 // The kernels live in twoldg_impl.h, instantiated per arithmetic type and
 // dimension in inst.hip.
+#include <map>
 #include "twoldg_impl.h"
 
 namespace twoldg_detail {
@@ -60,7 +61,8 @@ void destroy(twoldg_problem* p) {
                   p->tweights, p->tchol, (void*)p->spacing, (void*)p->rowptr, (void*)p->cb,
                   (void*)p->cl, p->cval_u, p->cval_g, (void*)p->dir_rows, (void*)p->dir_x,
                   (void*)p->part, (void*)p->red, (void*)p->nbr6, p->uf[0], p->uf[1], p->pf[0],
-                  p->pf[1]})
+                  p->pf[1], (void*)p->frows, (void*)p->fsrc, p->fwu,
+                  p->fwg, p->fgu, p->fgg})
     dfree(b);
   for (hipEvent_t e : p->tev) (void)hipEventDestroy(e);
   if (p->ev0) (void)hipEventDestroy(p->ev0);
@@ -116,41 +118,41 @@ int create_buffers(twoldg_problem* p, const twoldg_desc* d, const Csr& csr) {
   if ((rc = dalloc_bytes(p, &p->acc, n * sizeof(T)))) return rc;
   if (p->precision == TWOLDG_F32C) {
     if ((rc = dalloc_bytes(p, &p->lo, n * sizeof(T)))) return rc;
-    HIP_TRY(hipMemset(p->lo, 0, n * sizeof(T)));
+    HIP_TRY(hipMemsetAsync(p->lo, 0, n * sizeof(T), p->stream));
   }
   const int D = p->D;
   if ((rc = dalloc_bytes(p, &p->ug, p->nrows * sizeof(T)))) return rc;
   if ((rc = dalloc_bytes(p, &p->gg, p->nrows * (p->full ? D : 1) * sizeof(T)))) return rc;
-  HIP_TRY(hipMemset(p->ug, 0, p->nrows * sizeof(T)));
-  HIP_TRY(hipMemset(p->gg, 0, p->nrows * (p->full ? D : 1) * sizeof(T)));
+  HIP_TRY(hipMemsetAsync(p->ug, 0, p->nrows * sizeof(T), p->stream));
+  HIP_TRY(hipMemsetAsync(p->gg, 0, p->nrows * (p->full ? D : 1) * sizeof(T), p->stream));
   if ((rc = dalloc_bytes(p, &p->grads, p->sh.nb * D * p->sh.nn * sizeof(T)))) return rc;
   if ((rc = upload_tables<T>(p))) return rc;
   if ((rc = dalloc(p, &p->spacing, p->sh.nb))) return rc;
-  HIP_TRY(hipMemcpy(p->spacing, d->spacing, p->sh.nb * sizeof(double), hipMemcpyHostToDevice));
+  if ((rc = h2d(p, p->spacing, d->spacing, p->sh.nb * sizeof(double)))) return rc;
   const size_t nnz = csr.cb.size();
   if ((rc = dalloc(p, &p->rowptr, csr.rowptr.size()))) return rc;
-  HIP_TRY(hipMemcpy(p->rowptr, csr.rowptr.data(), csr.rowptr.size() * 4, hipMemcpyHostToDevice));
+  if ((rc = h2d(p, p->rowptr, csr.rowptr.data(), csr.rowptr.size() * 4))) return rc;
   if (nnz) {
     if ((rc = dalloc(p, &p->cb, nnz))) return rc;
     if ((rc = dalloc(p, &p->cl, nnz))) return rc;
-    HIP_TRY(hipMemcpy(p->cb, csr.cb.data(), nnz * 4, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(p->cl, csr.cl.data(), nnz * 4, hipMemcpyHostToDevice));
+    if ((rc = h2d(p, p->cb, csr.cb.data(), nnz * 4))) return rc;
+    if ((rc = h2d(p, p->cl, csr.cl.data(), nnz * 4))) return rc;
     if ((rc = upload_as<T>(p, &p->cval_u, csr.vu.data(), nnz))) return rc;
     if ((rc = upload_as<T>(p, &p->cval_g, csr.vg.data(), nnz))) return rc;
   }
   if (p->full) {
     const size_t nc = (size_t)p->sh.nb * (1 + D * D) * p->sh.nn;
     if ((rc = upload_as<T>(p, &p->coef, d->coef, nc))) return rc;
-    if ((rc = dalloc(p, &p->flat, p->sh.nb))) return rc;
+    if ((rc = dalloc(p, &p->flat, (p->sh.nb + 3) / 4 * 4))) return rc;
     std::vector<uint8_t> fl(p->sh.nb, 0);
     if (d->flat)
       for (int64_t b = 0; b < p->sh.nb; ++b) fl[b] = d->flat[b] ? 1 : 0;
-    HIP_TRY(hipMemcpy(p->flat, fl.data(), p->sh.nb, hipMemcpyHostToDevice));
+    if ((rc = h2d(p, p->flat, fl.data(), p->sh.nb))) return rc;
   }
   if (d->tslot) {
     p->has_tables = true;
     if ((rc = dalloc(p, &p->tslot, p->sh.nb))) return rc;
-    HIP_TRY(hipMemcpy(p->tslot, d->tslot, p->sh.nb * 4, hipMemcpyHostToDevice));
+    if ((rc = h2d(p, p->tslot, d->tslot, p->sh.nb * 4))) return rc;
     const int64_t nE = ipow(p->E, D), nq = ipow(p->P1, D), mq = ipow(p->P, D);
     if ((rc = upload_as<T>(p, &p->tmodes, d->tmodes, nq * mq))) return rc;
     if ((rc = upload_as<T>(p, &p->tweights, d->tweights, d->ntables * nE * nq))) return rc;
@@ -159,7 +161,7 @@ int create_buffers(twoldg_problem* p, const twoldg_desc* d, const Csr& csr) {
   if (!csr.dir_rows.empty()) {
     p->ndir = (int64_t)csr.dir_rows.size();
     if ((rc = dalloc(p, &p->dir_rows, p->ndir))) return rc;
-    HIP_TRY(hipMemcpy(p->dir_rows, csr.dir_rows.data(), p->ndir * 8, hipMemcpyHostToDevice));
+    if ((rc = h2d(p, p->dir_rows, csr.dir_rows.data(), p->ndir * 8))) return rc;
     std::vector<double> x(p->ndir * D);
     for (int64_t q = 0; q < p->ndir; ++q) {
       const int64_t r = csr.dir_rows[q];
@@ -167,21 +169,90 @@ int create_buffers(twoldg_problem* p, const twoldg_desc* d, const Csr& csr) {
       for (int c = 0; c < D; ++c) x[q * D + c] = d->dirichlet_x[(face * D + c) * p->sh.Nt + t];
     }
     if ((rc = dalloc(p, &p->dir_x, p->ndir * D))) return rc;
-    HIP_TRY(hipMemcpy(p->dir_x, x.data(), x.size() * 8, hipMemcpyHostToDevice));
+    if ((rc = h2d(p, p->dir_x, x.data(), x.size() * 8))) return rc;
   }
-  // fused one-launch stages: every face same-level or a mirror, flat, 3D
+  // fused one-launch stages (flat metric, 3D): faces same-level or mirrors read
+  // the neighbours' face buffers directly; with coarse-fine faces the face
+  // ghosts are formed first by two face-sized launches (the ghosted form)
   bool eligible = !p->full && fused_supported(D, p->P1, p->E);
-  for (int64_t i = 0; eligible && i < p->sh.nb * 2 * D; ++i)
-    if (d->faces[i].kind != TWOLDG_FACE_SAME && d->faces[i].kind != TWOLDG_FACE_MIRROR) eligible = false;
+  bool nonconforming = false;
+  for (int64_t i = 0; eligible && i < p->sh.nb * 2 * D; ++i) {
+    const int k = d->faces[i].kind;
+    if (k == TWOLDG_FACE_COARSER || k == TWOLDG_FACE_FINER)
+      nonconforming = true;
+    else if (k != TWOLDG_FACE_SAME && k != TWOLDG_FACE_MIRROR)
+      eligible = false;
+  }
+  if (eligible && nonconforming && !p->no_fused) {
+    // the coarse-fine faces' exchange rows, re-indexed into the face buffers
+    // (sources (block, face, t)) and grouped by entry count (P+1)^2 from a
+    // coarser neighbour, (2P+2)^2 from the finer ones; each group is stored
+    // entry-major (ELL), so that a wave's rows read consecutive entries
+    const int N = p->N;
+    const int64_t N2 = (int64_t)N * N;
+    std::map<int, std::vector<int32_t>> group;  // entry count -> rows
+    for (int64_t b = 0; b < p->sh.nb; ++b)
+      for (int f = 0; f < 6; ++f) {
+        const int kind = d->faces[b * 6 + f].kind;
+        if (kind != TWOLDG_FACE_COARSER && kind != TWOLDG_FACE_FINER) continue;
+        for (int64_t t = 0; t < N2; ++t) {
+          const int64_t r = (b * 6 + f) * N2 + t;
+          if (r > INT32_MAX) return fail(TWOLDG_EINVAL, "face buffers too large");
+          group[csr.rowptr[r + 1] - csr.rowptr[r]].push_back((int32_t)r);
+        }
+      }
+    std::vector<int32_t> rows, src;
+    std::vector<double> wu, wg;
+    for (const auto& g : group) {
+      const int cnt = g.first;
+      const std::vector<int32_t>& gr = g.second;
+      const int64_t R = (int64_t)gr.size();
+      p->fgroups.push_back({(int64_t)rows.size(), (int64_t)src.size(), R, cnt});
+      rows.insert(rows.end(), gr.begin(), gr.end());
+      const size_t e0 = src.size();
+      src.resize(e0 + (size_t)cnt * R);
+      wu.resize(src.size());
+      wg.resize(src.size());
+      for (int64_t i = 0; i < R; ++i) {
+        const int64_t r = gr[i];
+        const int f = (int)((r / N2) % 6), dd = f / 2, side = f % 2, sf = 2 * dd + (1 - side);
+        const int t0 = dd == 0 ? 1 : 0, t1 = dd == 2 ? 1 : 2;
+        for (int j = 0; j < cnt; ++j) {
+          const int32_t k = csr.rowptr[r] + j;
+          const int64_t l = csr.cl[k];
+          const int I[3] = {(int)(l % N), (int)((l / N) % N), (int)(l / N2)};
+          if (I[dd] != (side ? 0 : N - 1))
+            return fail(TWOLDG_EINVAL, "an exchange source is not on the expected face");
+          const int64_t v = ((int64_t)csr.cb[k] * 6 + sf) * N2 + I[t0] + (int64_t)N * I[t1];
+          if (v > INT32_MAX) return fail(TWOLDG_EINVAL, "face buffers too large");
+          const size_t at = e0 + (size_t)j * R + i;
+          src[at] = (int32_t)v;
+          wu[at] = csr.vu[k];
+          wg[at] = csr.vg[k];
+        }
+      }
+    }
+    if ((rc = dalloc(p, &p->frows, rows.size()))) return rc;
+    if ((rc = dalloc(p, &p->fsrc, src.size()))) return rc;
+    if ((rc = h2d(p, p->frows, rows.data(), rows.size() * 4))) return rc;
+    if ((rc = h2d(p, p->fsrc, src.data(), src.size() * 4))) return rc;
+    if ((rc = upload_as<T>(p, &p->fwu, wu.data(), wu.size()))) return rc;
+    if ((rc = upload_as<T>(p, &p->fwg, wg.data(), wg.size()))) return rc;
+    const size_t nf = (size_t)p->sh.nb * 6 * N2;
+    if ((rc = dalloc_bytes(p, &p->fgu, nf * sizeof(T)))) return rc;
+    if ((rc = dalloc_bytes(p, &p->fgg, nf * sizeof(T)))) return rc;
+    p->ghosted = true;
+  }
   if (eligible && !p->no_fused) {
     std::vector<int32_t> nbr(p->sh.nb * 6);
     for (int64_t b = 0; b < p->sh.nb; ++b)
       for (int f = 0; f < 6; ++f) {
         const twoldg_face& F = d->faces[b * 6 + f];
-        nbr[b * 6 + f] = F.kind == TWOLDG_FACE_SAME ? (int32_t)F.nb[0] : -1;
+        nbr[b * 6 + f] = F.kind == TWOLDG_FACE_SAME ? (int32_t)F.nb[0]
+                         : F.kind == TWOLDG_FACE_MIRROR ? -1 : -2;
       }
     if ((rc = dalloc(p, &p->nbr6, nbr.size()))) return rc;
-    HIP_TRY(hipMemcpy(p->nbr6, nbr.data(), nbr.size() * 4, hipMemcpyHostToDevice));
+    if ((rc = h2d(p, p->nbr6, nbr.data(), nbr.size() * 4))) return rc;
     const size_t nf = (size_t)p->sh.nb * 6 * p->sh.Nt;
     for (int k = 0; k < 2; ++k) {
       if ((rc = dalloc_bytes(p, &p->uf[k], nf * sizeof(T)))) return rc;
@@ -283,20 +354,19 @@ int twoldg_problem_info(const twoldg_problem* p, int64_t* state_len, int64_t* de
 int twoldg_state_upload(twoldg_problem* p, int slot, const double* host) {
   if (!p || !host || slot < 0 || slot > 2) return fail(TWOLDG_EINVAL, "bad state upload");
   const int64_t n = p->state_len;
-  double* tmp = nullptr;
-  HIP_TRY(hipMalloc((void**)&tmp, n * sizeof(double)));
-  hipError_t e = hipMemcpyAsync(tmp, host, n * sizeof(double), hipMemcpyHostToDevice, p->stream);
-  if (e == hipSuccess) {
-    void* lo = (slot == TWOLDG_SLOT_STATE) ? p->lo : nullptr;
-    if (p->precision == TWOLDG_F64)
-      hipLaunchKernelGGL((k_from_double<double>), dim3(grid_for(n, 256)), dim3(256), 0, p->stream,
-                         n, tmp, (double*)p->slot[slot], (double*)nullptr);
-    else
-      hipLaunchKernelGGL((k_from_double<float>), dim3(grid_for(n, 256)), dim3(256), 0, p->stream,
-                         n, tmp, (float*)p->slot[slot], (float*)lo);
-    e = hipStreamSynchronize(p->stream);
-  }
-  (void)hipFree(tmp);
+  // staged in pinned host memory and converted by a kernel that reads it there
+  double* pinned = nullptr;
+  HIP_TRY(hipHostMalloc((void**)&pinned, n * sizeof(double), hipHostMallocDefault));
+  std::memcpy(pinned, host, n * sizeof(double));
+  void* lo = (slot == TWOLDG_SLOT_STATE) ? p->lo : nullptr;
+  if (p->precision == TWOLDG_F64)
+    hipLaunchKernelGGL((k_from_double<double>), dim3(grid_for(n, 256)), dim3(256), 0, p->stream, n,
+                       (const double*)pinned, (double*)p->slot[slot], (double*)nullptr);
+  else
+    hipLaunchKernelGGL((k_from_double<float>), dim3(grid_for(n, 256)), dim3(256), 0, p->stream, n,
+                       (const double*)pinned, (float*)p->slot[slot], (float*)lo);
+  hipError_t e = hipStreamSynchronize(p->stream);
+  (void)hipHostFree(pinned);
   if (e != hipSuccess) return fail(TWOLDG_EDEVICE, std::string("state upload: ") + hipGetErrorString(e));
   p->uploaded[slot] = true;
   return 0;

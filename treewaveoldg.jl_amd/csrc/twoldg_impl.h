@@ -690,6 +690,20 @@ struct twoldg_problem {
   int32_t* nbr6 = nullptr;  // (nb, 6)
   void* uf[2] = {nullptr, nullptr};
   void* pf[2] = {nullptr, nullptr};
+  // ghosted fused stages (coarse-fine faces): the exchange rows re-indexed into
+  // the face buffers, and the complete face ghosts (nb, 6, N^2)
+  bool ghosted = false;
+  struct FaceGroup {
+    int64_t row0, entry0, nrows;  // offsets into frows and the entry arrays
+    int count;                    // entries per row
+  };
+  std::vector<FaceGroup> fgroups;
+  int32_t* frows = nullptr;    // exchange rows of the coarse-fine faces, by group
+  int32_t* fsrc = nullptr;     // face-buffer index of each entry (entry-major)
+  void* fwu = nullptr;         // weights (u), in T
+  void* fwg = nullptr;         // weights (gradient), in T
+  void* fgu = nullptr;
+  void* fgg = nullptr;
   // per-launch timing of the stage kernels (bench): event pairs
   bool timing = false;
   std::vector<hipEvent_t> tev;
@@ -718,6 +732,32 @@ int dalloc_bytes(twoldg_problem* p, void** ptr, size_t bytes) {
     return fail(TWOLDG_ENOMEM, std::string("hipMalloc of ") + std::to_string(bytes) +
                                    " bytes: " + hipGetErrorString(e));
   p->device_bytes += (int64_t)bytes;
+  return 0;
+}
+
+// Host data into device memory: staged in pinned host memory and written by a
+// kernel on the problem's stream, so that the device's caches hold the new
+// values.  Tables copied into freshly allocated device memory by the DMA path
+// (hipMemcpy) were at times read stale by the stage kernels of the first
+// problem built on reused memory; a kernel's stores go through the L2.
+__global__ void k_copy_words(int64_t n, const uint32_t* __restrict__ src, uint32_t* __restrict__ dst) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+int h2d(twoldg_problem* p, void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return 0;
+  const size_t words = (bytes + 3) / 4;
+  void* pinned = nullptr;
+  HIP_TRY(hipHostMalloc(&pinned, words * 4, hipHostMallocDefault));
+  std::memcpy(pinned, src, bytes);
+  if (bytes % 4) std::memset((char*)pinned + bytes, 0, words * 4 - bytes);
+  // whole words only: the device allocations are at least 4-byte granular
+  hipLaunchKernelGGL(k_copy_words, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, p->stream,
+                     (int64_t)words, (const uint32_t*)pinned, (uint32_t*)dst);
+  hipError_t e = hipStreamSynchronize(p->stream);
+  (void)hipHostFree(pinned);
+  if (e != hipSuccess) return fail(TWOLDG_EDEVICE, std::string("table upload: ") + hipGetErrorString(e));
   return 0;
 }
 
@@ -875,7 +915,7 @@ int upload_as(twoldg_problem* p, void** dst, const double* src, size_t n) {
   for (size_t i = 0; i < n; ++i) tmp[i] = (T)src[i];
   int rc = dalloc_bytes(p, dst, n * sizeof(T));
   if (rc) return rc;
-  HIP_TRY(hipMemcpy(*dst, tmp.data(), n * sizeof(T), hipMemcpyHostToDevice));
+  return h2d(p, *dst, tmp.data(), n * sizeof(T));
   return 0;
 }
 
@@ -892,7 +932,7 @@ int upload_tables(twoldg_problem* p) {
   }
   int rc = dalloc_bytes(p, &p->tables, sizeof tb);
   if (rc) return rc;
-  HIP_TRY(hipMemcpy(p->tables, &tb, sizeof tb, hipMemcpyHostToDevice));
+  if ((rc = h2d(p, p->tables, &tb, sizeof tb))) return rc;
   return 0;
 }
 
@@ -1003,7 +1043,7 @@ struct Launch {
   }
 };
 
-template <class T, int P1, int E>
+template <class T, int P1, int E, bool GHOST>
 int fused_steps(twoldg_problem* p, double dt, int64_t nsteps) {
   constexpr int N = P1 * E;
   constexpr int threads = (N * N + 63) / 64 * 64;
@@ -1022,6 +1062,23 @@ int fused_steps(twoldg_problem* p, double dt, int64_t nsteps) {
   A.spacing = p->spacing;
   A.tables = (const Tables<T>*)p->tables;
   A.truncate = p->truncate ? 1 : 0;
+  A.gu = (const T*)p->fgu;
+  A.gg = (const T*)p->fgg;
+  // the complete ghosts of the coarse-fine faces from the stage input's face
+  // buffers (ghosted form)
+  auto ghosts = [&](int in) {
+    if constexpr (GHOST) {
+      for (const auto& g : p->fgroups)
+        hipLaunchKernelGGL((k_face_ghost_u<T>), dim3(grid_for(g.nrows, 256)), dim3(256), 0,
+                           p->stream, g.nrows, g.count, p->frows + g.row0, p->fsrc + g.entry0,
+                           (const T*)p->fwu + g.entry0, (const T*)p->uf[in], (T*)p->fgu);
+      for (const auto& g : p->fgroups)
+        hipLaunchKernelGGL((k_face_ghost_g<T, P1>), dim3(grid_for(g.nrows, 256)), dim3(256), 0,
+                           p->stream, g.nrows, g.count, (int64_t)N * N, p->frows + g.row0,
+                           p->fsrc + g.entry0, (const T*)p->fwg + g.entry0, (const T*)p->pf[in],
+                           (const T*)p->fgu, p->spacing, (T*)p->fgg);
+    }
+  };
   auto io = [&](int in) {
     A.uf_in = (const T*)p->uf[in];
     A.pf_in = (const T*)p->pf[in];
@@ -1034,26 +1091,30 @@ int fused_steps(twoldg_problem* p, double dt, int64_t nsteps) {
     A.a = (T)(dt / 2);
     io(0);
     tmark(p);
-    hipLaunchKernelGGL((k_fused3<T, P1, E, 1>), dim3((unsigned)nb), dim3(threads), 0, p->stream, A);
+    ghosts(0);
+    hipLaunchKernelGGL((k_fused3<T, P1, E, 1, GHOST>), dim3((unsigned)nb), dim3(threads), 0, p->stream, A);
     tmark(p);
     A.ys = (T*)p->ys;
     io(1);
     tmark(p);
-    hipLaunchKernelGGL((k_fused3<T, P1, E, 2>), dim3((unsigned)nb), dim3(threads), 0, p->stream, A);
+    ghosts(1);
+    hipLaunchKernelGGL((k_fused3<T, P1, E, 2, GHOST>), dim3((unsigned)nb), dim3(threads), 0, p->stream, A);
     tmark(p);
     A.a = (T)dt;
     io(0);
     tmark(p);
-    hipLaunchKernelGGL((k_fused3<T, P1, E, 2>), dim3((unsigned)nb), dim3(threads), 0, p->stream, A);
+    ghosts(0);
+    hipLaunchKernelGGL((k_fused3<T, P1, E, 2, GHOST>), dim3((unsigned)nb), dim3(threads), 0, p->stream, A);
     tmark(p);
     A.a = (T)(dt / 6);
     A.yo = yn;
     io(1);
     tmark(p);
+    ghosts(1);
     if (p->precision == TWOLDG_F32C)
-      hipLaunchKernelGGL((k_fused3<T, P1, E, 4>), dim3((unsigned)nb), dim3(threads), 0, p->stream, A);
+      hipLaunchKernelGGL((k_fused3<T, P1, E, 4, GHOST>), dim3((unsigned)nb), dim3(threads), 0, p->stream, A);
     else
-      hipLaunchKernelGGL((k_fused3<T, P1, E, 3>), dim3((unsigned)nb), dim3(threads), 0, p->stream, A);
+      hipLaunchKernelGGL((k_fused3<T, P1, E, 3, GHOST>), dim3((unsigned)nb), dim3(threads), 0, p->stream, A);
     tmark(p);
   }
   HIP_TRY(hipGetLastError());
@@ -1066,10 +1127,14 @@ int fused_dispatch(twoldg_problem* p, double dt, int64_t nsteps) {
   if constexpr (D == 3) {
     switch (p->E) {
       case 2:
-        if constexpr (P1 * 2 <= 32) return fused_steps<T, P1, 2>(p, dt, nsteps);
+        if constexpr (P1 * 2 <= 32)
+          return p->ghosted ? fused_steps<T, P1, 2, true>(p, dt, nsteps)
+                            : fused_steps<T, P1, 2, false>(p, dt, nsteps);
         break;
       case 4:
-        if constexpr (P1 * 4 <= 24) return fused_steps<T, P1, 4>(p, dt, nsteps);
+        if constexpr (P1 * 4 <= 24)
+          return p->ghosted ? fused_steps<T, P1, 4, true>(p, dt, nsteps)
+                            : fused_steps<T, P1, 4, false>(p, dt, nsteps);
         break;
     }
   }
