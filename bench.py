@@ -162,7 +162,8 @@ def config3(large=False, steps=20):
     """Reflecting box with nested cubes (3 levels, finest half-width 1/8): P=3,
     E=2, roots=8 (1408 blocks), or the large refined forest P=4, E=4, roots=16
     (11264 blocks of 20^3) for the bandwidth figure; cos mode m=2, FP64 and
-    FP32c, generic exchange path (coarse-fine faces)."""
+    FP32c; fused stages with the coarse-fine face ghosts formed by two
+    face-sized launches per stage."""
     import twoldg as T
     P, E, roots = (4, 4, 16) if large else (3, 2, 8)
     out = {}
