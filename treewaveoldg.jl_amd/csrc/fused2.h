@@ -48,7 +48,7 @@ struct BatchArgs {
 
 // The general body: N = E (P+1) nodes per block side, E even, N^2 <= 256.
 template <class T, int P1, int MODE>
-__device__ void batch_blocks(const BatchArgs<T>& A, const BatchItem& it, int N, T* lds) {
+__device__ void batch_blocks(const BatchArgs<T>& A, const BatchItem& it, int N, T* lds, T* ltab) {
   constexpr int PW = P1 + 2;
   constexpr int P = P1 - 1;
   const int NT = N * N, NP = N + 2;
@@ -58,7 +58,14 @@ __device__ void batch_blocks(const BatchArgs<T>& A, const BatchItem& it, int N, 
   const int node = tid - bl * NT;
   const int x = act ? node % N : 0, y = act ? node / N : 0;
   const int nbk = it.count;
-  const Tables<T>& tb = A.tables[P1 - 2];
+  // per-node rows of W and the top mode from LDS; the uniform entries are
+  // compile-time constants
+  constexpr FusedTab<T, P1> tb = fused_tab<T, P1>();
+  {
+    const Tables<T>& gt = A.tables[P1 - 2];
+    if (tid < P1 * PW) ltab[tid] = gt.W[tid / PW][tid % PW];
+    if (tid < P1) ltab[P1 * PW + tid] = gt.mode[tid];
+  }
   // LDS carve: per block U (NP^2), GX (N NP), GY (NP N), VA, VB, UN (N^2)
   const int per = NP * NP + 2 * N * NP + 3 * NT;
   T* base = lds + (act ? bl : 0) * per;
@@ -100,14 +107,14 @@ __device__ void batch_blocks(const BatchArgs<T>& A, const BatchItem& it, int N, 
     if (y == 0) U[x + 1] = nyl >= 0 ? A.uf_in[fidx(nyl, 3, x)] : uz;
     if (y == N - 1) U[(N + 1) * NP + x + 1] = nyh >= 0 ? A.uf_in[fidx(nyh, 2, x)] : uz;
   }
-  __syncthreads();
+  lds_barrier();
   // (2) gradients and their ghosts
   if (act) {
     T gx = 0, gy = 0;
 #pragma unroll
     for (int k = 0; k < PW; ++k) {
-      gx += tb.W[ix][k] * U[(y + 1) * NP + (sx + k + 1)];
-      gy += tb.W[iy][k] * U[(sy + k + 1) * NP + x + 1];
+      gx += ltab[ix * PW + k] * U[(y + 1) * NP + (sx + k + 1)];
+      gy += ltab[iy * PW + k] * U[(sy + k + 1) * NP + x + 1];
     }
     gx *= rh;
     gy *= rh;
@@ -118,15 +125,15 @@ __device__ void batch_blocks(const BatchArgs<T>& A, const BatchItem& it, int N, 
     if (y == 0) GY[x] = nyl >= 0 ? (A.pf_in[fidx(nyl, 3, x)] + Whi * uz) * (T(1) / (T)A.spacing[nyl]) : -gy;
     if (y == N - 1) GY[(N + 1) * N + x] = nyh >= 0 ? (A.pf_in[fidx(nyh, 2, x)] + Wlo * uz) * (T(1) / (T)A.spacing[nyh]) : -gy;
   }
-  __syncthreads();
+  lds_barrier();
   // (3) the Laplacian and the stage combination
   T unext = 0, w = 0;
   if (act) {
     T lx = 0, ly = 0;
 #pragma unroll
     for (int k = 0; k < PW; ++k) {
-      lx += tb.W[ix][k] * GX[y * NP + (sx + k + 1)];
-      ly += tb.W[iy][k] * GY[(sy + k + 1) * N + x];
+      lx += ltab[ix * PW + k] * GX[y * NP + (sx + k + 1)];
+      ly += ltab[iy * PW + k] * GY[(sy + k + 1) * N + x];
     }
     const T kv = ((T(0) + lx) + ly) * rh;
     if (MODE == 1) {
@@ -153,7 +160,7 @@ __device__ void batch_blocks(const BatchArgs<T>& A, const BatchItem& it, int N, 
     VA[node] = w;
     UN[node] = unext;
   }
-  __syncthreads();
+  lds_barrier();
   // (4) faces of the next u; truncation along x
   if (act) {
     if (x == 0) {
@@ -188,12 +195,12 @@ __device__ void batch_blocks(const BatchArgs<T>& A, const BatchItem& it, int N, 
       T c = 0;
 #pragma unroll
       for (int j = 0; j < P1; ++j) c += tb.dual[j] * VA[y * N + (x - ix) + j];
-      VB[node] = w - tb.mode[ix] * c;
+      VB[node] = w - ltab[P1 * PW + ix] * c;
     } else {
       VB[node] = w;
     }
   }
-  __syncthreads();
+  lds_barrier();
   // (5) truncation along y; v out
   if (act) {
     T vv = VB[node];
@@ -201,7 +208,7 @@ __device__ void batch_blocks(const BatchArgs<T>& A, const BatchItem& it, int N, 
       T c = 0;
 #pragma unroll
       for (int j = 0; j < P1; ++j) c += tb.dual[j] * VB[((y - iy) + j) * N + x];
-      vv -= tb.mode[iy] * c;
+      vv -= ltab[P1 * PW + iy] * c;
     }
     if (MODE == 4) {
       const T yk = vv - A.lo[iv], hv = A.yn[iv], tv = hv + yk;
@@ -217,16 +224,17 @@ __device__ void batch_blocks(const BatchArgs<T>& A, const BatchItem& it, int N, 
 template <class T, int MODE>
 __global__ void __launch_bounds__(kBatchThreads) k_batch2(BatchArgs<T> A) {
   __shared__ T lds[kBatchLds];
+  __shared__ T ltab[8 * 10 + 8];  // W rows and the top mode of the entry's order
   const BatchItem it = A.items[blockIdx.x];
   const int N = it.N;
   switch (it.P1) {
-    case 2: batch_blocks<T, 2, MODE>(A, it, N, lds); break;
-    case 3: batch_blocks<T, 3, MODE>(A, it, N, lds); break;
-    case 4: batch_blocks<T, 4, MODE>(A, it, N, lds); break;
-    case 5: batch_blocks<T, 5, MODE>(A, it, N, lds); break;
-    case 6: batch_blocks<T, 6, MODE>(A, it, N, lds); break;
-    case 7: batch_blocks<T, 7, MODE>(A, it, N, lds); break;
-    case 8: batch_blocks<T, 8, MODE>(A, it, N, lds); break;
+    case 2: batch_blocks<T, 2, MODE>(A, it, N, lds, ltab); break;
+    case 3: batch_blocks<T, 3, MODE>(A, it, N, lds, ltab); break;
+    case 4: batch_blocks<T, 4, MODE>(A, it, N, lds, ltab); break;
+    case 5: batch_blocks<T, 5, MODE>(A, it, N, lds, ltab); break;
+    case 6: batch_blocks<T, 6, MODE>(A, it, N, lds, ltab); break;
+    case 7: batch_blocks<T, 7, MODE>(A, it, N, lds, ltab); break;
+    case 8: batch_blocks<T, 8, MODE>(A, it, N, lds, ltab); break;
     default: break;
   }
 }
