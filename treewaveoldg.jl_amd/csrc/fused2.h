@@ -13,7 +13,7 @@
 
 #pragma once
 
-constexpr int kBatchThreads = 256;
+constexpr int kBatchThreads = 512;
 constexpr int kBatchLds = 4096;  // values of T per workgroup
 
 struct BatchItem {
