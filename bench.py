@@ -131,7 +131,28 @@ def cpu_baseline(seconds=15.0):
         f"-> {value:.3e} node-updates/s ({el / steps / nodes * 1e9:.2f} ns per node and step)")
     return dict(value=value, unit="node-updates/s", cores=threads, kind="port",
                 sample=f"{steps} RK4 steps of the same 32.8M-node forest after one warm-up "
-                       f"step, FP64 C++/OpenMP restatement (oracle), {threads} threads")
+                       f"step, FP64 C++/OpenMP restatement (oracle), {threads} threads",
+                host=host_info(), ns_per_node_update=el / steps / nodes * 1e9)
+
+
+def host_info():
+    """The host the CPU baseline ran on: CPU model, logical CPUs visible to this
+    process, NUMA nodes."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    numa = None
+    try:
+        numa = len([d for d in os.listdir("/sys/devices/system/node") if d.startswith("node")])
+    except OSError:
+        pass
+    return dict(cpu_model=model, nproc=len(os.sched_getaffinity(0)), numa_nodes=numa)
 
 
 def timed_integrator(T, prob, u0, dt, steps, warmup=1):

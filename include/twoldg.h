@@ -144,6 +144,16 @@ int twoldg_synchronize(twoldg_problem* p);
 int twoldg_exchange_plan(const twoldg_desc* desc, int64_t* nrows, int64_t* nnz,
                          int32_t* rowptr, int32_t* cb, int32_t* cl, double* vu, double* vg);
 
+/* Logical positions of every owned node of a flat forest, on the host:
+   x[(b*D + d)*N^D + node] = lo[b*D + d] + h_b ((c_d N + q (P+1)) + offsets[i])
+   with node = (i_1, ..., i_D) (first index fastest), i_d = q (P+1) + i, c the
+   block's integer position at its level (corner[b*D + d]) and lo its patch's
+   origin (lo[b*D + d]), in this order of rounding.  Replaces the positions the
+   reference's layout computes (src/layout.jl:59-76, TreeAMR's nodeposition)
+   that the Julia side needs to evaluate initial data f(x, v). */
+int twoldg_node_coordinates(const twoldg_desc* desc, const double* lo, const int64_t* corner,
+                            double* x);
+
 /* --- measurement ---------------------------------------------------------- */
 /* Run nsteps RK4 steps like twoldg_rk4, timed with HIP events on the problem's
    stream: out_ms = the whole run, out_kernel_ms = the summed time of the stage

@@ -46,6 +46,9 @@ def _plan_vs_oracle(D, P, E, roots, reflecting=False, regions=()):
     assert [(lf[0], lf[1], tuple(lf[2:2 + D])) for lf in f.leaves] == [b[:3] for b in ob]
     assert np.array_equal(f.spacings(), np.array([b[3] for b in ob]))
     assert np.array_equal(f.node_logical(T.Element(P).offsets), o.coords())
+    # the C ABI's host-side positions, bitwise
+    from twoldg.problem import node_coordinates
+    assert np.array_equal(node_coordinates(f), o.coords())
     y = np.random.default_rng(1).standard_normal(o.size())
     ug, gg = o.ghosts(y)
     g = o.gradients(y).reshape(D, f.nblocks, -1)

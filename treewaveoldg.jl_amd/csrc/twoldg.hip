@@ -460,6 +460,32 @@ int twoldg_synchronize(twoldg_problem* p) {
   return 0;
 }
 
+int twoldg_node_coordinates(const twoldg_desc* d, const double* lo, const int64_t* corner,
+                            double* x) {
+  if (!d || !lo || !corner || !x) return fail(TWOLDG_EINVAL, "node coordinates need all arrays");
+  if (d->abi_version != TWOLDG_ABI_VERSION) return fail(TWOLDG_EINVAL, "ABI version mismatch");
+  if (d->D < 1 || d->D > 3 || d->P < 1 || d->P > twoldg::kMaxP || d->E < 1 || d->nblocks < 0 ||
+      (d->nblocks > 0 && !d->spacing))
+    return fail(TWOLDG_EINVAL, "bad forest description");
+  const int D = d->D, P1 = d->P + 1, N = d->E * P1;
+  const double* off = twoldg::kElementTables[d->P - 1].offsets;
+  int64_t nn = 1;
+  for (int k = 0; k < D; ++k) nn *= N;
+  for (int64_t b = 0; b < d->nblocks; ++b)
+    for (int dd = 0; dd < D; ++dd) {
+      const double h = d->spacing[b], o = lo[b * D + dd];
+      const int64_t c = corner[b * D + dd];
+      int64_t stride = 1;
+      for (int k = 0; k < dd; ++k) stride *= N;
+      double* xo = x + (b * D + dd) * nn;
+      for (int64_t node = 0; node < nn; ++node) {
+        const int id = (int)((node / stride) % N), q = id / P1, i = id % P1;
+        xo[node] = o + h * ((double)(c * N + (int64_t)q * P1) + off[i]);
+      }
+    }
+  return 0;
+}
+
 int twoldg_exchange_plan(const twoldg_desc* d, int64_t* nrows, int64_t* nnz, int32_t* rowptr,
                          int32_t* cb, int32_t* cl, double* vu, double* vg) {
   int rc = check_desc(d);

@@ -74,6 +74,7 @@ def lib():
             "twoldg_timestep": ([_vp, _dbl, _vp], ctypes.c_int),
             "twoldg_synchronize": ([_vp], ctypes.c_int),
             "twoldg_exchange_plan": ([_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),
+            "twoldg_node_coordinates": ([_vp, _vp, _vp, _vp], ctypes.c_int),
             "twoldg_rk4_timed": ([_vp, _dbl, _dbl, _i64, ctypes.c_int, _vp, _vp, _vp],
                                  ctypes.c_int),
             "twoldg_batch_create": ([_vp, _i64, _vp, _vp], ctypes.c_int),
@@ -110,6 +111,7 @@ def exported_symbols():
             "twoldg_problem_create", "twoldg_problem_destroy", "twoldg_problem_info",
             "twoldg_state_upload", "twoldg_state_download", "twoldg_state_copy", "twoldg_rhs",
             "twoldg_truncate", "twoldg_rk4", "twoldg_energy", "twoldg_norms", "twoldg_timestep",
-            "twoldg_synchronize", "twoldg_exchange_plan", "twoldg_rk4_timed",
+            "twoldg_synchronize", "twoldg_exchange_plan", "twoldg_node_coordinates",
+            "twoldg_rk4_timed",
             "twoldg_batch_create", "twoldg_batch_destroy", "twoldg_batch_upload",
             "twoldg_batch_download", "twoldg_batch_rk4", "twoldg_batch_info"]

@@ -163,6 +163,27 @@ class WaveProblem:
         return x
 
 
+def node_coordinates(forest):
+    """Logical node positions of a flat forest from the library's host-side
+    twoldg_node_coordinates, (nblocks, D, N^D) (layout.jl:59-76)."""
+    sp = forest.spacings()
+    desc = _lib.Desc()
+    desc.abi_version = _lib.ABI_VERSION
+    desc.D, desc.P, desc.E = forest.D, forest.P, forest.E
+    desc.precision = _lib.F64
+    desc.nblocks = forest.nblocks
+    desc.spacing = _ptr(sp)
+    lo = np.ascontiguousarray([forest.patches[p][1][:forest.D] for (p, lev, *c) in forest.leaves],
+                              dtype=np.float64).reshape(forest.nblocks, forest.D)
+    corner = np.ascontiguousarray([c[:forest.D] for (p, lev, *c) in forest.leaves],
+                                  dtype=np.int64).reshape(forest.nblocks, forest.D)
+    x = np.empty((forest.nblocks, forest.D, forest.N ** forest.D))
+    _lib.check(_lib.lib().twoldg_node_coordinates(ctypes.byref(desc), _ptr(lo),
+                                                  corner.ctypes.data_as(ctypes.c_void_p),
+                                                  _ptr(x)))
+    return x
+
+
 def exchange_plan(forest, el=None):
     """The exchange operator the library builds for `forest` (host only): a dict
     of rowptr, cb, cl, vu, vg (see include/twoldg.h twoldg_exchange_plan)."""
