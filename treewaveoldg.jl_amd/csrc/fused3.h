@@ -571,47 +571,77 @@ k_fused3(FusedArgs<T> A) {
   }
 }
 
-// The complete ghosts of the coarse-fine block faces, from the face buffers the
-// previous stage wrote.  One group of rows with `count` entries each: rows[i]
-// is an exchange row (block, face, tangential node) of such a face; its entry j
-// (stored at j * nrows + i) reads face-buffer value src (source block, its
-// face, tangential node) with weight wu (u) or wg (gradient): the tangential
-// interpolation from a coarser neighbour, or the restriction from the finer
-// ones, injection along the normal.
-template <class T>
-__global__ void k_face_ghost_u(int64_t nrows, int count, const int32_t* __restrict__ rows,
-                               const int32_t* __restrict__ src, const T* __restrict__ wu,
-                               const T* __restrict__ uf, T* __restrict__ gu) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nrows) return;
-  T acc = 0;
-#pragma unroll 8
-  for (int j = 0; j < count; ++j) acc += wu[j * nrows + i] * uf[src[j * nrows + i]];
-  gu[rows[i]] = acc;
+// The coarse-fine rows of a problem: groups of rows with `count` exchange
+// entries each, stored entry-major (entry j of the group's row i at
+// entry0 + j * nrows + i), the groups' rows one after another.
+struct FaceGroups {
+  int ngroups;
+  int count[4];
+  int64_t row0[4], entry0[4], nrows[4];
+};
+
+// Row i of the concatenated groups: its group and entry base and stride.
+__device__ __forceinline__ bool face_group_row(const FaceGroups& G, int64_t i, int64_t& e,
+                                               int64_t& stride, int& count) {
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    if (g < G.ngroups && i >= G.row0[g] && i < G.row0[g] + G.nrows[g]) {
+      e = G.entry0[g] + (i - G.row0[g]);
+      stride = G.nrows[g];
+      count = G.count[g];
+      return true;
+    }
+  }
+  return false;
 }
 
-// The ghost gradient: the weighted source faces' complete face gradients, each
-// the block-local part plus the ghost term with the source's own ghost u (a
-// coarse-fine face too), divided by the source block's spacing
-// (evolution.jl:72-110 at a face node).
+// The complete ghosts of the coarse-fine block faces, from the face buffers the
+// previous stage wrote.  rows[i] is an exchange row (block, face, tangential
+// node) of such a face; its entries read face-buffer values src (source block,
+// its face, tangential node) with weights wu (u) and wg (gradient): the
+// tangential interpolation from a coarser neighbour, or the restriction from
+// the finer ones, injection along the normal.
+//
+// First the ghost u of every row, and the row's own complete face gradient:
+// the block-local part plus the ghost term with that ghost u, divided by the
+// block's spacing (evolution.jl:72-110 at a face node) -- every source of a
+// coarse-fine row lies on a coarse-fine face, so these are the gradients the
+// second launch gathers.
 template <class T, int P1>
-__global__ void k_face_ghost_g(int64_t nrows, int count, int64_t N2, const int32_t* __restrict__ rows,
-                               const int32_t* __restrict__ src, const T* __restrict__ wg,
-                               const T* __restrict__ pf, const T* __restrict__ gu,
-                               const double* __restrict__ spacing, T* __restrict__ gg) {
+__global__ void k_face_ghost_u(int64_t ntot, FaceGroups G, int64_t N2,
+                               const int32_t* __restrict__ rows, const int32_t* __restrict__ src,
+                               const T* __restrict__ wu, const T* __restrict__ uf,
+                               const T* __restrict__ pf, const double* __restrict__ spacing,
+                               T* __restrict__ gu, T* __restrict__ fg) {
   constexpr FusedTab<T, P1> tb = fused_tab<T, P1>();
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nrows) return;
+  if (i >= ntot) return;
+  int64_t e, stride;
+  int count;
+  if (!face_group_row(G, i, e, stride, count)) return;
   T acc = 0;
 #pragma unroll 8
-  for (int j = 0; j < count; ++j) {
-    const int64_t s = src[j * nrows + i];
-    const T w = wg[j * nrows + i];
-    const int64_t face = s / N2;  // source block * 6 + its face
-    const T wc = (face % 2) ? tb.W[P1 - 1][P1 + 1] : tb.W[0][0];
-    const T fgrad = (pf[s] + wc * gu[s]) * (T(1) / (T)spacing[face / 6]);
-    acc += w * fgrad;
-  }
+  for (int j = 0; j < count; ++j) acc += wu[e + j * stride] * uf[src[e + j * stride]];
+  const int64_t r = rows[i];
+  gu[r] = acc;
+  const int64_t face = r / N2;  // block * 6 + face
+  const T wc = (face % 2) ? tb.W[P1 - 1][P1 + 1] : tb.W[0][0];
+  fg[r] = (pf[r] + wc * acc) * (T(1) / (T)spacing[face / 6]);
+}
+
+// Then the ghost gradient of every row: the weighted sources' face gradients.
+template <class T>
+__global__ void k_face_ghost_g(int64_t ntot, FaceGroups G, const int32_t* __restrict__ rows,
+                               const int32_t* __restrict__ src, const T* __restrict__ wg,
+                               const T* __restrict__ fg, T* __restrict__ gg) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ntot) return;
+  int64_t e, stride;
+  int count;
+  if (!face_group_row(G, i, e, stride, count)) return;
+  T acc = 0;
+#pragma unroll 8
+  for (int j = 0; j < count; ++j) acc += wg[e + j * stride] * fg[src[e + j * stride]];
   gg[rows[i]] = acc;
 }
 

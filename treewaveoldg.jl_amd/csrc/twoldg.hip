@@ -62,7 +62,7 @@ void destroy(twoldg_problem* p) {
                   (void*)p->cl, p->cval_u, p->cval_g, (void*)p->dir_rows, (void*)p->dir_x,
                   (void*)p->part, (void*)p->red, (void*)p->nbr6, p->uf[0], p->uf[1], p->pf[0],
                   p->pf[1], (void*)p->frows, (void*)p->fsrc, p->fwu,
-                  p->fwg, p->fgu, p->fgg})
+                  p->fwg, p->fgu, p->fgg, p->ffg})
     dfree(b);
   for (hipEvent_t e : p->tev) (void)hipEventDestroy(e);
   if (p->ev0) (void)hipEventDestroy(p->ev0);
@@ -207,7 +207,12 @@ int create_buffers(twoldg_problem* p, const twoldg_desc* d, const Csr& csr) {
       const int cnt = g.first;
       const std::vector<int32_t>& gr = g.second;
       const int64_t R = (int64_t)gr.size();
-      p->fgroups.push_back({(int64_t)rows.size(), (int64_t)src.size(), R, cnt});
+      if (p->fgroups.ngroups == 4) return fail(TWOLDG_EINVAL, "more than four coarse-fine row kinds");
+      const int gi = p->fgroups.ngroups++;
+      p->fgroups.count[gi] = cnt;
+      p->fgroups.row0[gi] = (int64_t)rows.size();
+      p->fgroups.entry0[gi] = (int64_t)src.size();
+      p->fgroups.nrows[gi] = R;
       rows.insert(rows.end(), gr.begin(), gr.end());
       const size_t e0 = src.size();
       src.resize(e0 + (size_t)cnt * R);
@@ -241,6 +246,8 @@ int create_buffers(twoldg_problem* p, const twoldg_desc* d, const Csr& csr) {
     const size_t nf = (size_t)p->sh.nb * 6 * N2;
     if ((rc = dalloc_bytes(p, &p->fgu, nf * sizeof(T)))) return rc;
     if ((rc = dalloc_bytes(p, &p->fgg, nf * sizeof(T)))) return rc;
+    if ((rc = dalloc_bytes(p, &p->ffg, nf * sizeof(T)))) return rc;
+    p->nfrows = (int64_t)rows.size();
     p->ghosted = true;
   }
   if (eligible && !p->no_fused) {

@@ -693,17 +693,15 @@ struct twoldg_problem {
   // ghosted fused stages (coarse-fine faces): the exchange rows re-indexed into
   // the face buffers, and the complete face ghosts (nb, 6, N^2)
   bool ghosted = false;
-  struct FaceGroup {
-    int64_t row0, entry0, nrows;  // offsets into frows and the entry arrays
-    int count;                    // entries per row
-  };
-  std::vector<FaceGroup> fgroups;
+  FaceGroups fgroups{};        // the groups of coarse-fine rows (entry counts)
+  int64_t nfrows = 0;
   int32_t* frows = nullptr;    // exchange rows of the coarse-fine faces, by group
   int32_t* fsrc = nullptr;     // face-buffer index of each entry (entry-major)
   void* fwu = nullptr;         // weights (u), in T
   void* fwg = nullptr;         // weights (gradient), in T
   void* fgu = nullptr;
   void* fgg = nullptr;
+  void* ffg = nullptr;         // the coarse-fine rows' own face gradients
   // per-launch timing of the stage kernels (bench): event pairs
   bool timing = false;
   std::vector<hipEvent_t> tev;
@@ -1068,15 +1066,15 @@ int fused_steps(twoldg_problem* p, double dt, int64_t nsteps) {
   // buffers (ghosted form)
   auto ghosts = [&](int in) {
     if constexpr (GHOST) {
-      for (const auto& g : p->fgroups)
-        hipLaunchKernelGGL((k_face_ghost_u<T>), dim3(grid_for(g.nrows, 256)), dim3(256), 0,
-                           p->stream, g.nrows, g.count, p->frows + g.row0, p->fsrc + g.entry0,
-                           (const T*)p->fwu + g.entry0, (const T*)p->uf[in], (T*)p->fgu);
-      for (const auto& g : p->fgroups)
-        hipLaunchKernelGGL((k_face_ghost_g<T, P1>), dim3(grid_for(g.nrows, 256)), dim3(256), 0,
-                           p->stream, g.nrows, g.count, (int64_t)N * N, p->frows + g.row0,
-                           p->fsrc + g.entry0, (const T*)p->fwg + g.entry0, (const T*)p->pf[in],
-                           (const T*)p->fgu, p->spacing, (T*)p->fgg);
+      const int64_t nfr = p->nfrows;
+      if (nfr == 0) return;
+      hipLaunchKernelGGL((k_face_ghost_u<T, P1>), dim3(grid_for(nfr, 256)), dim3(256), 0,
+                         p->stream, nfr, p->fgroups, (int64_t)N * N, p->frows, p->fsrc,
+                         (const T*)p->fwu, (const T*)p->uf[in], (const T*)p->pf[in], p->spacing,
+                         (T*)p->fgu, (T*)p->ffg);
+      hipLaunchKernelGGL((k_face_ghost_g<T>), dim3(grid_for(nfr, 256)), dim3(256), 0, p->stream,
+                         nfr, p->fgroups, p->frows, p->fsrc, (const T*)p->fwg,
+                         (const T*)p->ffg, (T*)p->fgg);
     }
   };
   auto io = [&](int in) {
