@@ -221,6 +221,29 @@ def test_fused_refined_matches_oracle(case):
         assert np.linalg.norm(integ.u - y) <= 1e-13 * np.linalg.norm(y)
 
 
+@pytest.mark.parametrize("case", [FUSED[0], FUSED_REFINED[0]], ids=_id)
+def test_fused_step_graph_replays_match_eager_steps(case):
+    """n steps in one call replay the captured step graph; one step per call runs
+    the launches eagerly.  Several calls with eager work and synchronization in
+    between reuse the graph; every result equals the eager one bitwise."""
+    forest = T.wave_forest(case["D"], case["P"], case["E"], roots=case["roots"],
+                           reflecting=case.get("reflecting", False),
+                           refined=list(case["regions"]) if case.get("regions") else False)
+    a = T.WaveProblem(forest, T.Element(case["P"]))
+    b = T.WaveProblem(forest, T.Element(case["P"]))
+    assert a.info()["fused"]
+    u0 = T.initial_state(a, T.wave_exact(3, 1.0, 1, 0.0))
+    dt = T.oldg_timestep(a, 0.5)
+    ia = T.wave_integrator(a, u0, 0.0, 9 * dt, dt)
+    ib = T.wave_integrator(b, u0, 0.0, 9 * dt, dt)
+    for n in (3, 2, 4):
+        ia.step(n)                      # graph replays
+        for _ in range(n):
+            ib.step(1)                  # eager launches
+        T.wave_energy(b, ib.u)          # other work on the device between calls
+        assert np.array_equal(ia.u, ib.u), n
+
+
 def test_fused_large_forests_each_first_problem():
     """Large forests of small blocks (one wave per workgroup, several
     workgroups per CU), each forest's first problem and a rebuilt one, one step

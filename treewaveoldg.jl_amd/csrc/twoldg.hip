@@ -64,6 +64,8 @@ void destroy(twoldg_problem* p) {
                   p->pf[1], (void*)p->frows, (void*)p->fsrc, p->fwu,
                   p->fwg, p->fgu, p->fgg, p->ffg})
     dfree(b);
+  if (p->step_exec) (void)hipGraphExecDestroy(p->step_exec);
+  if (p->step_graph) (void)hipGraphDestroy(p->step_graph);
   for (hipEvent_t e : p->tev) (void)hipEventDestroy(e);
   if (p->ev0) (void)hipEventDestroy(p->ev0);
   if (p->ev1) (void)hipEventDestroy(p->ev1);

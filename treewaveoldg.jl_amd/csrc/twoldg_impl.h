@@ -702,6 +702,10 @@ struct twoldg_problem {
   void* fgu = nullptr;
   void* fgg = nullptr;
   void* ffg = nullptr;         // the coarse-fine rows' own face gradients
+  // the fused RK4 step captured as a graph (replayed for every step of a call)
+  hipGraph_t step_graph = nullptr;
+  hipGraphExec_t step_exec = nullptr;
+  double step_dt = 0;
   // per-launch timing of the stage kernels (bench): event pairs
   bool timing = false;
   std::vector<hipEvent_t> tev;
@@ -1083,7 +1087,7 @@ int fused_steps(twoldg_problem* p, double dt, int64_t nsteps) {
     A.uf_out = (T*)p->uf[1 - in];
     A.pf_out = (T*)p->pf[1 - in];
   };
-  for (int64_t s = 0; s < nsteps; ++s) {
+  auto one_step = [&]() {
     A.ys = yn;
     A.yo = (T*)p->ys;
     A.a = (T)(dt / 2);
@@ -1114,6 +1118,24 @@ int fused_steps(twoldg_problem* p, double dt, int64_t nsteps) {
     else
       hipLaunchKernelGGL((k_fused3<T, P1, E, 3, GHOST>), dim3((unsigned)nb), dim3(threads), 0, p->stream, A);
     tmark(p);
+  };
+  if (p->timing || nsteps < 2) {
+    for (int64_t s = 0; s < nsteps; ++s) one_step();
+  } else {
+    // the step's launches replayed as one graph: they are the same every step
+    // (the stage buffers ping-pong within the step); rebuilt when dt changes
+    if (!p->step_exec || p->step_dt != dt) {
+      if (p->step_exec) (void)hipGraphExecDestroy(p->step_exec);
+      if (p->step_graph) (void)hipGraphDestroy(p->step_graph);
+      p->step_exec = nullptr;
+      p->step_graph = nullptr;
+      HIP_TRY(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
+      one_step();
+      HIP_TRY(hipStreamEndCapture(p->stream, &p->step_graph));
+      HIP_TRY(hipGraphInstantiate(&p->step_exec, p->step_graph, nullptr, nullptr, 0));
+      p->step_dt = dt;
+    }
+    for (int64_t s = 0; s < nsteps; ++s) HIP_TRY(hipGraphLaunch(p->step_exec, p->stream));
   }
   HIP_TRY(hipGetLastError());
   return 0;
