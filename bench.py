@@ -21,6 +21,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "treewaveoldg.jl_amd"))
 
@@ -182,33 +184,43 @@ def config1():
                 nodes=nodes, note="6400 nodes: launch-bound, a parity run")
 
 
-def config3(large=False, steps=20):
+def config3(large=False, steps=500):
     """Reflecting box with nested cubes (3 levels, finest half-width 1/8): P=3,
-    E=2, roots=8 (1408 blocks), or the large refined forest P=4, E=4, roots=16
-    (11264 blocks of 20^3) for the bandwidth figure; cos mode m=2, FP64 and
-    FP32c; fused stages with the coarse-fine face ghosts formed by two
-    face-sized launches per stage."""
+    E=2, roots=8 (1408 blocks), 500 steps at the finest level's CFL (0.5); or
+    the large refined forest P=4, E=4, roots=16 (11264 blocks of 20^3) for the
+    bandwidth figure.  Cos mode m=2, FP64 and FP32c; fused stages with the
+    coarse-fine face ghosts formed by two face-sized launches per stage.
+    Reported: throughput over the steps after a two-step warm-up, the energy
+    drift over all steps, and FP32c's L2 distance from the FP64 run."""
     import twoldg as T
     P, E, roots = (4, 4, 16) if large else (3, 2, 8)
     out = {}
     forest = T.wave_forest(3, P, E, roots=roots, reflecting=True,
                            refined=T.nested_cubes(3, 1 / 8, 3))
     nodes = forest.nblocks * forest.N ** 3
+    warm = 2
+    final = {}
     for prec in ("f64", "f32c"):
         prob = T.WaveProblem(forest, T.Element(P), precision=prec)
         u0 = T.initial_state(prob, T.reflect_exact(3, 1.0, 2, 0.0))
         dt = T.oldg_timestep(prob, 0.5)
         e0 = T.wave_energy(prob, u0)
-        wall, kms, nl, integ = timed_integrator(T, prob, u0, dt, steps)
+        wall, kms, nl, integ = timed_integrator(T, prob, u0, dt, steps - warm, warmup=warm)
         e1 = integ.energy()
-        alg = VALUES_PER_NODE_UPDATE[prec] * BYTES_PER_VALUE[prec] * nodes * steps
-        out[prec] = dict(value=nodes * steps / wall, ms_per_step=wall * 1e3 / steps,
+        timed = steps - warm
+        alg = VALUES_PER_NODE_UPDATE[prec] * BYTES_PER_VALUE[prec] * nodes * timed
+        final[prec] = integ.u
+        out[prec] = dict(value=nodes * timed / wall, ms_per_step=wall * 1e3 / timed,
                          alg_gbs_whole_step=alg / wall / 1e9, energy_drift=(e1 - e0) / e0,
-                         steps=steps + 1)
+                         steps=steps)
         log(f"[bench] config 3{'-large' if large else ''} {prec}: {forest.nblocks} blocks, "
-            f"{nodes} nodes, {nodes * steps / wall:.3e} node-updates/s, "
+            f"{steps} steps, {nodes * timed / wall:.3e} node-updates/s, "
             f"{alg / wall / 1e9:.0f} GB/s algorithmic, energy drift {(e1 - e0) / e0:.2e}")
         prob.close()
+    rel = float(np.linalg.norm(final["f32c"] - final["f64"]) / np.linalg.norm(final["f64"]))
+    out["f32c_vs_f64_rel_l2"] = rel
+    log(f"[bench] config 3{'-large' if large else ''}: FP32c vs FP64 after {steps} steps, "
+        f"relative L2 {rel:.2e}")
     out["nodes"] = nodes
     out["blocks"] = forest.nblocks
     return out
