@@ -420,54 +420,60 @@ __global__ void k_truncate(Shape sh, int E, const Tables<T>* __restrict__ tbp,
 
 // Per-element sqrt(g)-orthogonal projection (evolution.jl:384-425):
 // c = Psi^T (W_e a); U^T y = c; U z = y; a = Psi z.  D = 3.
+// One workgroup per element, a thread per node (and per mode): the mode sums
+// in node order, the two triangular solves column by column (a mode's value
+// once final is broadcast through LDS and subtracted by the modes it feeds),
+// then a thread per node forms the projected value.
 template <class T, int P1>
-__global__ void k_truncate_elem(Shape sh, int E, const int32_t* __restrict__ tslot,
-                                const T* __restrict__ tmodes, const T* __restrict__ tweights,
-                                const T* __restrict__ tchol, T* __restrict__ y) {
+__global__ void __launch_bounds__(((P1 * P1 * P1) + 63) / 64 * 64)
+k_truncate_elem(Shape sh, int E, const int32_t* __restrict__ tslot,
+                const T* __restrict__ tmodes, const T* __restrict__ tweights,
+                const T* __restrict__ tchol, T* __restrict__ y) {
   constexpr int P = P1 - 1;
   constexpr int n = P1 * P1 * P1, m = P * P * P;
+  __shared__ T sWA[n];
+  __shared__ T sc[m];
   const int64_t ne = (int64_t)E * E * E;
-  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= sh.nb * ne) return;
-  const int64_t b = g / ne, e = g % ne;
+  const int64_t b = blockIdx.x / ne, e = blockIdx.x % ne;
   const int s = tslot[b];
-  if (s < 0) return;
+  if (s < 0) return;  // the whole workgroup: a block with tensor truncation
+  const int k = threadIdx.x;
   const int N = sh.N;
   const int J0 = (int)(e % E), J1 = (int)((e / E) % E), J2 = (int)(e / ((int64_t)E * E));
   T* a = y + b * 2 * sh.nn + sh.nn;
   const int64_t base = (int64_t)J0 * P1 + (int64_t)N * (J1 * P1) + (int64_t)N * N * (J2 * P1);
   const T* w = tweights + ((int64_t)s * ne + e) * n;
   const T* U = tchol + ((int64_t)s * ne + e) * m * m;
-  auto at = [&](int k) -> int64_t {
-    const int k0 = k % P1, k1 = (k / P1) % P1, k2 = k / (P1 * P1);
+  auto at = [&](int q) -> int64_t {
+    const int k0 = q % P1, k1 = (q / P1) % P1, k2 = q / (P1 * P1);
     return base + k0 + (int64_t)N * k1 + (int64_t)N * N * k2;
   };
-  T c[m];
-#pragma unroll
-  for (int q = 0; q < m; ++q) c[q] = 0;
-  for (int k = 0; k < n; ++k) {
-    const T wa = w[k] * a[at(k)];
-#pragma unroll
-    for (int q = 0; q < m; ++q) c[q] += tmodes[k * m + q] * wa;
-  }
-#pragma unroll
+  if (k < n) sWA[k] = w[k] * a[at(k)];
+  __syncthreads();
+  T cq = 0;
+  if (k < m)
+    for (int i = 0; i < n; ++i) cq += tmodes[i * m + k] * sWA[i];
+  // U^T y = c, mode by mode
   for (int q = 0; q < m; ++q) {
-    T acc = c[q];
-#pragma unroll
-    for (int r = 0; r < q; ++r) acc -= U[r + m * q] * c[r];
-    c[q] = acc / U[q + m * q];
+    if (k == q) {
+      cq = cq / U[q + m * q];
+      sc[q] = cq;
+    }
+    __syncthreads();
+    if (k > q && k < m) cq -= U[q + m * k] * sc[q];
   }
-#pragma unroll
+  // U z = y, from the last mode down
   for (int q = m - 1; q >= 0; --q) {
-    T acc = c[q];
-#pragma unroll
-    for (int r = q + 1; r < m; ++r) acc -= U[q + m * r] * c[r];
-    c[q] = acc / U[q + m * q];
+    if (k == q) {
+      cq = cq / U[q + m * q];
+      sc[q] = cq;
+    }
+    __syncthreads();
+    if (k < q) cq -= U[k + m * q] * sc[q];
   }
-  for (int k = 0; k < n; ++k) {
+  if (k < n) {
     T z = 0;
-#pragma unroll
-    for (int q = 0; q < m; ++q) z += tmodes[k * m + q] * c[q];
+    for (int q = 0; q < m; ++q) z += tmodes[k * m + q] * sc[q];
     a[at(k)] = z;
   }
 }
@@ -1014,7 +1020,8 @@ struct Launch {
   }
   static void launch_elem(twoldg_problem* p, T* y, int64_t n) {
     if constexpr (D == 3 && P1 <= 5)
-      hipLaunchKernelGGL((k_truncate_elem<T, P1>), dim3(grid_for(n, 64)), dim3(64), 0, p->stream,
+      hipLaunchKernelGGL((k_truncate_elem<T, P1>), dim3((unsigned)n),
+                         dim3((P1 * P1 * P1 + 63) / 64 * 64), 0, p->stream,
                          p->sh, p->E, p->tslot, (const T*)p->tmodes, (const T*)p->tweights,
                          (const T*)p->tchol, y);
   }
