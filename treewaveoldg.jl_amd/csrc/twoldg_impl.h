@@ -362,59 +362,46 @@ __global__ void k_stage_cart(Shape sh, const Tables<T>* __restrict__ tbp,
 // ---------------------------------------------------------------------------
 // Truncation: per element, for d = 1..D and every line along d,
 // c = dual . a, a -= mode c (evolution.jl:238-264).  v of the state.
+// One workgroup per block, a thread per line of the current direction (the
+// block's lines along d, all elements), a barrier between directions.
 // ---------------------------------------------------------------------------
+constexpr int kTruncThreads = 256;
+
 template <class T, int D, int P1>
-__global__ void k_truncate(Shape sh, int E, const Tables<T>* __restrict__ tbp,
-                           const int32_t* __restrict__ tslot, T* __restrict__ y) {
-  int64_t ne = E;
-  if (D >= 2) ne *= E;
-  if (D >= 3) ne *= E;
-  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= sh.nb * ne) return;
-  const int64_t b = g / ne, e = g % ne;
+__global__ void __launch_bounds__(kTruncThreads)
+k_truncate(Shape sh, int E, const Tables<T>* __restrict__ tbp,
+           const int32_t* __restrict__ tslot, T* __restrict__ y) {
+  const int64_t b = blockIdx.x;
   if (tslot && tslot[b] >= 0) return;  // per-element tables (k_truncate_elem)
   const Tables<T>& tb = *tbp;
   const int N = sh.N;
-  int J[kMaxD] = {(int)(e % E), 0, 0};
-  if (D >= 2) J[1] = (int)((e / E) % E);
-  if (D >= 3) J[2] = (int)(e / ((int64_t)E * E));
   T* a = y + b * 2 * sh.nn + sh.nn;
-  int64_t base = 0, strides[kMaxD] = {1, N, (int64_t)N * N};
-#pragma unroll
-  for (int d = 0; d < D; ++d) base += (int64_t)J[d] * P1 * strides[d];
+  const int64_t strides[kMaxD] = {1, N, (int64_t)N * N};
+  const int64_t nlines = sh.nn / P1;  // lines along one direction: N^(D-1) E
 #pragma unroll
   for (int d = 0; d < D; ++d) {
-    // tangential strides
-    int64_t s1 = 0, s2 = 0;
-    int n1 = 1, n2 = 1;
-    {
-      int k = 0;
+    for (int64_t l = threadIdx.x; l < nlines; l += blockDim.x) {
+      // line l: element index along d (l % E), then the tangential node indices
+      const int ed = (int)(l % E);
+      int64_t rest = l / E, base = (int64_t)ed * P1 * strides[d];
 #pragma unroll
       for (int q = 0; q < D; ++q)
         if (q != d) {
-          if (k == 0) {
-            s1 = strides[q];
-            n1 = P1;
-          } else {
-            s2 = strides[q];
-            n2 = P1;
-          }
-          ++k;
+          base += (rest % N) * strides[q];
+          rest /= N;
         }
-    }
-    for (int k2 = 0; k2 < n2; ++k2)
-      for (int k1 = 0; k1 < n1; ++k1) {
-        T* line = a + base + k1 * s1 + k2 * s2;
-        T vals[P1];
-        T c = 0;
+      T* line = a + base;
+      T vals[P1];
+      T c = 0;
 #pragma unroll
-        for (int j = 0; j < P1; ++j) {
-          vals[j] = line[j * strides[d]];
-          c += tb.dual[j] * vals[j];
-        }
-#pragma unroll
-        for (int i = 0; i < P1; ++i) line[i * strides[d]] = vals[i] - tb.mode[i] * c;
+      for (int j = 0; j < P1; ++j) {
+        vals[j] = line[j * strides[d]];
+        c += tb.dual[j] * vals[j];
       }
+#pragma unroll
+      for (int i = 0; i < P1; ++i) line[i * strides[d]] = vals[i] - tb.mode[i] * c;
+    }
+    if (d + 1 < D) __syncthreads();
   }
 }
 
@@ -1013,8 +1000,8 @@ struct Launch {
     if (!p->truncate) return;
     const int64_t ne = ipow(p->E, D);
     const int64_t n = p->sh.nb * ne;
-    hipLaunchKernelGGL((k_truncate<T, D, P1>), dim3(grid_for(n, 128)), dim3(128), 0, p->stream,
-                       p->sh, p->E, (const Tables<T>*)p->tables, p->tslot, y);
+    hipLaunchKernelGGL((k_truncate<T, D, P1>), dim3((unsigned)p->sh.nb), dim3(kTruncThreads), 0,
+                       p->stream, p->sh, p->E, (const Tables<T>*)p->tables, p->tslot, y);
     if (D == 3 && p->has_tables)
       launch_elem(p, y, n);
   }
