@@ -45,12 +45,20 @@ constexpr int fused_waves_per_eu() {
 }
 
 // Whether the next element layer's column stays in registers from its first
-// read (FP32) or is read again (FP64, whose registers are spent).
+// read, or is read again.
 #ifndef TWOLDG_FUSED_CARRY_F64
-#define TWOLDG_FUSED_CARRY_F64 0
+#define TWOLDG_FUSED_CARRY_F64 1
 #endif
 #ifndef TWOLDG_FUSED_CARRY_F32
 #define TWOLDG_FUSED_CARRY_F32 1
+#endif
+// Whether the stage update's outputs wait in LDS (in the slots of the Laplacian
+// sums it has just consumed) and are stored by the columns in the last phase.
+// The vector memory counter covers stores as well as loads, so stores issued
+// between the stage operand loads make every wait for an operand wait for them
+// too.
+#ifndef TWOLDG_FUSED_DEFER
+#define TWOLDG_FUSED_DEFER 1
 #endif
 #ifndef TWOLDG_FUSED_NT
 #define TWOLDG_FUSED_NT 0
@@ -79,6 +87,15 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
   __builtin_amdgcn_s_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// A read of a table no kernel of the launch writes (neighbour codes, spacings)
+// at an address the whole workgroup shares: through the constant address space
+// it becomes a scalar load, waited for with the LDS counter instead of the
+// vector memory counter that also covers the stage's outstanding stores.
+template <class T>
+__device__ __forceinline__ T ld_const(const T* p) {
+  return *(const __attribute__((address_space(4))) T*)p;
 }
 
 template <class T>
@@ -169,7 +186,7 @@ k_fused3(FusedArgs<T> A) {
     return v;
   };
 
-  const T h = (T)A.spacing[b];
+  const T h = (T)ld_const(A.spacing + b);
   const T rh = T(1) / h;
   const int32_t* nb6 = A.nbr + b * 6;
   // the block's arrays: uniform bases, per-thread 32-bit offsets (z stride N2,
@@ -183,7 +200,10 @@ k_fused3(FusedArgs<T> A) {
   T* lo = A.lo ? A.lo + b * 2 * nn : nullptr;
   T* ufo = A.uf_out + b * 6 * N2;
   T* pfo = A.pf_out + b * 6 * N2;
-  const int nxl = nb6[0], nxh = nb6[1], nyl = nb6[2], nyh = nb6[3];
+  const int nxl = ld_const(nb6 + 0), nxh = ld_const(nb6 + 1), nyl = ld_const(nb6 + 2),
+            nyh = ld_const(nb6 + 3), nzl = ld_const(nb6 + 4), nzh = ld_const(nb6 + 5);
+  // the reciprocal spacing of a same-level neighbour, read where it is used
+  auto rsp = [&](int nb) { return T(1) / (T)ld_const(A.spacing + nb); };
   constexpr int CF = -2;  // a coarse-fine face
 
   auto load_layer = [&](int e, T* dst) {
@@ -254,14 +274,8 @@ k_fused3(FusedArgs<T> A) {
     for (int j = 0; j < P1; ++j) v[j] -= tb.mode[j] * c;
   };
 
-  T uc[P1];
-  T prev_last = 0, prev_part = 0;  // previous layer: last u, row-P partial sum
-  load_layer(0, uc);
-
-#pragma unroll 1
-  for (int e = 0; e < E; ++e) {
-    // ---- the layer's x/y ghosts: the neighbours' face values, issued first ----
-    T gv[GPT];
+  // a layer's x/y ghosts: the neighbours' face values (two per thread)
+  auto load_ghosts = [&](int e, T* gv) {
 #pragma unroll
     for (int k = 0; k < GPT; ++k) {
       const int idx = tid + k * NTH;
@@ -276,23 +290,36 @@ k_fused3(FusedArgs<T> A) {
           gv[k] = (q ? A.gg : A.gu)[((int64_t)b * 6 + f) * N2 + e * P1 * N + t];
       }
     }
+  };
+  constexpr bool carry = sizeof(T) == 8 ? TWOLDG_FUSED_CARRY_F64 : TWOLDG_FUSED_CARRY_F32;
+  constexpr bool defer = TWOLDG_FUSED_DEFER;
+
+  T uc[P1];   // the layer's column
+  T un2[P1];  // the next layer's column
+  T gv[GPT];  // the layer's x/y ghosts
+  T prev_last = 0, prev_part = 0;  // previous layer: last u, row-P partial sum
+  load_layer(0, uc);
+
+#pragma unroll 1
+  for (int e = 0; e < E; ++e) {
+    // ---- the layer's x/y ghosts, issued first ----
+    load_ghosts(e, gv);
 
     // ---- (1) columns: z-gradients (one node either side) and z-Laplacian ----
-    T un2[P1];  // the next layer's column
     {
       // the z ghosts: the next layer's first u and its row-0 partial sum, or
       // on the block's z faces the neighbour's face u and face gradient (a
       // mirror's ghost u is the own face u, its gradient minus the own face
       // gradient)
       T before = prev_last, after, gbefore = 0, gafter = 0, next_part = 0;
-      if (GHOST && e == 0 && nb6[4] == CF) {
+      if (GHOST && e == 0 && nzl == CF) {
         before = act ? A.gu[((int64_t)b * 6 + 4) * N2 + col] : T(0);
         gbefore = act ? A.gg[((int64_t)b * 6 + 4) * N2 + col] : T(0);
       } else if (e == 0) {
-        const int nlo = nb6[4];
+        const int nlo = nzl;
         before = (act && nlo >= 0) ? A.uf_in[((int64_t)nlo * 6 + 5) * N2 + col] : uc[0];
         if (act && nlo >= 0) {
-          gbefore = (A.pf_in[((int64_t)nlo * 6 + 5) * N2 + col] + Whi * uc[0]) * (T(1) / (T)A.spacing[nlo]);
+          gbefore = (A.pf_in[((int64_t)nlo * 6 + 5) * N2 + col] + Whi * uc[0]) * rsp(nlo);
         } else {
           T g0 = Wlo * before;
 #pragma unroll
@@ -305,14 +332,14 @@ k_fused3(FusedArgs<T> A) {
         after = un2[0];
 #pragma unroll
         for (int j = 0; j < P1; ++j) next_part += tb.W[0][j + 1] * un2[j];
-      } else if (GHOST && nb6[5] == CF) {
+      } else if (GHOST && nzh == CF) {
         after = act ? A.gu[((int64_t)b * 6 + 5) * N2 + col] : T(0);
         gafter = act ? A.gg[((int64_t)b * 6 + 5) * N2 + col] : T(0);
       } else {
-        const int nhi = nb6[5];
+        const int nhi = nzh;
         after = (act && nhi >= 0) ? A.uf_in[((int64_t)nhi * 6 + 4) * N2 + col] : uc[P];
         if (act && nhi >= 0) {
-          gafter = (A.pf_in[((int64_t)nhi * 6 + 4) * N2 + col] + Wlo * uc[P]) * (T(1) / (T)A.spacing[nhi]);
+          gafter = (A.pf_in[((int64_t)nhi * 6 + 4) * N2 + col] + Wlo * uc[P]) * rsp(nhi);
         } else {
           T gN = Whi * after;
 #pragma unroll
@@ -389,11 +416,11 @@ k_fused3(FusedArgs<T> A) {
             ex, [&](int c) { return Ur[c]; }, [&]() { return GH[0][g]; }, [&]() { return GH[1][g]; },
             [&](T uf, T gf) {
               if (GHOST && nxl == CF) return GH[4][g];
-              return nxl >= 0 ? (GH[4][g] + Whi * uf) * (T(1) / (T)A.spacing[nxl]) : -gf;
+              return nxl >= 0 ? (GH[4][g] + Whi * uf) * rsp(nxl) : -gf;
             },
             [&](T uf, T gf) {
               if (GHOST && nxh == CF) return GH[5][g];
-              return nxh >= 0 ? (GH[5][g] + Wlo * uf) * (T(1) / (T)A.spacing[nxh]) : -gf;
+              return nxh >= 0 ? (GH[5][g] + Wlo * uf) * rsp(nxh) : -gf;
             },
             lx);
 #pragma unroll
@@ -405,11 +432,11 @@ k_fused3(FusedArgs<T> A) {
           ey, [&](int c) { return Uc[c * N]; }, [&]() { return GH[2][g]; }, [&]() { return GH[3][g]; },
           [&](T uf, T gf) {
             if (GHOST && nyl == CF) return GH[6][g];
-            return nyl >= 0 ? (GH[6][g] + Whi * uf) * (T(1) / (T)A.spacing[nyl]) : -gf;
+            return nyl >= 0 ? (GH[6][g] + Whi * uf) * rsp(nyl) : -gf;
           },
           [&](T uf, T gf) {
             if (GHOST && nyh == CF) return GH[7][g];
-            return nyh >= 0 ? (GH[7][g] + Wlo * uf) * (T(1) / (T)A.spacing[nyh]) : -gf;
+            return nyh >= 0 ? (GH[7][g] + Wlo * uf) * rsp(nyh) : -gf;
           },
           ly);
     }
@@ -426,7 +453,7 @@ k_fused3(FusedArgs<T> A) {
       auto ops = [&](int j, T& ku, T& yu, T& yv, T& au, T& av) {
         const int iu = base + j * N, iv = iu + nn;
         ku = ld_once(vs + iu);
-        yu = ld_once(un_ + iu);                          // compensated form: the high word
+        yu = ld_once(un_ + iu);                                 // compensated form: the high word
         yv = MODE == 4 ? ld_once(lo + iu) : ld_once(un_ + iv);  // compensated form: the low word
         au = MODE >= 2 ? ld_once(acc + iu) : T(0);
         av = MODE >= 2 ? ld_once(acc + iv) : T(0);
@@ -445,14 +472,23 @@ k_fused3(FusedArgs<T> A) {
         const T lap = ((T(0) + LX[l]) + ly[j]) + LZ[l];
         const T kv = lap * rh;
         T unext;
+        // the accumulator (modes 1, 2) or the compensation of u (mode 4): stored
+        // now, or left in LX and LZ at l (read above by this thread only)
+        auto out = [&](T* pu, T* pv, T ou, T ov) {
+          if (defer) {
+            LX[l] = ou;
+            if (pv) LZ[l] = ov;
+          } else {
+            st_once(pu + iu, ou);
+            if (pv) st_once(pv + iu, ov);
+          }
+        };
         if (MODE == 1) {
-          st_once(acc + iu, ku);
-          st_once(acc + iv, kv);
+          out(acc, acc + nn, ku, kv);
           unext = yu + A.a * ku;
           w[j] = yv + A.a * kv;
         } else if (MODE == 2) {
-          st_once(acc + iu, au + T(2) * ku);
-          st_once(acc + iv, av + T(2) * kv);
+          out(acc, acc + nn, au + T(2) * ku, av + T(2) * kv);
           unext = yu + A.a * ku;
           w[j] = yv + A.a * kv;
         } else if (MODE == 3) {
@@ -461,25 +497,27 @@ k_fused3(FusedArgs<T> A) {
         } else {  // compensated: u by Kahan now, v's increment truncated first
           const T du = A.a * (au + ku);
           const T yk = du - yv, tu = yu + yk;
-          st_once(lo + iu, (tu - yu) - yk);
+          out(lo, nullptr, (tu - yu) - yk, T(0));
           unext = tu;
           w[j] = A.a * (av + kv);
         }
-        st_once(yo + iu, unext);
+        if (!defer) st_once(yo + iu, unext);
         UN[l] = unext;
         // faces of the next u along y
         if (ey == 0) {
           plo += tb.W[0][j + 1] * unext;
-          if (j == 0) ufo[2 * N2 + xy + N * z] = unext;
+          if (!defer && j == 0) ufo[2 * N2 + xy + N * z] = unext;
         }
         if (ey == E - 1) {
           phi += tb.W[P][j + 1] * unext;
-          if (j == P) ufo[3 * N2 + xy + N * z] = unext;
+          if (!defer && j == P) ufo[3 * N2 + xy + N * z] = unext;
         }
         if (ahead) {
           ku = ku1; yu = yu1; yv = yv1; au = au1; av = av1;
         }
       }
+      if (defer && ey == 0) ufo[2 * N2 + xy + N * z] = UN[py * NT + xy];
+      if (defer && ey == E - 1) ufo[3 * N2 + xy + N * z] = UN[py * NT + (N - 1) * N + xy];
       if (ey == 0) pfo[2 * N2 + xy + N * z] = plo;
       if (ey == E - 1) pfo[3 * N2 + xy + N * z] = phi;
       if (A.truncate) truncate_line(w);
@@ -542,6 +580,9 @@ k_fused3(FusedArgs<T> A) {
     }
     lds_barrier();
     // ---- (5) columns: truncation along z in registers; v out ----
+    // The next layer's column is read again (registers allowing no carry) ahead
+    // of this phase's stores, so that waiting for it does not wait for them.
+    if (!carry && e + 1 < E) load_layer(e + 1, uc);
     if (act) {
       T wl[P1];
 #pragma unroll
@@ -550,6 +591,15 @@ k_fused3(FusedArgs<T> A) {
 #pragma unroll
       for (int j = 0; j < P1; ++j) {
         const int iv = col + (e * P1 + j) * N2 + nn;
+        if (defer) {  // the stage update's outputs of the column
+          const int li = j * NT + col;
+          st_once(yo + iv - nn, UN[li]);
+          if (MODE == 1 || MODE == 2) {
+            st_once(acc + iv - nn, LX[li]);
+            st_once(acc + iv, LZ[li]);
+          }
+          if (MODE == 4) st_once(lo + iv - nn, LX[li]);
+        }
         if (MODE == 4) {
           const T yk = wl[j] - lov[j], tv = hv[j] + yk;
           st_once(lo + iv, (tv - hv[j]) - yk);
@@ -560,13 +610,9 @@ k_fused3(FusedArgs<T> A) {
       }
     }
     // slide the column window
-    if (e + 1 < E) {
-      if (sizeof(T) == 8 ? TWOLDG_FUSED_CARRY_F64 : TWOLDG_FUSED_CARRY_F32) {
+    if (carry && e + 1 < E) {
 #pragma unroll
-        for (int j = 0; j < P1; ++j) uc[j] = un2[j];
-      } else {
-        load_layer(e + 1, uc);
-      }
+      for (int j = 0; j < P1; ++j) uc[j] = un2[j];
     }
   }
 }
