@@ -282,6 +282,16 @@ def config5(steps=20):
     return out
 
 
+# The side workloads by name, each at the size the default run reports.
+WORKLOADS = {
+    "config1": lambda: config1(),
+    "config3": lambda: config3(),
+    "config3-large": lambda: config3(large=True, steps=10),
+    "config4": lambda: config4(),
+    "config5": lambda: config5(),
+}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -293,11 +303,16 @@ def main():
     ap.add_argument("--single", action="store_true", help="time only --precision's form")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip configs 1, 3, 4 and 5 (reported beside the main line)")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS),
+                    help="run only this side workload and print its result (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for "
               f"--gpus {args.gpus})", file=sys.stderr)
         sys.exit(2)
+    if args.workload:
+        print(json.dumps(WORKLOADS[args.workload](), default=float))
+        return
     main_res = gpu_run(args, args.precision)
     other = "f32c" if args.precision == "f64" else "f64"
     other_res = {} if args.single else gpu_run(args, other)

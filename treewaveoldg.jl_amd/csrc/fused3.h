@@ -101,7 +101,7 @@ __device__ __forceinline__ T ld_const(const T* p) {
 template <class T>
 struct FusedArgs {
   const T* ys;       // stage input (u, v)
-  const T* yn;       // step start
+  const T* yn;       // step start (the same as ys in mode 1)
   T* acc;            // RK accumulator
   T* yo;             // output state
   T* lo;             // compensation of yn (final stage, compensated form)
@@ -453,8 +453,15 @@ k_fused3(FusedArgs<T> A) {
       auto ops = [&](int j, T& ku, T& yu, T& yv, T& au, T& av) {
         const int iu = base + j * N, iv = iu + nn;
         ku = ld_once(vs + iu);
-        yu = ld_once(un_ + iu);                                 // compensated form: the high word
-        yv = MODE == 4 ? ld_once(lo + iu) : ld_once(un_ + iv);  // compensated form: the low word
+        if (MODE == 1) {
+          // stage 1: the stage input is the step start (ys = yn), whose u is
+          // the layer's plane in LDS (this thread's node, not yet overwritten)
+          yu = U[py * NT + (ey * P1 + j) * N + xy];
+          yv = ku;
+        } else {
+          yu = ld_once(un_ + iu);                                 // compensated form: the high word
+          yv = MODE == 4 ? ld_once(lo + iu) : ld_once(un_ + iv);  // compensated form: the low word
+        }
         au = MODE >= 2 ? ld_once(acc + iu) : T(0);
         av = MODE >= 2 ? ld_once(acc + iv) : T(0);
       };

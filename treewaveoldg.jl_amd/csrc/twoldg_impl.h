@@ -1082,7 +1082,7 @@ int fused_steps(twoldg_problem* p, double dt, int64_t nsteps) {
     A.pf_out = (T*)p->pf[1 - in];
   };
   auto one_step = [&]() {
-    A.ys = yn;
+    A.ys = yn;  // stage 1 (mode 1) relies on ys = yn
     A.yo = (T*)p->ys;
     A.a = (T)(dt / 2);
     io(0);
