@@ -88,18 +88,39 @@ __device__ void batch_blocks(const BatchArgs<T>& A, const BatchItem& it, int N, 
   const int sx = x - ix - 1, sy = y - iy - 1;
   auto fidx = [&](int64_t blk, int f, int t) { return A.foff[blk] + (int64_t)f * N + t; };
 
-  // (1) u and its ghosts; the stage operands
-  T uz = 0, ku = 0, yu = 0, yv = 0, au = 0, av = 0;
+  // (1) u and its ghosts; every global operand of the stage, so that no later
+  // wait for a load also waits for the stage's stores (the vector memory
+  // counter covers both)
+  T uz = 0, ku = 0, yu = 0, yv = 0, au = 0, av = 0, lu = 0, lv = 0;
+  // an edge node's neighbour across its x (y) face: partial face gradient, 1/h
+  T pgx = 0, rgx = 0, pgy = 0, rgy = 0;
   if (act) {
     uz = A.ys[iu];
     ku = A.ys[iv];
-    if (MODE != 4) {
+    if (MODE == 1) {  // stage 1: the stage input is the step start
+      yu = uz;
+      yv = ku;
+    } else {  // the compensated form: the high words
       yu = A.yn[iu];
       yv = A.yn[iv];
+    }
+    if (MODE == 4) {
+      lu = A.lo[iu];
+      lv = A.lo[iv];
     }
     if (MODE >= 2) {
       au = A.acc[iu];
       av = A.acc[iv];
+    }
+    const int fx = x == 0 ? nxl : (x == N - 1 ? nxh : -1);
+    const int fy = y == 0 ? nyl : (y == N - 1 ? nyh : -1);
+    if (fx >= 0) {
+      pgx = A.pf_in[fidx(fx, x == 0 ? 1 : 0, y)];
+      rgx = T(1) / (T)A.spacing[fx];
+    }
+    if (fy >= 0) {
+      pgy = A.pf_in[fidx(fy, y == 0 ? 3 : 2, x)];
+      rgy = T(1) / (T)A.spacing[fy];
     }
     U[(y + 1) * NP + x + 1] = uz;
     if (x == 0) U[(y + 1) * NP] = nxl >= 0 ? A.uf_in[fidx(nxl, 1, y)] : uz;
@@ -120,10 +141,10 @@ __device__ void batch_blocks(const BatchArgs<T>& A, const BatchItem& it, int N, 
     gy *= rh;
     GX[y * NP + x + 1] = gx;
     GY[(y + 1) * N + x] = gy;
-    if (x == 0) GX[y * NP] = nxl >= 0 ? (A.pf_in[fidx(nxl, 1, y)] + Whi * uz) * (T(1) / (T)A.spacing[nxl]) : -gx;
-    if (x == N - 1) GX[y * NP + N + 1] = nxh >= 0 ? (A.pf_in[fidx(nxh, 0, y)] + Wlo * uz) * (T(1) / (T)A.spacing[nxh]) : -gx;
-    if (y == 0) GY[x] = nyl >= 0 ? (A.pf_in[fidx(nyl, 3, x)] + Whi * uz) * (T(1) / (T)A.spacing[nyl]) : -gy;
-    if (y == N - 1) GY[(N + 1) * N + x] = nyh >= 0 ? (A.pf_in[fidx(nyh, 2, x)] + Wlo * uz) * (T(1) / (T)A.spacing[nyh]) : -gy;
+    if (x == 0) GX[y * NP] = nxl >= 0 ? (pgx + Whi * uz) * rgx : -gx;
+    if (x == N - 1) GX[y * NP + N + 1] = nxh >= 0 ? (pgx + Wlo * uz) * rgx : -gx;
+    if (y == 0) GY[x] = nyl >= 0 ? (pgy + Whi * uz) * rgy : -gy;
+    if (y == N - 1) GY[(N + 1) * N + x] = nyh >= 0 ? (pgy + Wlo * uz) * rgy : -gy;
   }
   lds_barrier();
   // (3) the Laplacian and the stage combination
@@ -151,8 +172,8 @@ __device__ void batch_blocks(const BatchArgs<T>& A, const BatchItem& it, int N, 
       w = yv + a * (av + kv);
     } else {
       const T du = a * (au + ku);
-      const T yk = du - A.lo[iu], hu = A.yn[iu], tu = hu + yk;
-      A.lo[iu] = (tu - hu) - yk;
+      const T yk = du - lu, tu = yu + yk;
+      A.lo[iu] = (tu - yu) - yk;
       unext = tu;
       w = a * (av + kv);
     }
@@ -211,8 +232,8 @@ __device__ void batch_blocks(const BatchArgs<T>& A, const BatchItem& it, int N, 
       vv -= ltab[P1 * PW + iy] * c;
     }
     if (MODE == 4) {
-      const T yk = vv - A.lo[iv], hv = A.yn[iv], tv = hv + yk;
-      A.lo[iv] = (tv - hv) - yk;
+      const T yk = vv - lv, tv = yv + yk;
+      A.lo[iv] = (tv - yv) - yk;
       A.yo[iv] = tv;
     } else {
       A.yo[iv] = vv;
