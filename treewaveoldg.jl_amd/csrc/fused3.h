@@ -79,16 +79,6 @@ __device__ __forceinline__ void st_once(T* p, T v) {
     *p = v;
 }
 
-// A workgroup barrier that orders LDS only.  __syncthreads() also fences global
-// memory, which makes every wave wait for its outstanding global loads at each
-// barrier and so serializes the plane's loads with the LDS phases; the fused
-// stage shares nothing through global memory inside a launch.
-__device__ __forceinline__ void lds_barrier() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-  __builtin_amdgcn_s_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
-
 // A read of a table no kernel of the launch writes (neighbour codes, spacings)
 // at an address the whole workgroup shares: through the constant address space
 // it becomes a scalar load, waited for with the LDS counter instead of the

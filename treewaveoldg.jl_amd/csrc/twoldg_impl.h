@@ -405,6 +405,16 @@ k_truncate(Shape sh, int E, const Tables<T>* __restrict__ tbp,
   }
 }
 
+// A workgroup barrier that orders LDS only.  __syncthreads() also fences global
+// memory, which makes every wave wait for its outstanding global loads at each
+// barrier and so serializes the plane's loads with the LDS phases; the kernels
+// that use it share nothing through global memory inside a launch.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Per-element sqrt(g)-orthogonal projection (evolution.jl:384-425):
 // c = Psi^T (W_e a); U^T y = c; U z = y; a = Psi z.  D = 3.
 // One workgroup per element, a thread per node (and per mode): the mode sums
@@ -420,6 +430,7 @@ k_truncate_elem(Shape sh, int E, const int32_t* __restrict__ tslot,
   constexpr int n = P1 * P1 * P1, m = P * P * P;
   __shared__ T sWA[n];
   __shared__ T sc[m];
+  __shared__ T sU[m * m];  // the Cholesky factor, read at every step of the solves
   const int64_t ne = (int64_t)E * E * E;
   const int64_t b = blockIdx.x / ne, e = blockIdx.x % ne;
   const int s = tslot[b];
@@ -435,28 +446,29 @@ k_truncate_elem(Shape sh, int E, const int32_t* __restrict__ tslot,
     const int k0 = q % P1, k1 = (q / P1) % P1, k2 = q / (P1 * P1);
     return base + k0 + (int64_t)N * k1 + (int64_t)N * N * k2;
   };
+  for (int i = k; i < m * m; i += blockDim.x) sU[i] = U[i];
   if (k < n) sWA[k] = w[k] * a[at(k)];
   __syncthreads();
   T cq = 0;
   if (k < m)
     for (int i = 0; i < n; ++i) cq += tmodes[i * m + k] * sWA[i];
-  // U^T y = c, mode by mode
+  // U^T y = c, mode by mode (LDS only between the steps)
   for (int q = 0; q < m; ++q) {
     if (k == q) {
-      cq = cq / U[q + m * q];
+      cq = cq / sU[q + m * q];
       sc[q] = cq;
     }
-    __syncthreads();
-    if (k > q && k < m) cq -= U[q + m * k] * sc[q];
+    lds_barrier();
+    if (k > q && k < m) cq -= sU[q + m * k] * sc[q];
   }
   // U z = y, from the last mode down
   for (int q = m - 1; q >= 0; --q) {
     if (k == q) {
-      cq = cq / U[q + m * q];
+      cq = cq / sU[q + m * q];
       sc[q] = cq;
     }
-    __syncthreads();
-    if (k < q) cq -= U[k + m * q] * sc[q];
+    lds_barrier();
+    if (k < q) cq -= sU[k + m * q] * sc[q];
   }
   if (k < n) {
     T z = 0;
