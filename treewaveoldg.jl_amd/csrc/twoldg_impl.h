@@ -83,15 +83,20 @@ __device__ __forceinline__ int64_t tlin(const int I[kMaxD], int d, int N) {
 // MODE 1: flat gradient g_d from grads (nb, D, nn), component d of the row
 // MODE 2: Cartesian gradient, all D components, into (rows/Nt, D, Nt)
 // ---------------------------------------------------------------------------
+struct ExchangeRows {
+  int64_t nrows;
+  const int32_t* rowptr;
+  const int32_t* cb;
+  const int32_t* cl;
+};
 template <class T, int MODE, int D>
-__global__ void k_exchange(int64_t nrows, Shape sh, const int32_t* __restrict__ rowptr,
-                           const int32_t* __restrict__ cb, const int32_t* __restrict__ cl,
-                           const T* __restrict__ coef, const T* __restrict__ src,
-                           T* __restrict__ ghost) {
-  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= nrows) return;
-  const int k0 = rowptr[r], k1 = rowptr[r + 1];
-  if (k0 == k1) return;  // Dirichlet rows are written by their own kernel
+__device__ __forceinline__ void exchange_row(int64_t r, const Shape& sh, const ExchangeRows& X,
+                                             const T* __restrict__ coef,
+                                             const T* __restrict__ src, T* __restrict__ ghost) {
+  const int32_t* __restrict__ cb = X.cb;
+  const int32_t* __restrict__ cl = X.cl;
+  const int k0 = X.rowptr[r], k1 = X.rowptr[r + 1];
+  if (k0 == k1) return;  // a Dirichlet row: written by dirichlet_row
   if (MODE == 0) {
     T acc = 0;
     for (int k = k0; k < k1; ++k) acc += coef[k] * src[(int64_t)cb[k] * 2 * sh.nn + cl[k]];
@@ -113,17 +118,28 @@ __global__ void k_exchange(int64_t nrows, Shape sh, const int32_t* __restrict__ 
     }
   }
 }
+template <class T, int MODE, int D>
+__global__ void k_exchange(Shape sh, ExchangeRows X, const T* __restrict__ coef,
+                           const T* __restrict__ src, T* __restrict__ ghost) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < X.nrows) exchange_row<T, MODE, D>(r, sh, X, coef, src, ghost);
+}
 
 // Dirichlet ghosts of a plane wave sin(k.x - w t + phi): u (MODE 0) or the
-// Cartesian gradient k cos(...) (MODE 2).  rows: u-shaped ghost rows.
+// Cartesian gradient k cos(...) (MODE 2).  rows: u-shaped ghost rows, whose
+// exchange rows are empty.
+struct DirichletRows {
+  int64_t nrows;
+  const int64_t* rows;
+  const double* x;  // (nrows, D) node positions
+  double k0, k1, k2, w, phi;
+};
 template <class T, int MODE, int D>
-__global__ void k_dirichlet(int64_t nrows, Shape sh, const int64_t* __restrict__ rows,
-                            const double* __restrict__ x, double k0, double k1, double k2,
-                            double w, double phi, double t, T* __restrict__ ghost) {
-  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= nrows) return;
-  const int64_t r = rows[q];
-  const double* xp = x + q * D;
+__device__ __forceinline__ void dirichlet_row(int64_t q, const Shape& sh, const DirichletRows& R,
+                                              double t, T* __restrict__ ghost) {
+  const double k0 = R.k0, k1 = R.k1, k2 = R.k2, w = R.w, phi = R.phi;
+  const int64_t r = R.rows[q];
+  const double* xp = R.x + q * D;
   double sx = k0 * xp[0];
   if (D >= 2) sx += k1 * xp[1];
   if (D >= 3) sx += k2 * xp[2];
@@ -137,6 +153,17 @@ __global__ void k_dirichlet(int64_t nrows, Shape sh, const int64_t* __restrict__
 #pragma unroll
     for (int i = 0; i < D; ++i) ghost[(face * D + i) * sh.Nt + tt] = (T)(kk[i] * c);
   }
+}
+// The exchange rows and the Dirichlet rows (disjoint ghosts) in one launch.
+template <class T, int MODE, int D>
+__global__ void k_exchange_dirichlet(Shape sh, ExchangeRows X, const T* __restrict__ coef,
+                                     const T* __restrict__ src, DirichletRows R, double t,
+                                     T* __restrict__ ghost) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < X.nrows)
+    exchange_row<T, MODE, D>(r, sh, X, coef, src, ghost);
+  else if (r - X.nrows < R.nrows)
+    dirichlet_row<T, MODE, D>(r - X.nrows, sh, R, t, ghost);
 }
 
 // The wide derivative of `read` along d at owned node I, in units of 1/h
@@ -957,15 +984,28 @@ inline void tmark(twoldg_problem* p) {
 // ---- launch helpers, dispatched on (D, P1) at run time ----------------------
 template <class T, int D, int P1>
 struct Launch {
-  static void exchange_u(twoldg_problem* p, const T* y, double t) {
+  static ExchangeRows rows(twoldg_problem* p) { return {p->nrows, p->rowptr, p->cb, p->cl}; }
+  static DirichletRows dir(twoldg_problem* p) {
+    return {p->ndir, p->dir_rows, p->dir_x, p->wave_k[0], p->wave_k[1], p->wave_k[2],
+            p->wave_w, p->wave_phi};
+  }
+  // the ghosts of src: exchange rows, and the Dirichlet rows at time t
+  template <int MODE>
+  static void ghosts(twoldg_problem* p, const T* coef, const T* src, double t, T* ghost) {
     if (p->nrows == 0) return;
-    hipLaunchKernelGGL((k_exchange<T, 0, D>), dim3(grid_for(p->nrows, 256)), dim3(256), 0,
-                       p->stream, p->nrows, p->sh, p->rowptr, p->cb, p->cl, (const T*)p->cval_u,
-                       y, (T*)p->ug);
-    if (p->ndir)
-      hipLaunchKernelGGL((k_dirichlet<T, 0, D>), dim3(grid_for(p->ndir, 256)), dim3(256), 0,
-                         p->stream, p->ndir, p->sh, p->dir_rows, p->dir_x, p->wave_k[0],
-                         p->wave_k[1], p->wave_k[2], p->wave_w, p->wave_phi, t, (T*)p->ug);
+    if constexpr (MODE != 1) {  // Dirichlet data come with the Cartesian gradients
+      if (p->ndir) {
+        hipLaunchKernelGGL((k_exchange_dirichlet<T, MODE, D>),
+                           dim3(grid_for(p->nrows + p->ndir, 256)), dim3(256), 0, p->stream,
+                           p->sh, rows(p), coef, src, dir(p), t, ghost);
+        return;
+      }
+    }
+    hipLaunchKernelGGL((k_exchange<T, MODE, D>), dim3(grid_for(p->nrows, 256)), dim3(256), 0,
+                       p->stream, p->sh, rows(p), coef, src, ghost);
+  }
+  static void exchange_u(twoldg_problem* p, const T* y, double t) {
+    ghosts<0>(p, (const T*)p->cval_u, y, t, (T*)p->ug);
   }
   static void gradient(twoldg_problem* p, const T* y, double t) {
     exchange_u(p, y, t);
@@ -974,20 +1014,12 @@ struct Launch {
       hipLaunchKernelGGL((k_gradient<T, D, P1>), dim3(grid_for(n, 256)), dim3(256), 0, p->stream,
                          p->sh, (const Tables<T>*)p->tables, p->spacing, y, (const T*)p->ug,
                          (T*)p->grads);
-      hipLaunchKernelGGL((k_exchange<T, 1, D>), dim3(grid_for(p->nrows, 256)), dim3(256), 0,
-                         p->stream, p->nrows, p->sh, p->rowptr, p->cb, p->cl,
-                         (const T*)p->cval_g, (const T*)p->grads, (T*)p->gg);
+      ghosts<1>(p, (const T*)p->cval_g, (const T*)p->grads, t, (T*)p->gg);
     } else {
       hipLaunchKernelGGL((k_gradient_cart<T, D, P1>), dim3(grid_for(n, 256)), dim3(256), 0,
                          p->stream, p->sh, (const Tables<T>*)p->tables, p->spacing, y,
                          (const T*)p->ug, (const T*)p->coef, p->flat, (T*)p->grads);
-      hipLaunchKernelGGL((k_exchange<T, 2, D>), dim3(grid_for(p->nrows, 256)), dim3(256), 0,
-                         p->stream, p->nrows, p->sh, p->rowptr, p->cb, p->cl,
-                         (const T*)p->cval_u, (const T*)p->grads, (T*)p->gg);
-      if (p->ndir)
-        hipLaunchKernelGGL((k_dirichlet<T, 2, D>), dim3(grid_for(p->ndir, 256)), dim3(256), 0,
-                           p->stream, p->ndir, p->sh, p->dir_rows, p->dir_x, p->wave_k[0],
-                           p->wave_k[1], p->wave_k[2], p->wave_w, p->wave_phi, t, (T*)p->gg);
+      ghosts<2>(p, (const T*)p->cval_u, (const T*)p->grads, t, (T*)p->gg);
     }
   }
   template <int MODE>
