@@ -303,13 +303,13 @@ k_fused3(FusedArgs<T> A) {
       // gradient)
       T before = prev_last, after, gbefore = 0, gafter = 0, next_part = 0;
       if (GHOST && e == 0 && nzl == CF) {
-        before = act ? A.gu[((int64_t)b * 6 + 4) * N2 + col] : T(0);
-        gbefore = act ? A.gg[((int64_t)b * 6 + 4) * N2 + col] : T(0);
+        before = act ? A.gu[((int64_t)b * 6 + 4) * N2 + opaque(col)] : T(0);
+        gbefore = act ? A.gg[((int64_t)b * 6 + 4) * N2 + opaque(col)] : T(0);
       } else if (e == 0) {
         const int nlo = nzl;
-        before = (act && nlo >= 0) ? A.uf_in[((int64_t)nlo * 6 + 5) * N2 + col] : uc[0];
+        before = (act && nlo >= 0) ? A.uf_in[((int64_t)nlo * 6 + 5) * N2 + opaque(col)] : uc[0];
         if (act && nlo >= 0) {
-          gbefore = (A.pf_in[((int64_t)nlo * 6 + 5) * N2 + col] + Whi * uc[0]) * rsp(nlo);
+          gbefore = (A.pf_in[((int64_t)nlo * 6 + 5) * N2 + opaque(col)] + Whi * uc[0]) * rsp(nlo);
         } else {
           T g0 = Wlo * before;
 #pragma unroll
@@ -323,13 +323,13 @@ k_fused3(FusedArgs<T> A) {
 #pragma unroll
         for (int j = 0; j < P1; ++j) next_part += tb.W[0][j + 1] * un2[j];
       } else if (GHOST && nzh == CF) {
-        after = act ? A.gu[((int64_t)b * 6 + 5) * N2 + col] : T(0);
-        gafter = act ? A.gg[((int64_t)b * 6 + 5) * N2 + col] : T(0);
+        after = act ? A.gu[((int64_t)b * 6 + 5) * N2 + opaque(col)] : T(0);
+        gafter = act ? A.gg[((int64_t)b * 6 + 5) * N2 + opaque(col)] : T(0);
       } else {
         const int nhi = nzh;
-        after = (act && nhi >= 0) ? A.uf_in[((int64_t)nhi * 6 + 4) * N2 + col] : uc[P];
+        after = (act && nhi >= 0) ? A.uf_in[((int64_t)nhi * 6 + 4) * N2 + opaque(col)] : uc[P];
         if (act && nhi >= 0) {
-          gafter = (A.pf_in[((int64_t)nhi * 6 + 4) * N2 + col] + Wlo * uc[P]) * rsp(nhi);
+          gafter = (A.pf_in[((int64_t)nhi * 6 + 4) * N2 + opaque(col)] + Wlo * uc[P]) * rsp(nhi);
         } else {
           T gN = Whi * after;
 #pragma unroll
@@ -556,15 +556,15 @@ k_fused3(FusedArgs<T> A) {
         T s = 0;
 #pragma unroll
         for (int i = 0; i < P1; ++i) s += tb.W[0][i + 1] * UN[i * NT + col];
-        ufo[4 * N2 + col] = UN[col];
-        pfo[4 * N2 + col] = s;
+        ufo[4 * N2 + opaque(col)] = UN[col];
+        pfo[4 * N2 + opaque(col)] = s;
       }
       if (e == E - 1) {
         T s = 0;
 #pragma unroll
         for (int i = 0; i < P1; ++i) s += tb.W[P][i + 1] * UN[i * NT + col];
-        ufo[5 * N2 + col] = UN[P * NT + col];
-        pfo[5 * N2 + col] = s;
+        ufo[5 * N2 + opaque(col)] = UN[P * NT + col];
+        pfo[5 * N2 + opaque(col)] = s;
       }
       if (MODE == 4) {
 #pragma unroll
