@@ -79,15 +79,6 @@ __device__ __forceinline__ void st_once(T* p, T v) {
     *p = v;
 }
 
-// A read of a table no kernel of the launch writes (neighbour codes, spacings)
-// at an address the whole workgroup shares: through the constant address space
-// it becomes a scalar load, waited for with the LDS counter instead of the
-// vector memory counter that also covers the stage's outstanding stores.
-template <class T>
-__device__ __forceinline__ T ld_const(const T* p) {
-  return *(const __attribute__((address_space(4))) T*)p;
-}
-
 template <class T>
 struct FusedArgs {
   const T* ys;       // stage input (u, v)
@@ -176,7 +167,7 @@ k_fused3(FusedArgs<T> A) {
     return v;
   };
 
-  const T h = (T)ld_const(A.spacing + b);
+  const T h = (T)A.spacing[b];
   const T rh = T(1) / h;
   const int32_t* nb6 = A.nbr + b * 6;
   // the block's arrays: uniform bases, per-thread 32-bit offsets (z stride N2,
@@ -190,10 +181,9 @@ k_fused3(FusedArgs<T> A) {
   T* lo = A.lo ? A.lo + b * 2 * nn : nullptr;
   T* ufo = A.uf_out + b * 6 * N2;
   T* pfo = A.pf_out + b * 6 * N2;
-  const int nxl = ld_const(nb6 + 0), nxh = ld_const(nb6 + 1), nyl = ld_const(nb6 + 2),
-            nyh = ld_const(nb6 + 3), nzl = ld_const(nb6 + 4), nzh = ld_const(nb6 + 5);
+  const int nxl = nb6[0], nxh = nb6[1], nyl = nb6[2], nyh = nb6[3], nzl = nb6[4], nzh = nb6[5];
   // the reciprocal spacing of a same-level neighbour, read where it is used
-  auto rsp = [&](int nb) { return T(1) / (T)ld_const(A.spacing + nb); };
+  auto rsp = [&](int nb) { return T(1) / (T)A.spacing[nb]; };
   constexpr int CF = -2;  // a coarse-fine face
 
   auto load_layer = [&](int e, T* dst) {

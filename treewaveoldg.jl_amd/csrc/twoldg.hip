@@ -58,7 +58,7 @@ void destroy(twoldg_problem* p) {
   if (p->stream) (void)hipStreamSynchronize(p->stream);
   for (void* b : {p->slot[0], p->slot[1], p->slot[2], p->lo, p->ys, p->acc, p->ug, p->gg,
                   p->grads, p->tables, p->coef, (void*)p->flat, (void*)p->tslot, p->tmodes,
-                  p->tweights, p->tchol, (void*)p->spacing, (void*)p->rowptr, (void*)p->cb,
+                  p->tweights, p->tginv, (void*)p->spacing, (void*)p->rowptr, (void*)p->cb,
                   (void*)p->cl, p->cval_u, p->cval_g, (void*)p->dir_rows, (void*)p->dir_x,
                   (void*)p->part, (void*)p->red, (void*)p->nbr6, p->uf[0], p->uf[1], p->pf[0],
                   p->pf[1], (void*)p->frows, (void*)p->fsrc, p->fwu,
@@ -158,7 +158,32 @@ int create_buffers(twoldg_problem* p, const twoldg_desc* d, const Csr& csr) {
     const int64_t nE = ipow(p->E, D), nq = ipow(p->P1, D), mq = ipow(p->P, D);
     if ((rc = upload_as<T>(p, &p->tmodes, d->tmodes, nq * mq))) return rc;
     if ((rc = upload_as<T>(p, &p->tweights, d->tweights, d->ntables * nE * nq))) return rc;
-    if ((rc = upload_as<T>(p, &p->tchol, d->tchol, d->ntables * nE * mq * mq))) return rc;
+    // G^-1 = U^-1 U^-T per element from the upper Cholesky factor U (column-major)
+    const int64_t nel = d->ntables * nE;
+    std::vector<double> ginv((size_t)(nel * mq * mq));
+    std::vector<double> V((size_t)(mq * mq));
+    for (int64_t el = 0; el < nel; ++el) {
+      const double* U = d->tchol + el * mq * mq;
+      // V = U^-1 (upper triangular), column by column: U V[:, j] = e_j
+      std::fill(V.begin(), V.end(), 0.0);
+      for (int64_t j = 0; j < mq; ++j) {
+        for (int64_t i = j; i >= 0; --i) {
+          double r = (i == j) ? 1.0 : 0.0;
+          for (int64_t l = i + 1; l <= j; ++l) r -= U[i + mq * l] * V[l + mq * j];
+          if (U[i + mq * i] == 0.0) return fail(TWOLDG_EINVAL, "singular truncation table");
+          V[i + mq * j] = r / U[i + mq * i];
+        }
+      }
+      double* Gi = ginv.data() + el * mq * mq;
+      for (int64_t q = 0; q < mq; ++q)
+        for (int64_t k = 0; k <= q; ++k) {
+          double g = 0;  // (V V^T)[k][q] = sum over j >= q of V[k][j] V[q][j]
+          for (int64_t j = q; j < mq; ++j) g += V[k + mq * j] * V[q + mq * j];
+          Gi[k + mq * q] = g;
+          Gi[q + mq * k] = g;
+        }
+    }
+    if ((rc = upload_as<T>(p, &p->tginv, ginv.data(), nel * mq * mq))) return rc;
   }
   if (!csr.dir_rows.empty()) {
     p->ndir = (int64_t)csr.dir_rows.size();

@@ -443,21 +443,22 @@ __device__ __forceinline__ void lds_barrier() {
 }
 
 // Per-element sqrt(g)-orthogonal projection (evolution.jl:384-425):
-// c = Psi^T (W_e a); U^T y = c; U z = y; a = Psi z.  D = 3.
-// One workgroup per element, a thread per node (and per mode): the mode sums
-// in node order, the two triangular solves column by column (a mode's value
-// once final is broadcast through LDS and subtracted by the modes it feeds),
-// then a thread per node forms the projected value.
+// c = Psi^T (W_e a); z = G^-1 c with G = Psi^T W_e Psi = U^T U (the element's
+// Gram matrix, whose Cholesky factor U the caller supplies; its inverse is
+// formed once per problem on the host, so the two triangular solves become one
+// mat-vec); a = Psi z.  D = 3.  One workgroup per element, a thread per node
+// (and per mode), three LDS barriers.
 template <class T, int P1>
 __global__ void __launch_bounds__(((P1 * P1 * P1) + 63) / 64 * 64)
 k_truncate_elem(Shape sh, int E, const int32_t* __restrict__ tslot,
                 const T* __restrict__ tmodes, const T* __restrict__ tweights,
-                const T* __restrict__ tchol, T* __restrict__ y) {
+                const T* __restrict__ tginv, T* __restrict__ y) {
   constexpr int P = P1 - 1;
   constexpr int n = P1 * P1 * P1, m = P * P * P;
   __shared__ T sWA[n];
   __shared__ T sc[m];
-  __shared__ T sU[m * m];  // the Cholesky factor, read at every step of the solves
+  __shared__ T sz[m];
+  __shared__ T sG[m * m];  // G^-1, column-major (symmetric)
   const int64_t ne = (int64_t)E * E * E;
   const int64_t b = blockIdx.x / ne, e = blockIdx.x % ne;
   const int s = tslot[b];
@@ -468,38 +469,29 @@ k_truncate_elem(Shape sh, int E, const int32_t* __restrict__ tslot,
   T* a = y + b * 2 * sh.nn + sh.nn;
   const int64_t base = (int64_t)J0 * P1 + (int64_t)N * (J1 * P1) + (int64_t)N * N * (J2 * P1);
   const T* w = tweights + ((int64_t)s * ne + e) * n;
-  const T* U = tchol + ((int64_t)s * ne + e) * m * m;
+  const T* G = tginv + ((int64_t)s * ne + e) * m * m;
   auto at = [&](int q) -> int64_t {
     const int k0 = q % P1, k1 = (q / P1) % P1, k2 = q / (P1 * P1);
     return base + k0 + (int64_t)N * k1 + (int64_t)N * N * k2;
   };
-  for (int i = k; i < m * m; i += blockDim.x) sU[i] = U[i];
+  for (int i = k; i < m * m; i += blockDim.x) sG[i] = G[i];
   if (k < n) sWA[k] = w[k] * a[at(k)];
   __syncthreads();
-  T cq = 0;
-  if (k < m)
+  if (k < m) {
+    T cq = 0;
     for (int i = 0; i < n; ++i) cq += tmodes[i * m + k] * sWA[i];
-  // U^T y = c, mode by mode (LDS only between the steps)
-  for (int q = 0; q < m; ++q) {
-    if (k == q) {
-      cq = cq / sU[q + m * q];
-      sc[q] = cq;
-    }
-    lds_barrier();
-    if (k > q && k < m) cq -= sU[q + m * k] * sc[q];
+    sc[k] = cq;
   }
-  // U z = y, from the last mode down
-  for (int q = m - 1; q >= 0; --q) {
-    if (k == q) {
-      cq = cq / sU[q + m * q];
-      sc[q] = cq;
-    }
-    lds_barrier();
-    if (k < q) cq -= sU[k + m * q] * sc[q];
+  lds_barrier();
+  if (k < m) {
+    T zq = 0;
+    for (int q = 0; q < m; ++q) zq += sG[k + m * q] * sc[q];
+    sz[k] = zq;
   }
+  lds_barrier();
   if (k < n) {
     T z = 0;
-    for (int q = 0; q < m; ++q) z += tmodes[k * m + q] * sc[q];
+    for (int q = 0; q < m; ++q) z += tmodes[k * m + q] * sz[q];
     a[at(k)] = z;
   }
 }
@@ -699,7 +691,7 @@ struct twoldg_problem {
   int32_t* tslot = nullptr;
   void* tmodes = nullptr;
   void* tweights = nullptr;
-  void* tchol = nullptr;
+  void* tginv = nullptr;  // per element: the inverse of the truncation's Gram matrix
   double* spacing = nullptr;
   int32_t* rowptr = nullptr;
   int32_t* cb = nullptr;
@@ -1054,7 +1046,7 @@ struct Launch {
       hipLaunchKernelGGL((k_truncate_elem<T, P1>), dim3((unsigned)n),
                          dim3((P1 * P1 * P1 + 63) / 64 * 64), 0, p->stream,
                          p->sh, p->E, p->tslot, (const T*)p->tmodes, (const T*)p->tweights,
-                         (const T*)p->tchol, y);
+                         (const T*)p->tginv, y);
   }
   static void energy(twoldg_problem* p, const T* y) {
     if (!p->full)
