@@ -183,7 +183,10 @@ k_fused3(FusedArgs<T> A) {
   T* pfo = A.pf_out + b * 6 * N2;
   const int nxl = nb6[0], nxh = nb6[1], nyl = nb6[2], nyh = nb6[3], nzl = nb6[4], nzh = nb6[5];
   // the reciprocal spacing of a same-level neighbour, read where it is used
-  auto rsp = [&](int nb) { return T(1) / (T)A.spacing[nb]; };
+  // the reciprocal spacings of the same-level neighbours, per face, through LDS:
+  // read there, they wait on the LDS counter, not on the stage's stores
+  __shared__ T RSP[6];
+  auto rsp = [&](int f) { return RSP[f]; };
   constexpr int CF = -2;  // a coarse-fine face
 
   auto load_layer = [&](int e, T* dst) {
@@ -279,6 +282,11 @@ k_fused3(FusedArgs<T> A) {
   T gv[GPT];  // the layer's x/y ghosts
   T prev_last = 0, prev_part = 0;  // previous layer: last u, row-P partial sum
   load_layer(0, uc);
+  if (tid < 6) {
+    const int nb = nb6[tid];
+    RSP[tid] = nb >= 0 ? T(1) / (T)A.spacing[nb] : T(0);
+  }
+  lds_barrier();
 
 #pragma unroll 1
   for (int e = 0; e < E; ++e) {
@@ -299,7 +307,7 @@ k_fused3(FusedArgs<T> A) {
         const int nlo = nzl;
         before = (act && nlo >= 0) ? A.uf_in[((int64_t)nlo * 6 + 5) * N2 + opaque(col)] : uc[0];
         if (act && nlo >= 0) {
-          gbefore = (A.pf_in[((int64_t)nlo * 6 + 5) * N2 + opaque(col)] + Whi * uc[0]) * rsp(nlo);
+          gbefore = (A.pf_in[((int64_t)nlo * 6 + 5) * N2 + opaque(col)] + Whi * uc[0]) * rsp(4);
         } else {
           T g0 = Wlo * before;
 #pragma unroll
@@ -319,7 +327,7 @@ k_fused3(FusedArgs<T> A) {
         const int nhi = nzh;
         after = (act && nhi >= 0) ? A.uf_in[((int64_t)nhi * 6 + 4) * N2 + opaque(col)] : uc[P];
         if (act && nhi >= 0) {
-          gafter = (A.pf_in[((int64_t)nhi * 6 + 4) * N2 + opaque(col)] + Wlo * uc[P]) * rsp(nhi);
+          gafter = (A.pf_in[((int64_t)nhi * 6 + 4) * N2 + opaque(col)] + Wlo * uc[P]) * rsp(5);
         } else {
           T gN = Whi * after;
 #pragma unroll
@@ -396,11 +404,11 @@ k_fused3(FusedArgs<T> A) {
             ex, [&](int c) { return Ur[c]; }, [&]() { return GH[0][g]; }, [&]() { return GH[1][g]; },
             [&](T uf, T gf) {
               if (GHOST && nxl == CF) return GH[4][g];
-              return nxl >= 0 ? (GH[4][g] + Whi * uf) * rsp(nxl) : -gf;
+              return nxl >= 0 ? (GH[4][g] + Whi * uf) * rsp(0) : -gf;
             },
             [&](T uf, T gf) {
               if (GHOST && nxh == CF) return GH[5][g];
-              return nxh >= 0 ? (GH[5][g] + Wlo * uf) * rsp(nxh) : -gf;
+              return nxh >= 0 ? (GH[5][g] + Wlo * uf) * rsp(1) : -gf;
             },
             lx);
 #pragma unroll
@@ -412,11 +420,11 @@ k_fused3(FusedArgs<T> A) {
           ey, [&](int c) { return Uc[c * N]; }, [&]() { return GH[2][g]; }, [&]() { return GH[3][g]; },
           [&](T uf, T gf) {
             if (GHOST && nyl == CF) return GH[6][g];
-            return nyl >= 0 ? (GH[6][g] + Whi * uf) * rsp(nyl) : -gf;
+            return nyl >= 0 ? (GH[6][g] + Whi * uf) * rsp(2) : -gf;
           },
           [&](T uf, T gf) {
             if (GHOST && nyh == CF) return GH[7][g];
-            return nyh >= 0 ? (GH[7][g] + Wlo * uf) * rsp(nyh) : -gf;
+            return nyh >= 0 ? (GH[7][g] + Wlo * uf) * rsp(3) : -gf;
           },
           ly);
     }
