@@ -447,12 +447,15 @@ __device__ __forceinline__ void lds_barrier() {
 // Gram matrix, whose Cholesky factor U the caller supplies; its inverse is
 // formed once per problem on the host, so the two triangular solves become one
 // mat-vec); a = Psi z.  D = 3.  One workgroup per element, a thread per node
-// (and per mode), three LDS barriers.
+// (and per mode), three LDS barriers.  The elements of blocks without tables
+// (tslot < 0) get the tensor projector here too, direction by direction with
+// k_truncate's arithmetic, so one launch covers the forest.
 template <class T, int P1>
 __global__ void __launch_bounds__(((P1 * P1 * P1) + 63) / 64 * 64)
-k_truncate_elem(Shape sh, int E, const int32_t* __restrict__ tslot,
-                const T* __restrict__ tmodes, const T* __restrict__ tweights,
-                const T* __restrict__ tginv, T* __restrict__ y) {
+k_truncate_elem(Shape sh, int E, const Tables<T>* __restrict__ tbp,
+                const int32_t* __restrict__ tslot, const T* __restrict__ tmodes,
+                const T* __restrict__ tweights, const T* __restrict__ tginv,
+                T* __restrict__ y) {
   constexpr int P = P1 - 1;
   constexpr int n = P1 * P1 * P1, m = P * P * P;
   __shared__ T sWA[n];
@@ -462,18 +465,39 @@ k_truncate_elem(Shape sh, int E, const int32_t* __restrict__ tslot,
   const int64_t ne = (int64_t)E * E * E;
   const int64_t b = blockIdx.x / ne, e = blockIdx.x % ne;
   const int s = tslot[b];
-  if (s < 0) return;  // the whole workgroup: a block with tensor truncation
   const int k = threadIdx.x;
   const int N = sh.N;
   const int J0 = (int)(e % E), J1 = (int)((e / E) % E), J2 = (int)(e / ((int64_t)E * E));
   T* a = y + b * 2 * sh.nn + sh.nn;
   const int64_t base = (int64_t)J0 * P1 + (int64_t)N * (J1 * P1) + (int64_t)N * N * (J2 * P1);
-  const T* w = tweights + ((int64_t)s * ne + e) * n;
-  const T* G = tginv + ((int64_t)s * ne + e) * m * m;
   auto at = [&](int q) -> int64_t {
     const int k0 = q % P1, k1 = (q / P1) % P1, k2 = q / (P1 * P1);
     return base + k0 + (int64_t)N * k1 + (int64_t)N * N * k2;
   };
+  if (s < 0) {  // the tensor projector (evolution.jl:238-264), x, y, then z
+    const Tables<T>& tb = *tbp;
+    if (k < n) sWA[k] = a[at(k)];
+    T v = 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      lds_barrier();
+      if (k < n) {
+        const int st = d == 0 ? 1 : (d == 1 ? P1 : P1 * P1);
+        const int i = (k / st) % P1;
+        const int l0 = k - i * st;
+        T c = 0;
+#pragma unroll
+        for (int j = 0; j < P1; ++j) c += tb.dual[j] * sWA[l0 + j * st];
+        v = sWA[k] - tb.mode[i] * c;
+      }
+      lds_barrier();
+      if (k < n) sWA[k] = v;
+    }
+    if (k < n) a[at(k)] = v;
+    return;
+  }
+  const T* w = tweights + ((int64_t)s * ne + e) * n;
+  const T* G = tginv + ((int64_t)s * ne + e) * m * m;
   for (int i = k; i < m * m; i += blockDim.x) sG[i] = G[i];
   if (k < n) sWA[k] = w[k] * a[at(k)];
   __syncthreads();
@@ -1036,17 +1060,17 @@ struct Launch {
     if (!p->truncate) return;
     const int64_t ne = ipow(p->E, D);
     const int64_t n = p->sh.nb * ne;
+    if constexpr (D == 3 && P1 <= 5) {
+      if (p->has_tables) {  // every block: per-element tables or the tensor projector
+        hipLaunchKernelGGL((k_truncate_elem<T, P1>), dim3((unsigned)n),
+                           dim3((P1 * P1 * P1 + 63) / 64 * 64), 0, p->stream, p->sh, p->E,
+                           (const Tables<T>*)p->tables, p->tslot, (const T*)p->tmodes,
+                           (const T*)p->tweights, (const T*)p->tginv, y);
+        return;
+      }
+    }
     hipLaunchKernelGGL((k_truncate<T, D, P1>), dim3((unsigned)p->sh.nb), dim3(kTruncThreads), 0,
                        p->stream, p->sh, p->E, (const Tables<T>*)p->tables, p->tslot, y);
-    if (D == 3 && p->has_tables)
-      launch_elem(p, y, n);
-  }
-  static void launch_elem(twoldg_problem* p, T* y, int64_t n) {
-    if constexpr (D == 3 && P1 <= 5)
-      hipLaunchKernelGGL((k_truncate_elem<T, P1>), dim3((unsigned)n),
-                         dim3((P1 * P1 * P1 + 63) / 64 * 64), 0, p->stream,
-                         p->sh, p->E, p->tslot, (const T*)p->tmodes, (const T*)p->tweights,
-                         (const T*)p->tginv, y);
   }
   static void energy(twoldg_problem* p, const T* y) {
     if (!p->full)
