@@ -21,8 +21,8 @@
 
 // Waves per SIMD the fused stage is compiled for: four (128 VGPRs) lets two
 // workgroups of the 20^3-node blocks share a CU, which hides the latency of
-// one's barriers and loads behind the other's work.  FP64 then keeps the stage
-// operands one plane at a time (NB 0).
+// one's barriers and loads behind the other's work.  Both types load the stage
+// operands of a y line one node ahead (NB 1 for FP64).
 #ifndef TWOLDG_FUSED_WAVES_F64
 #define TWOLDG_FUSED_WAVES_F64 4
 #endif
@@ -30,7 +30,7 @@
 #define TWOLDG_FUSED_WAVES_F32 4
 #endif
 #ifndef TWOLDG_FUSED_NB_F64
-#define TWOLDG_FUSED_NB_F64 0
+#define TWOLDG_FUSED_NB_F64 1
 #endif
 // Waves per SIMD that the workgroups the LDS admits on one CU can bring (the
 // most loaded SIMD), capped by the above.
