@@ -522,7 +522,15 @@ k_fused3(FusedArgs<T> A) {
     lds_barrier();
     // ---- (4) x lines: v's truncation along x, faces of the next u along x;
     // columns: faces of the next u along z ----
-    T lov[P1], hv[P1];  // the compensated form's v operands
+    T lov[P1], hv[P1];  // the compensated form's v operands, ahead of this phase's stores
+    if (act && MODE == 4) {
+#pragma unroll
+      for (int j = 0; j < P1; ++j) {
+        const int iv = col + (e * P1 + j) * N2 + nn;
+        lov[j] = ld_once(lo + iv);
+        hv[j] = ld_once(un_ + iv);
+      }
+    }
     if (act) {
       const int t4 = opaque(tid);
       const int ex = t4 % E, yx = (t4 / E) % N, px = t4 / (E * N);
@@ -563,14 +571,6 @@ k_fused3(FusedArgs<T> A) {
         for (int i = 0; i < P1; ++i) s += tb.W[P][i + 1] * UN[i * NT + col];
         ufo[5 * N2 + opaque(col)] = UN[P * NT + col];
         pfo[5 * N2 + opaque(col)] = s;
-      }
-      if (MODE == 4) {
-#pragma unroll
-        for (int j = 0; j < P1; ++j) {
-          const int iv = col + (e * P1 + j) * N2 + nn;
-          lov[j] = ld_once(lo + iv);
-          hv[j] = ld_once(un_ + iv);
-        }
       }
     }
     lds_barrier();
