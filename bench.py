@@ -95,7 +95,7 @@ def gpu_run(args, precision):
 
 
 def accuracy(T, prob, nsteps, dt):
-    """L2 error against the exact standing mode after nsteps steps (config 2's
+    """L2 error against the exact standing mode after nsteps steps (the 320^3 workload's
     200), and the discrete energy drift."""
     u0 = T.initial_state(prob, T.wave_exact(3, 1.0, 1, 0.0))
     e0 = T.wave_energy(prob, u0)
@@ -170,21 +170,21 @@ def timed_integrator(T, prob, u0, dt, steps, warmup=2):
     return wall, None, None, integ
 
 
-def config1():
+def periodic2d():
     """wave_errors(Val(2); P=4, E=2, roots=8) end to end on the GPU (parity run)."""
     import twoldg as T
     t0 = time.perf_counter()
     r = T.wave_errors(2, 4, 2, roots=8)
     el = time.perf_counter() - t0
     nodes = r["nblocks"] * 100
-    log(f"[bench] config 1: l2 {r['l2']:.4e} (u {r['l2u']:.3e}, v {r['l2v']:.3e}), "
+    log(f"[bench] 2D periodic 80^2: l2 {r['l2']:.4e} (u {r['l2u']:.3e}, v {r['l2v']:.3e}), "
         f"E1/E0 {r['energy'] / r['energy0']:.12f}, {r['nsteps']} steps, {el:.2f} s wall")
     return dict(l2=r["l2"], l2u=r["l2u"], l2v=r["l2v"], linf=r["linf"],
                 energy_ratio=r["energy"] / r["energy0"], nsteps=r["nsteps"], wall_s=el,
                 nodes=nodes, note="6400 nodes: launch-bound, a parity run")
 
 
-def config3(large=False, steps=500):
+def nested_cubes(large=False, steps=500):
     """Reflecting box with nested cubes (3 levels, finest half-width 1/8): P=3,
     E=2, roots=8 (1408 blocks), 500 steps at the finest level's CFL (0.5); or
     the large refined forest P=4, E=4, roots=16 (11264 blocks of 20^3) for the
@@ -213,20 +213,20 @@ def config3(large=False, steps=500):
         out[prec] = dict(value=nodes * timed / wall, ms_per_step=wall * 1e3 / timed,
                          alg_gbs_whole_step=alg / wall / 1e9, energy_drift=(e1 - e0) / e0,
                          steps=steps)
-        log(f"[bench] config 3{'-large' if large else ''} {prec}: {forest.nblocks} blocks, "
+        log(f"[bench] nested cubes{' (large)' if large else ''} {prec}: {forest.nblocks} blocks, "
             f"{steps} steps, {nodes * timed / wall:.3e} node-updates/s, "
             f"{alg / wall / 1e9:.0f} GB/s algorithmic, energy drift {(e1 - e0) / e0:.2e}")
         prob.close()
     rel = float(np.linalg.norm(final["f32c"] - final["f64"]) / np.linalg.norm(final["f64"]))
     out["f32c_vs_f64_rel_l2"] = rel
-    log(f"[bench] config 3{'-large' if large else ''}: FP32c vs FP64 after {steps} steps, "
+    log(f"[bench] nested cubes{' (large)' if large else ''}: FP32c vs FP64 after {steps} steps, "
         f"relative L2 {rel:.2e}")
     out["nodes"] = nodes
     out["blocks"] = forest.nblocks
     return out
 
 
-def config4(steps=20):
+def batch2d(steps=20):
     """512 independent 2D periodic problems, P in {2..6}, roots in {4, 8, 16},
     E=2, seeded Gaussian pulses, one launch per stage, graph-replayed steps."""
     import twoldg as T
@@ -248,7 +248,7 @@ def config4(steps=20):
         out[prec] = dict(value=nodes * steps / (ms * 1e-3), ms_per_step=ms / steps,
                          alg_gbs=alg / (ms * 1e-3) / 1e9, tiles=info["tiles"],
                          workgroups_per_launch=info["workgroups"], setup_s=setup)
-        log(f"[bench] config 4 {prec}: {len(probs)} problems, {nodes} nodes, "
+        log(f"[bench] 2D batch {prec}: {len(probs)} problems, {nodes} nodes, "
             f"{nodes * steps / (ms * 1e-3):.3e} node-updates/s ({ms / steps:.3f} ms/step, "
             f"{alg / (ms * 1e-3) / 1e9:.0f} GB/s), tiles {info['tiles']}")
         batch.close()
@@ -258,7 +258,7 @@ def config4(steps=20):
     return out
 
 
-def config5(steps=20):
+def ball(steps=20):
     """Seven-patch ball, P=3, E=2, roots=(4,4) (448 blocks), plane wave
     k = 2 pi (1,1,1)/sqrt(3), phi = 0, Dirichlet sphere; FP64 and FP32c; l2 at
     t = 1 for both forms."""
@@ -274,7 +274,7 @@ def config5(steps=20):
         out[prec] = dict(l2=r["l2"], l2u=r["l2u"], l2v=r["l2v"], linf=r["linf"],
                          energy_ratio=r["energy"] / r["energy0"], nsteps=r["nsteps"],
                          value=nodes * steps / wall, ms_per_step=wall * 1e3 / steps)
-        log(f"[bench] config 5 {prec}: {prob.nblocks} blocks, l2 {r['l2']:.4e} at t=1 "
+        log(f"[bench] seven-patch ball {prec}: {prob.nblocks} blocks, l2 {r['l2']:.4e} at t=1 "
             f"({r['nsteps']} steps), E1/E0 {r['energy'] / r['energy0']:.6f}, "
             f"{nodes * steps / wall:.3e} node-updates/s")
         prob.close()
@@ -284,11 +284,11 @@ def config5(steps=20):
 
 # The side workloads by name, each at the size the default run reports.
 WORKLOADS = {
-    "config1": lambda: config1(),
-    "config3": lambda: config3(),
-    "config3-large": lambda: config3(large=True, steps=10),
-    "config4": lambda: config4(),
-    "config5": lambda: config5(),
+    "periodic2d": lambda: periodic2d(),
+    "nested-cubes": lambda: nested_cubes(),
+    "nested-cubes-large": lambda: nested_cubes(large=True, steps=10),
+    "batch2d": lambda: batch2d(),
+    "ball": lambda: ball(),
 }
 
 
@@ -302,7 +302,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--single", action="store_true", help="time only --precision's form")
     ap.add_argument("--no-configs", action="store_true",
-                    help="skip configs 1, 3, 4 and 5 (reported beside the main line)")
+                    help="skip the side workloads (reported beside the main line)")
     ap.add_argument("--workload", choices=sorted(WORKLOADS),
                     help="run only this side workload and print its result (for profiling)")
     args = ap.parse_args()
@@ -318,11 +318,8 @@ def main():
     other_res = {} if args.single else gpu_run(args, other)
     configs = {}
     if not args.no_configs:
-        configs["config1"] = config1()
-        configs["config3"] = config3()
-        configs["config3_large"] = config3(large=True, steps=10)
-        configs["config4"] = config4()
-        configs["config5"] = config5()
+        for name, run in WORKLOADS.items():
+            configs[name] = run()
     cpu = None if args.no_cpu else cpu_baseline()
     prec = args.precision
     launch_ms = main_res["kernel_ms"] / max(main_res["launches"], 1)

@@ -33,7 +33,7 @@ def main():
     res = {}
     keys = ("l2", "l2u", "l2v", "linf", "energy0", "energy", "nsteps")
     r = O.wave_errors(2, 4, 2, roots=8)
-    res["config1"] = {k: r[k] for k in keys}
+    res["wave_d2_p4_r8"] = {k: r[k] for k in keys}
     r = O.wave_errors(2, 2, 2, roots=4, reflecting=True)
     res["reflect_d2_p2_r4"] = {k: r[k] for k in keys}
     r = O.wave_errors(2, 3, 2, roots=4, regions=[((0.25, 0.75),) * 2])

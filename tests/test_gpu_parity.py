@@ -29,10 +29,10 @@ def _pair(D, P, E, roots, reflecting=False, regions=(), truncate=True, precision
 
 CASES = [
     dict(D=1, P=3, E=2, roots=4),
-    dict(D=2, P=4, E=2, roots=8),                       # config 1's forest
+    dict(D=2, P=4, E=2, roots=8),                       # the 80^2 periodic forest of the published table
     dict(D=2, P=2, E=2, roots=4, reflecting=True),
     dict(D=3, P=3, E=2, roots=2),
-    dict(D=3, P=4, E=4, roots=2),                       # config 2's block shape
+    dict(D=3, P=4, E=4, roots=2),                       # the 20^3 blocks of the 320^3 workload
     dict(D=2, P=3, E=2, roots=4, regions=[((0.25, 0.75),) * 2]),
     dict(D=3, P=3, E=2, roots=8, reflecting=True, regions=O.nested_cubes(3, 1 / 8, 3)),
 ]
@@ -92,7 +92,7 @@ def test_rhs_is_pure(rough):
     assert np.array_equal(a, b)
 
 
-def test_config1_wave_errors_match_oracle():
+def test_wave_errors_2d_p4_matches_oracle():
     """wave_errors(Val(2); P=4, E=2, roots=8): the published 4.54e-5 (u 6.61e-6,
     v 6.38e-5, CODE.md:4142) and the oracle's run to sqrt(eps)."""
     r = T.wave_errors(2, 4, 2, roots=8)
@@ -130,7 +130,7 @@ def test_rk4_matches_oracle(case):
 
 
 def test_f32c_close_to_f64():
-    """The FP32-compensated form on config 1: FP32 rounding only (the reference's
+    """The FP32-compensated form on the 80^2 periodic forest (P=4, E=2, roots=8): FP32 rounding only (the reference's
     FP32 test holds FP32 to rtol 1e-3 of FP64 at roots=2, sinewave_tests.jl:49-59)."""
     r64 = T.wave_errors(2, 4, 2, roots=8)
     r32 = T.wave_errors(2, 4, 2, roots=8, precision="f32c")
@@ -141,7 +141,7 @@ def test_f32c_close_to_f64():
 FUSED = [
     dict(D=3, P=3, E=2, roots=2),
     dict(D=3, P=2, E=2, roots=4, reflecting=True),
-    dict(D=3, P=4, E=4, roots=2),                       # config 2's 20^3 blocks
+    dict(D=3, P=4, E=4, roots=2),                       # the 320^3 workload's 20^3 blocks
     dict(D=3, P=4, E=2, roots=3),
     dict(D=3, P=1, E=4, roots=2, reflecting=True),
     dict(D=3, P=8, E=2, roots=2),                       # the largest order, 18^3 blocks
@@ -339,7 +339,7 @@ def test_sevenpatch_errors_published_and_oracle():
 
 
 def test_sevenpatch_f32c():
-    """FP32 stencil with FP64-built coefficients stored as FP32 (config 5's form)."""
+    """FP32 stencil with FP64-built coefficients stored as FP32 (the seven-patch ball's FP32 form)."""
     r64 = T.sevenpatch_errors(3, 2, roots=(2, 2))
     r32 = T.sevenpatch_errors(3, 2, roots=(2, 2), precision="f32c")
     assert r32["l2"] == pytest.approx(r64["l2"], rel=1e-3)
@@ -395,7 +395,7 @@ def test_golden_fixtures():
     tr = T.truncate_state(f["y"].copy(), pb)
     assert np.max(np.abs(tr - f["ytrunc"])) <= 1e-12 * np.max(np.abs(f["ytrunc"]))
     res = json.load(open(os.path.join(G, "results.json")))
-    runs = {"config1": T.wave_errors(2, 4, 2, roots=8),
+    runs = {"wave_d2_p4_r8": T.wave_errors(2, 4, 2, roots=8),
             "reflect_d2_p2_r4": T.reflect_errors(2, 2, 2, roots=4),
             "twolevel_d2_p3_r4": T.wave_errors(2, 3, 2, roots=4, refined=True),
             "sevenpatch_p3_s1": T.sevenpatch_errors(3, 2, roots=(2, 2))}

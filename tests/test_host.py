@@ -171,5 +171,5 @@ def test_oracle_reproduces_golden_fixtures():
     assert np.array_equal(pb.truncate(f["y"]), f["ytrunc"])
     res = json.load(open(os.path.join(GOLDEN, "results.json")))
     r = O.wave_errors(2, 4, 2, roots=8)
-    for k, v in res["config1"].items():
+    for k, v in res["wave_d2_p4_r8"].items():
         assert r[k] == pytest.approx(v, rel=1e-14)

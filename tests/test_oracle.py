@@ -247,7 +247,7 @@ def test_energy_rate_across_ball_glues():
 
 
 # --- published tables ------------------------------------------------------------------
-def test_config1_published_table():
+def test_wave_errors_2d_p4_published_table():
     """CODE.md:4142 (D=2, P=4, truncate): roots 2/4/8 u 1.46e-3 1.02e-4 6.61e-6,
     v 1.20e-2 9.31e-4 6.38e-5; CODE.md:4143 (no truncation) roots 8: 1.05e-7, 2.69e-5."""
     for roots, u, v in [(2, 1.46e-3, 1.20e-2), (4, 1.02e-4, 9.31e-4), (8, 6.61e-6, 6.38e-5)]:
