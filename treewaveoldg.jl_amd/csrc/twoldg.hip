@@ -269,7 +269,9 @@ int create_buffers(twoldg_problem* p, const twoldg_desc* d, const Csr& csr) {
     if ((rc = h2d(p, p->frows, rows.data(), rows.size() * 4))) return rc;
     if ((rc = h2d(p, p->fsrc, src.data(), src.size() * 4))) return rc;
     if ((rc = upload_as<T>(p, &p->fwu, wu.data(), wu.size()))) return rc;
-    if ((rc = upload_as<T>(p, &p->fwg, wg.data(), wg.size()))) return rc;
+    // coarse-fine rows weigh u and the gradient alike: then both ghost launches
+    // read one array (fwg stays null)
+    if (wg != wu && (rc = upload_as<T>(p, &p->fwg, wg.data(), wg.size()))) return rc;
     const size_t nf = (size_t)p->sh.nb * 6 * N2;
     if ((rc = dalloc_bytes(p, &p->fgu, nf * sizeof(T)))) return rc;
     if ((rc = dalloc_bytes(p, &p->fgg, nf * sizeof(T)))) return rc;

@@ -1131,7 +1131,8 @@ int fused_steps(twoldg_problem* p, double dt, int64_t nsteps) {
                          (const T*)p->fwu, (const T*)p->uf[in], (const T*)p->pf[in], p->spacing,
                          (T*)p->fgu, (T*)p->ffg);
       hipLaunchKernelGGL((k_face_ghost_g<T>), dim3(grid_for(nfr, 256)), dim3(256), 0, p->stream,
-                         nfr, p->fgroups, p->frows, p->fsrc, (const T*)p->fwg,
+                         nfr, p->fgroups, p->frows, p->fsrc,
+                         (const T*)(p->fwg ? p->fwg : p->fwu),
                          (const T*)p->ffg, (T*)p->fgg);
     }
   };
