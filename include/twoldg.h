@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define TWOLDG_ABI_VERSION 1
+#define TWOLDG_ABI_VERSION 2
 
 /* status codes */
 #define TWOLDG_OK 0
@@ -49,10 +49,16 @@ extern "C" {
 #define TWOLDG_FACE_GLUE 5      /* glued patch face: nb[0], its face nface, tangential map */
 #define TWOLDG_FACE_DIRICHLET 6 /* outer face with Dirichlet data */
 
-/* slots of a problem's device state buffers */
-#define TWOLDG_SLOT_STATE 0  /* the integrator's state y_n */
+/* slots of a problem's device state buffers.  Only twoldg_rk4 advances STATE;
+   the pure entry points (rhs, truncate, energy, norms) read and write the slots
+   they are given, so a caller that keeps an integrator's state in STATE passes
+   its own vectors through ARG (and DU, AUX) without disturbing it.  Slots other
+   than STATE are allocated on their first use. */
+#define TWOLDG_SLOT_STATE 0  /* the integrator's state y_n (FP32C: with its compensation) */
 #define TWOLDG_SLOT_DU 1     /* right-hand side output */
 #define TWOLDG_SLOT_AUX 2    /* caller data, e.g. an exact solution for error norms */
+#define TWOLDG_SLOT_ARG 3    /* the argument of a pure entry point */
+#define TWOLDG_NSLOTS 4
 
 typedef struct twoldg_face {
   int32_t kind;
@@ -112,12 +118,15 @@ int twoldg_problem_info(const twoldg_problem* p, int64_t* state_len, int64_t* de
 /* --- state transfer (statevector / gather!; layout above) --------------- */
 int twoldg_state_upload(twoldg_problem* p, int slot, const double* host);
 int twoldg_state_download(twoldg_problem* p, int slot, double* host);
-/* copy one slot into another on the device */
+/* copy one slot into another on the device.  FP32C: a copy out of STATE holds
+   its compensated value rounded to FP32 (hi - lo), a copy into STATE starts a
+   zero compensation. */
 int twoldg_state_copy(twoldg_problem* p, int dst, int src);
 
 /* --- the hot path -------------------------------------------------------- */
-/* du <- f(u, t): wave_rhs!, src/evolution.jl:1367-1371 (pure: never writes u) */
-int twoldg_rhs(twoldg_problem* p, double t);
+/* slot dst <- f(slot src, t): wave_rhs!(du, u, p, t), src/evolution.jl:1367-1371
+   (pure: never writes src, keeps nothing between calls; dst != src) */
+int twoldg_rhs(twoldg_problem* p, int src, int dst, double t);
 /* v <- (Pi x ... x Pi) v in place: truncate_state!, src/evolution.jl:1383-1400 */
 int twoldg_truncate(twoldg_problem* p, int slot);
 /* nsteps classic RK4 steps of the state slot, truncation as stage and step
@@ -182,7 +191,10 @@ int twoldg_batch_download(twoldg_batch* b, int64_t i, double* host);
 /* nsteps RK4 steps of every problem (graph replays); out_ms: HIP-event time */
 int twoldg_batch_rk4(twoldg_batch* b, int64_t nsteps, double* out_ms);
 /* nodes per field over all problems, workgroups per stage launch, and per order
-   P = 1..8 the chosen blocks per workgroup (0: order absent) and its timed stage ms */
+   P = 1..8 the chosen blocks per workgroup (tiles[8]; 0: order absent) and the
+   autotune's timings (tune_ms[16]): [P-1] the stage launch of that order's
+   blocks alone at the first pass's tile, [8+P-1] the full stage launch with the
+   chosen tile (second pass), both in ms per launch */
 int twoldg_batch_info(const twoldg_batch* b, int64_t* nodes, int64_t* workgroups,
                       int32_t* tiles, double* tune_ms);
 

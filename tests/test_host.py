@@ -173,3 +173,56 @@ def test_oracle_reproduces_golden_fixtures():
     r = O.wave_errors(2, 4, 2, roots=8)
     for k, v in res["wave_d2_p4_r8"].items():
         assert r[k] == pytest.approx(v, rel=1e-14)
+
+
+def _batch_desc(forest, precision=_lib.F64, keep=None):
+    sp = forest.spacings()
+    faces = forest.faces()
+    d = _lib.Desc()
+    d.abi_version = _lib.ABI_VERSION
+    d.D, d.P, d.E = forest.D, forest.P, forest.E
+    d.precision = precision
+    d.truncate = 1
+    d.nblocks = forest.nblocks
+    d.spacing = sp.ctypes.data_as(ctypes.c_void_p)
+    d.faces = ctypes.cast(faces, ctypes.c_void_p)
+    if keep is not None:
+        keep += [sp, faces, d]
+    return d
+
+
+def test_batch_create_refusals():
+    """twoldg_batch_create checks every description before it reads any (no GPU
+    call happens on a refusal): a missing one, an unknown precision, no blocks."""
+    keep = []
+    f = T.wave_forest(2, 2, 2, roots=2)
+    good = _batch_desc(f, keep=keep)
+    bad_prec = _batch_desc(f, precision=7, keep=keep)
+    no_blocks = _batch_desc(f, keep=keep)
+    no_blocks.nblocks = 0
+    dt = np.array([0.01, 0.01])
+    out = ctypes.c_void_p()
+    L = _lib.lib()
+    for second, msg in [(None, "problem 1: no description"),
+                        (bad_prec, "problem 1: unknown precision 7"),
+                        (no_blocks, "problem 1: a problem needs at least one block")]:
+        arr = (ctypes.c_void_p * 2)()
+        arr[0] = ctypes.cast(ctypes.pointer(good), ctypes.c_void_p)
+        arr[1] = ctypes.cast(ctypes.pointer(second), ctypes.c_void_p) if second else None
+        rc = L.twoldg_batch_create(arr, 2, dt.ctypes.data_as(ctypes.c_void_p), ctypes.byref(out))
+        assert rc == -1
+        assert L.twoldg_last_error().decode().startswith(msg), L.twoldg_last_error()
+    arr = (ctypes.c_void_p * 1)()
+    arr[0] = None
+    assert L.twoldg_batch_create(arr, 1, dt.ctypes.data_as(ctypes.c_void_p),
+                                 ctypes.byref(out)) == -1
+
+
+def test_convergence_rate_closed_form():
+    """errs = C h^p gives the rate p (evolution.jl:1534-1540); a perturbed
+    sequence gives the least-squares slope."""
+    hs = [1 / 4, 1 / 8, 1 / 16, 1 / 32]
+    assert T.convergence_rate(hs, [3.0 * h ** 4 for h in hs]) == pytest.approx(4, abs=1e-12)
+    errs = [2.0 * h ** 3 * (1.1 if i % 2 else 1.0) for i, h in enumerate(hs)]
+    x, y = np.log(hs), np.log(errs)
+    assert T.convergence_rate(hs, errs) == pytest.approx(np.polyfit(x, y, 1)[0], rel=1e-12)

@@ -12,7 +12,8 @@ struct twoldg_batch {
   std::vector<int64_t> prob_off, prob_len;  // state slice of each problem
   std::vector<int32_t> P1s_host;
   int tiles[10] = {0};                      // blocks per workgroup by P1
-  double tune_ms[10] = {0};
+  double tune_ms[10] = {0};                 // first autotune pass: the order's launch alone
+  double tune_full_ms[10] = {0};            // second pass: the full launch with the chosen tile
   // device
   int64_t *off = nullptr, *foff = nullptr;
   int32_t *nbr = nullptr, *P1s = nullptr, *Ns = nullptr;
@@ -222,6 +223,7 @@ int autotune(twoldg_batch* B, const std::vector<int32_t>& Nh) {
       }
     }
     B->tiles[P1] = besttile;
+    B->tune_full_ms[P1] = best / kReps;
   }
   (void)hipFree(scratch);
   return 0;
@@ -344,31 +346,41 @@ int twoldg_batch_create(const twoldg_desc* const* descs, int64_t n, const double
   if (!out) return fail(TWOLDG_EINVAL, "no output handle");
   *out = nullptr;
   if (!descs || n < 1 || !dt) return fail(TWOLDG_EINVAL, "a batch needs problems and time steps");
-  const int prec = descs[0]->precision;
-  const int trunc = descs[0]->truncate;
   for (int64_t i = 0; i < n; ++i) {
     const twoldg_desc* d = descs[i];
-    if (!d || d->abi_version != TWOLDG_ABI_VERSION)
-      return fail(TWOLDG_EINVAL, "problem " + std::to_string(i) + ": bad description");
+    const std::string who = "problem " + std::to_string(i) + ": ";
+    if (!d) return fail(TWOLDG_EINVAL, who + "no description");
+    if (d->abi_version != TWOLDG_ABI_VERSION)
+      return fail(TWOLDG_EINVAL, who + "ABI version " + std::to_string(d->abi_version) +
+                                     " does not match the library's " +
+                                     std::to_string(TWOLDG_ABI_VERSION));
     if (d->D != 2 || d->coef)
-      return fail(TWOLDG_EINVAL, "a batch takes flat two-dimensional problems");
+      return fail(TWOLDG_EINVAL, who + "a batch takes flat two-dimensional problems");
+    if (d->precision != TWOLDG_F64 && d->precision != TWOLDG_F32C)
+      return fail(TWOLDG_EINVAL, who + "unknown precision " + std::to_string(d->precision));
     if (d->P < 1 || d->P > 7 || d->E < 2 || d->E % 2)
-      return fail(TWOLDG_EINVAL, "problem " + std::to_string(i) + ": P must be 1..7 and E even");
+      return fail(TWOLDG_EINVAL, who + "P must be 1..7 and E even");
+    if (d->nblocks < 1 || !d->spacing || !d->faces)
+      return fail(TWOLDG_EINVAL, who + "a problem needs at least one block, its spacings and faces");
     const int N = d->E * (d->P + 1);
     if (N * N > kBatchThreads || max_tile(N) < 1)
-      return fail(TWOLDG_EINVAL, "problem " + std::to_string(i) + ": blocks of " +
-                                     std::to_string(N) + "^2 nodes exceed a workgroup");
-    if (d->precision != prec || d->truncate != trunc)
+      return fail(TWOLDG_EINVAL, who + "blocks of " + std::to_string(N) +
+                                     "^2 nodes exceed a workgroup");
+    if (d->precision != descs[0]->precision || d->truncate != descs[0]->truncate)
       return fail(TWOLDG_EINVAL, "every problem of a batch shares its precision and truncation");
-    if (!(dt[i] > 0)) return fail(TWOLDG_EINVAL, "time steps must be positive");
+    if (!(dt[i] > 0)) return fail(TWOLDG_EINVAL, who + "time steps must be positive");
+    for (int64_t b = 0; b < d->nblocks; ++b)
+      if (!(d->spacing[b] > 0)) return fail(TWOLDG_EINVAL, who + "block spacing must be positive");
     for (int64_t f = 0; f < d->nblocks * 4; ++f) {
       const int k = d->faces[f].kind;
       if (k != TWOLDG_FACE_SAME && k != TWOLDG_FACE_MIRROR)
-        return fail(TWOLDG_EINVAL, "a batch takes uniform problems (same-level or mirror faces)");
+        return fail(TWOLDG_EINVAL, who + "a batch takes uniform problems (same-level or mirror faces)");
       if (k == TWOLDG_FACE_SAME && (d->faces[f].nb[0] < 0 || d->faces[f].nb[0] >= d->nblocks))
-        return fail(TWOLDG_EINVAL, "face neighbour out of range");
+        return fail(TWOLDG_EINVAL, who + "face neighbour out of range");
     }
   }
+  const int prec = descs[0]->precision;
+  const int trunc = descs[0]->truncate;
   auto* B = new twoldg_batch();
   B->precision = prec;
   B->truncate = trunc != 0;
@@ -454,7 +466,10 @@ int twoldg_batch_info(const twoldg_batch* B, int64_t* nodes, int64_t* workgroups
   if (workgroups) *workgroups = B->nitems;
   for (int P = 1; P <= 8; ++P) {
     if (tiles) tiles[P - 1] = B->tiles[P + 1];
-    if (tune_ms) tune_ms[P - 1] = B->tune_ms[P + 1];
+    if (tune_ms) {
+      tune_ms[P - 1] = B->tune_ms[P + 1];
+      tune_ms[8 + P - 1] = B->tune_full_ms[P + 1];
+    }
   }
   return 0;
 }

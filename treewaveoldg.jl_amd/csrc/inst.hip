@@ -6,7 +6,9 @@
 #define TW_CAT(a, b) TW_CAT2(a, b)
 #define TW_FN(name) TW_CAT(TW_CAT(name, _), TW_CAT(TW_TAG, TW_D))
 
-int TW_FN(twoldg_i_rhs)(twoldg_problem* p, double t) { return dispatch_p<RhsF, TW_T, TW_D>(p, t); }
+int TW_FN(twoldg_i_rhs)(twoldg_problem* p, int src, int dst, double t) {
+  return dispatch_p<RhsF, TW_T, TW_D>(p, src, dst, t);
+}
 int TW_FN(twoldg_i_rk4)(twoldg_problem* p, double t0, double dt, int64_t n) {
   return dispatch_p<Rk4F, TW_T, TW_D>(p, t0, dt, n);
 }

@@ -13,12 +13,12 @@ int fail(int code, const std::string& msg) {
 }  // namespace twoldg_detail
 using twoldg_detail::g_last_error;
 
-int twoldg_i_rhs_f64_1(twoldg_problem*, double);
-int twoldg_i_rhs_f64_2(twoldg_problem*, double);
-int twoldg_i_rhs_f64_3(twoldg_problem*, double);
-int twoldg_i_rhs_f32_1(twoldg_problem*, double);
-int twoldg_i_rhs_f32_2(twoldg_problem*, double);
-int twoldg_i_rhs_f32_3(twoldg_problem*, double);
+int twoldg_i_rhs_f64_1(twoldg_problem*, int, int, double);
+int twoldg_i_rhs_f64_2(twoldg_problem*, int, int, double);
+int twoldg_i_rhs_f64_3(twoldg_problem*, int, int, double);
+int twoldg_i_rhs_f32_1(twoldg_problem*, int, int, double);
+int twoldg_i_rhs_f32_2(twoldg_problem*, int, int, double);
+int twoldg_i_rhs_f32_3(twoldg_problem*, int, int, double);
 int twoldg_i_rk4_f64_1(twoldg_problem*, double, double, int64_t);
 int twoldg_i_rk4_f64_2(twoldg_problem*, double, double, int64_t);
 int twoldg_i_rk4_f64_3(twoldg_problem*, double, double, int64_t);
@@ -56,7 +56,8 @@ namespace {
 void destroy(twoldg_problem* p) {
   if (!p) return;
   if (p->stream) (void)hipStreamSynchronize(p->stream);
-  for (void* b : {p->slot[0], p->slot[1], p->slot[2], p->lo, p->ys, p->acc, p->ug, p->gg,
+  if (p->pinned) (void)hipHostFree(p->pinned);
+  for (void* b : {p->slot[0], p->slot[1], p->slot[2], p->slot[3], (void*)p->dstage, p->lo, p->ys, p->acc, p->ug, p->gg,
                   p->grads, p->tables, p->coef, (void*)p->flat, (void*)p->tslot, p->tmodes,
                   p->tweights, p->tginv, (void*)p->spacing, (void*)p->rowptr, (void*)p->cb,
                   (void*)p->cl, p->cval_u, p->cval_g, (void*)p->dir_rows, (void*)p->dir_x,
@@ -114,8 +115,7 @@ template <class T>
 int create_buffers(twoldg_problem* p, const twoldg_desc* d, const Csr& csr) {
   const int64_t n = p->state_len;
   int rc;
-  for (int s = 0; s < 3; ++s)
-    if ((rc = dalloc_bytes(p, &p->slot[s], n * sizeof(T)))) return rc;
+  if ((rc = dalloc_bytes(p, &p->slot[TWOLDG_SLOT_STATE], n * sizeof(T)))) return rc;
   if ((rc = dalloc_bytes(p, &p->ys, n * sizeof(T)))) return rc;
   if ((rc = dalloc_bytes(p, &p->acc, n * sizeof(T)))) return rc;
   if (p->precision == TWOLDG_F32C) {
@@ -301,6 +301,23 @@ int create_buffers(twoldg_problem* p, const twoldg_desc* d, const Csr& csr) {
   return 0;
 }
 
+bool valid_slot(int s) { return s >= 0 && s < TWOLDG_NSLOTS; }
+
+// A slot's device buffer, allocated on its first use (STATE exists from create).
+int ensure_slot(twoldg_problem* p, int s) {
+  if (p->slot[s]) return 0;
+  return dalloc_bytes(p, &p->slot[s], (size_t)p->state_len * p->elt);
+}
+
+// The staging buffers of state transfers: pinned host doubles, and (for a
+// download) device doubles, allocated once per problem and reused.
+int ensure_staging(twoldg_problem* p, bool device) {
+  const size_t bytes = (size_t)p->state_len * sizeof(double);
+  if (!p->pinned) HIP_TRY(hipHostMalloc((void**)&p->pinned, bytes, hipHostMallocDefault));
+  if (device && !p->dstage) return dalloc(p, &p->dstage, (size_t)p->state_len);
+  return 0;
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -388,69 +405,83 @@ int twoldg_problem_info(const twoldg_problem* p, int64_t* state_len, int64_t* de
 }
 
 int twoldg_state_upload(twoldg_problem* p, int slot, const double* host) {
-  if (!p || !host || slot < 0 || slot > 2) return fail(TWOLDG_EINVAL, "bad state upload");
+  if (!p || !host || !valid_slot(slot)) return fail(TWOLDG_EINVAL, "bad state upload");
+  int rc;
+  if ((rc = ensure_slot(p, slot)) || (rc = ensure_staging(p, false))) return rc;
   const int64_t n = p->state_len;
   // staged in pinned host memory and converted by a kernel that reads it there
-  double* pinned = nullptr;
-  HIP_TRY(hipHostMalloc((void**)&pinned, n * sizeof(double), hipHostMallocDefault));
-  std::memcpy(pinned, host, n * sizeof(double));
+  std::memcpy(p->pinned, host, n * sizeof(double));
   void* lo = (slot == TWOLDG_SLOT_STATE) ? p->lo : nullptr;
   if (p->precision == TWOLDG_F64)
     hipLaunchKernelGGL((k_from_double<double>), dim3(grid_for(n, 256)), dim3(256), 0, p->stream, n,
-                       (const double*)pinned, (double*)p->slot[slot], (double*)nullptr);
+                       (const double*)p->pinned, (double*)p->slot[slot], (double*)nullptr);
   else
     hipLaunchKernelGGL((k_from_double<float>), dim3(grid_for(n, 256)), dim3(256), 0, p->stream, n,
-                       (const double*)pinned, (float*)p->slot[slot], (float*)lo);
+                       (const double*)p->pinned, (float*)p->slot[slot], (float*)lo);
   hipError_t e = hipStreamSynchronize(p->stream);
-  (void)hipHostFree(pinned);
   if (e != hipSuccess) return fail(TWOLDG_EDEVICE, std::string("state upload: ") + hipGetErrorString(e));
   p->uploaded[slot] = true;
   return 0;
 }
 
 int twoldg_state_download(twoldg_problem* p, int slot, double* host) {
-  if (!p || !host || slot < 0 || slot > 2) return fail(TWOLDG_EINVAL, "bad state download");
+  if (!p || !host || !valid_slot(slot)) return fail(TWOLDG_EINVAL, "bad state download");
+  if (!p->uploaded[slot]) return fail(TWOLDG_ESTATE, "slot " + std::to_string(slot) + " holds no state");
+  int rc = ensure_staging(p, true);
+  if (rc) return rc;
   const int64_t n = p->state_len;
-  double* tmp = nullptr;
-  HIP_TRY(hipMalloc((void**)&tmp, n * sizeof(double)));
   const void* lo = (slot == TWOLDG_SLOT_STATE) ? p->lo : nullptr;
   if (p->precision == TWOLDG_F64)
     hipLaunchKernelGGL((k_to_double<double>), dim3(grid_for(n, 256)), dim3(256), 0, p->stream, n,
-                       (const double*)p->slot[slot], (const double*)nullptr, tmp);
+                       (const double*)p->slot[slot], (const double*)nullptr, p->dstage);
   else
     hipLaunchKernelGGL((k_to_double<float>), dim3(grid_for(n, 256)), dim3(256), 0, p->stream, n,
-                       (const float*)p->slot[slot], (const float*)lo, tmp);
-  hipError_t e = hipMemcpyAsync(host, tmp, n * sizeof(double), hipMemcpyDeviceToHost, p->stream);
+                       (const float*)p->slot[slot], (const float*)lo, p->dstage);
+  hipError_t e = hipMemcpyAsync(p->pinned, p->dstage, n * sizeof(double), hipMemcpyDeviceToHost,
+                                p->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
-  (void)hipFree(tmp);
   if (e != hipSuccess) return fail(TWOLDG_EDEVICE, std::string("state download: ") + hipGetErrorString(e));
+  std::memcpy(host, p->pinned, n * sizeof(double));
   return 0;
 }
 
 int twoldg_state_copy(twoldg_problem* p, int dst, int src) {
-  if (!p || dst < 0 || dst > 2 || src < 0 || src > 2) return fail(TWOLDG_EINVAL, "bad slot");
+  if (!p || !valid_slot(dst) || !valid_slot(src)) return fail(TWOLDG_EINVAL, "bad slot");
   if (dst == src) return 0;
-  HIP_TRY(hipMemcpyAsync(p->slot[dst], p->slot[src], p->state_len * p->elt,
-                         hipMemcpyDeviceToDevice, p->stream));
+  if (!p->uploaded[src]) return fail(TWOLDG_ESTATE, "slot " + std::to_string(src) + " holds no state");
+  int rc = ensure_slot(p, dst);
+  if (rc) return rc;
+  const int64_t n = p->state_len;
+  if (p->lo && src == TWOLDG_SLOT_STATE)  // the compensated value, rounded
+    hipLaunchKernelGGL((k_compensated_copy<float>), dim3(grid_for(n, 256)), dim3(256), 0,
+                       p->stream, n, (const float*)p->slot[src], (const float*)p->lo,
+                       (float*)p->slot[dst]);
+  else
+    HIP_TRY(hipMemcpyAsync(p->slot[dst], p->slot[src], n * p->elt, hipMemcpyDeviceToDevice,
+                           p->stream));
   if (p->lo && dst == TWOLDG_SLOT_STATE)
-    HIP_TRY(hipMemsetAsync(p->lo, 0, p->state_len * p->elt, p->stream));
+    HIP_TRY(hipMemsetAsync(p->lo, 0, n * p->elt, p->stream));
   HIP_TRY(hipStreamSynchronize(p->stream));
-  p->uploaded[dst] = p->uploaded[src];
+  p->uploaded[dst] = true;
   return 0;
 }
 
-int twoldg_rhs(twoldg_problem* p, double t) {
-  if (!p) return fail(TWOLDG_EINVAL, "no problem");
-  if (!p->uploaded[TWOLDG_SLOT_STATE]) return fail(TWOLDG_ESTATE, "no state uploaded");
-  int rc = TW_DISPATCH(rhs, p, t);
+int twoldg_rhs(twoldg_problem* p, int src, int dst, double t) {
+  if (!p || !valid_slot(src) || !valid_slot(dst) || src == dst)
+    return fail(TWOLDG_EINVAL, "the right-hand side needs two distinct slots");
+  if (!p->uploaded[src]) return fail(TWOLDG_ESTATE, "slot " + std::to_string(src) + " holds no state");
+  int rc = ensure_slot(p, dst);
+  if (rc) return rc;
+  rc = TW_DISPATCH(rhs, p, src, dst, t);
   if (rc) return rc;
   HIP_TRY(hipStreamSynchronize(p->stream));
-  p->uploaded[TWOLDG_SLOT_DU] = true;
+  p->uploaded[dst] = true;
   return 0;
 }
 
 int twoldg_truncate(twoldg_problem* p, int slot) {
-  if (!p || slot < 0 || slot > 2) return fail(TWOLDG_EINVAL, "bad slot");
+  if (!p || !valid_slot(slot)) return fail(TWOLDG_EINVAL, "bad slot");
+  if (!p->uploaded[slot]) return fail(TWOLDG_ESTATE, "slot " + std::to_string(slot) + " holds no state");
   int rc = TW_DISPATCH(trunc, p, slot);
   if (rc) return rc;
   HIP_TRY(hipStreamSynchronize(p->stream));
@@ -468,14 +499,17 @@ int twoldg_rk4(twoldg_problem* p, double t0, double dt, int64_t nsteps) {
 }
 
 int twoldg_energy(twoldg_problem* p, int slot, double t, double* out) {
-  if (!p || !out || slot < 0 || slot > 2) return fail(TWOLDG_EINVAL, "bad energy call");
+  if (!p || !out || !valid_slot(slot)) return fail(TWOLDG_EINVAL, "bad energy call");
+  if (!p->uploaded[slot]) return fail(TWOLDG_ESTATE, "slot " + std::to_string(slot) + " holds no state");
   return TW_DISPATCH(energy, p, slot, t, out);
 }
 
 int twoldg_norms(twoldg_problem* p, int a, int b, int vars_mask, int pnorm, double* lp,
                  double* linf) {
-  if (!p || !lp || !linf || a < 0 || a > 2 || b > 2 || b == a)
+  if (!p || !lp || !linf || !valid_slot(a) || b >= TWOLDG_NSLOTS || b == a)
     return fail(TWOLDG_EINVAL, "bad norm call");
+  if (!p->uploaded[a] || (b >= 0 && !p->uploaded[b]))
+    return fail(TWOLDG_ESTATE, "a norm of a slot that holds no state");
   if (pnorm < 1)
     return fail(TWOLDG_EINVAL, "wave_norm takes an integer p >= 1 or Inf, got p = " +
                                    std::to_string(pnorm));

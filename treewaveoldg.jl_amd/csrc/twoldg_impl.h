@@ -683,6 +683,15 @@ __global__ void k_to_double(int64_t n, const T* __restrict__ hi, const T* __rest
   dst[i] = x;
 }
 
+// The compensated value hi - lo rounded to T (a copy out of the compensated state).
+template <class T>
+__global__ void k_compensated_copy(int64_t n, const T* __restrict__ hi, const T* __restrict__ lo,
+                                   T* __restrict__ dst) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  dst[i] = (T)((double)hi[i] - (double)lo[i]);
+}
+
 #include "fused3.h"
 
 }  // namespace
@@ -702,7 +711,7 @@ struct twoldg_problem {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // device buffers (void*: element type follows precision)
-  void* slot[3] = {nullptr, nullptr, nullptr};
+  void* slot[TWOLDG_NSLOTS] = {nullptr, nullptr, nullptr, nullptr};  // STATE at create, others on first use
   void* lo = nullptr;       // compensation of the state (F32C)
   void* ys = nullptr;       // stage state
   void* acc = nullptr;      // RK accumulator
@@ -731,7 +740,11 @@ struct twoldg_problem {
   double hmin = 0, hspeed = 0;
   bool has_tables = false;
   int64_t device_bytes = 0;
-  bool uploaded[3] = {false, false, false};
+  bool uploaded[TWOLDG_NSLOTS] = {false, false, false, false};
+  // host <-> device staging of state transfers, allocated on the first
+  // transfer and kept: pinned host doubles and device doubles of one state
+  double* pinned = nullptr;
+  double* dstage = nullptr;
   // fused one-launch stages (3D, faces all same-level or mirror, flat metric)
   bool fused = false;
   bool no_fused = false;    // the caller asked for the generic stages
@@ -1259,11 +1272,11 @@ int rk4_steps(twoldg_problem* p, double t0, double dt, int64_t nsteps) {
 }
 
 template <class T, int D, int P1>
-int rhs_impl(twoldg_problem* p, double t) {
+int rhs_impl(twoldg_problem* p, int src, int dst, double t) {
   using L = Launch<T, D, P1>;
-  const T* y = (const T*)p->slot[TWOLDG_SLOT_STATE];
+  const T* y = (const T*)p->slot[src];
   L::gradient(p, y, t);
-  StageIO<T> io{y, y, nullptr, (T*)p->slot[TWOLDG_SLOT_DU], nullptr, (T)0};
+  StageIO<T> io{y, y, nullptr, (T*)p->slot[dst], nullptr, (T)0};
   L::template stage<0>(p, io);
   HIP_TRY(hipGetLastError());
   return 0;

@@ -10,7 +10,8 @@ from .problem import (WaveProblem, wave_rhs, truncate_state, truncate_stage, ini
                       fill_at_nodes, wave_energy, wave_norm, quadrature_norm, oldg_timestep,
                       wave_integrator, Integrator, exchange_plan)
 from .drivers import (wave_omega, wave_exact, reflect_omega, reflect_exact, gaussian_pulse,
-                      PlaneWave, sevenpatch_exact, wave_errors, reflect_errors, sevenpatch_errors)
+                      PlaneWave, sevenpatch_exact, wave_errors, reflect_errors, sevenpatch_errors,
+                      convergence_rate)
 from .batch import Batch, gaussian_pulses
 from . import _lib
 
@@ -20,6 +21,7 @@ __all__ = [
     "initial_state", "fill_at_nodes", "wave_energy", "wave_norm", "quadrature_norm",
     "oldg_timestep", "wave_integrator", "Integrator", "exchange_plan", "wave_omega",
     "wave_exact", "reflect_omega", "reflect_exact", "gaussian_pulse", "PlaneWave",
-    "sevenpatch_exact", "wave_errors", "reflect_errors", "sevenpatch_errors", "Batch",
+    "sevenpatch_exact", "wave_errors", "reflect_errors", "sevenpatch_errors",
+    "convergence_rate", "Batch",
     "gaussian_pulses",
 ]

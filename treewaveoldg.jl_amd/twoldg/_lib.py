@@ -10,10 +10,10 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(_HERE, "libtwoldg.so")
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 F64, F32C = 0, 1
 FACE_SAME, FACE_COARSER, FACE_FINER, FACE_MIRROR, FACE_GLUE, FACE_DIRICHLET = 1, 2, 3, 4, 5, 6
-SLOT_STATE, SLOT_DU, SLOT_AUX = 0, 1, 2
+SLOT_STATE, SLOT_DU, SLOT_AUX, SLOT_ARG = 0, 1, 2, 3
 FLAG_GENERIC = 1
 
 
@@ -65,7 +65,7 @@ def lib():
             "twoldg_state_upload": ([_vp, ctypes.c_int, _vp], ctypes.c_int),
             "twoldg_state_download": ([_vp, ctypes.c_int, _vp], ctypes.c_int),
             "twoldg_state_copy": ([_vp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
-            "twoldg_rhs": ([_vp, _dbl], ctypes.c_int),
+            "twoldg_rhs": ([_vp, ctypes.c_int, ctypes.c_int, _dbl], ctypes.c_int),
             "twoldg_truncate": ([_vp, ctypes.c_int], ctypes.c_int),
             "twoldg_rk4": ([_vp, _dbl, _dbl, _i64], ctypes.c_int),
             "twoldg_energy": ([_vp, ctypes.c_int, _dbl, _vp], ctypes.c_int),

@@ -82,12 +82,13 @@ class Batch:
         nodes = ctypes.c_int64()
         wg = ctypes.c_int64()
         tiles = (ctypes.c_int32 * 8)()
-        tms = (ctypes.c_double * 8)()
+        tms = (ctypes.c_double * 16)()
         _lib.check(_lib.lib().twoldg_batch_info(self._h, ctypes.byref(nodes), ctypes.byref(wg),
                                                 tiles, tms))
         return dict(nodes=nodes.value, workgroups=wg.value,
                     tiles={P: tiles[P - 1] for P in range(1, 9) if tiles[P - 1]},
-                    tune_ms={P: tms[P - 1] for P in range(1, 9) if tiles[P - 1]})
+                    tune_ms={P: tms[P - 1] for P in range(1, 9) if tiles[P - 1]},
+                    tune_full_ms={P: tms[8 + P - 1] for P in range(1, 9) if tiles[P - 1]})
 
 
 def gaussian_pulses(n=512, seed=20261203, orders=(2, 3, 4, 5, 6), roots=(4, 8, 16), E=2,

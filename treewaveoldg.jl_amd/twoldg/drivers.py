@@ -5,6 +5,7 @@ reflect_omega / reflect_exact  sinewave.jl:47-76
 wave_errors / reflect_errors   sinewave.jl:228-274
 PlaneWave / sevenpatch_exact   sevenpatch.jl:19-59
 sevenpatch_errors              sevenpatch.jl:84-117
+convergence_rate               evolution.jl:1534-1540
 
 The exact solutions are evaluated on the host (they are the caller's initial
 and comparison data, Julia closures in the reference); everything between the
@@ -18,6 +19,15 @@ from .element import Element
 from .forest import wave_forest
 from .problem import (WaveProblem, initial_state, fill_at_nodes, oldg_timestep, wave_integrator,
                       wave_norm, wave_energy, _lib)
+
+
+def convergence_rate(hs, errs):
+    """The least-squares slope of log(errs) against log(hs): the convergence
+    rate of a sequence of runs (evolution.jl:1534-1540)."""
+    x = np.log(np.asarray(hs, dtype=np.float64))
+    y = np.log(np.asarray(errs, dtype=np.float64))
+    xm, ym = x.mean(), y.mean()
+    return float(np.sum((x - xm) * (y - ym)) / np.sum((x - xm) ** 2))
 
 
 def wave_omega(D, L, m):

@@ -17,8 +17,11 @@ Mirrors TreeWaveOLDG.jl's Julia API (src/TreeWaveOLDG.jl:59-78):
 Julia's `!` functions mutate their first argument; here the numpy arrays passed
 in are written in place the same way.  State vectors are host numpy arrays in
 the statearray layout (block, variable, node; stepping.jl:36-41); the
-integrator keeps its state on the device between steps.  Every computation
-runs in libtwoldg on the GPU: there is no host fallback.
+integrator keeps its state on the device between steps, in the problem's state
+slot, which the pure entry points (wave_rhs, truncate_state, wave_energy, the
+norms) never touch: they pass their argument through the problem's argument
+slot.  Every computation runs in libtwoldg on the GPU: there is no host
+fallback.
 """
 import ctypes
 import math
@@ -115,6 +118,7 @@ class WaveProblem:
         _lib.check(_lib.lib().twoldg_problem_create(ctypes.byref(desc), ctypes.byref(handle)))
         self._h = handle
         self.state_len = self.nblocks * 2 * self.nn
+        self._state_owner = None  # the integrator whose state the state slot holds
 
     def close(self):
         if getattr(self, "_h", None):
@@ -222,9 +226,10 @@ def exchange_plan(forest, el=None):
 # The reference's entry points.
 # ---------------------------------------------------------------------------
 def wave_rhs(du, u, p, t=0.0):
-    """du <- f(u, t) in place (wave_rhs!).  Pure: u is not written."""
-    p.upload(u)
-    _lib.check(_lib.lib().twoldg_rhs(p._h, float(t)))
+    """du <- f(u, t) in place (wave_rhs!).  Pure: u is not written, and an
+    integrator of p keeps its state."""
+    p.upload(u, _lib.SLOT_ARG)
+    _lib.check(_lib.lib().twoldg_rhs(p._h, _lib.SLOT_ARG, _lib.SLOT_DU, float(t)))
     p.download(_lib.SLOT_DU, out=du)
     return None
 
@@ -233,9 +238,9 @@ def truncate_state(u, p):
     """v <- (Pi x ... x Pi) v in place; nothing when p does not truncate."""
     if not p.truncate:
         return u
-    p.upload(u)
-    _lib.check(_lib.lib().twoldg_truncate(p._h, _lib.SLOT_STATE))
-    p.download(_lib.SLOT_STATE, out=u)
+    p.upload(u, _lib.SLOT_ARG)
+    _lib.check(_lib.lib().twoldg_truncate(p._h, _lib.SLOT_ARG))
+    p.download(_lib.SLOT_ARG, out=u)
     return u
 
 
@@ -277,9 +282,9 @@ def wave_energy(p, u, t=None):
     if p.geometry is not None and t is None:
         raise ValueError("the problem has Dirichlet data, which depend on the time, and none "
                          "was given: pass `t`")
-    p.upload(u, _lib.SLOT_AUX)
+    p.upload(u, _lib.SLOT_ARG)
     out = ctypes.c_double()
-    _lib.check(_lib.lib().twoldg_energy(p._h, _lib.SLOT_AUX, float(t or 0.0), ctypes.byref(out)))
+    _lib.check(_lib.lib().twoldg_energy(p._h, _lib.SLOT_ARG, float(t or 0.0), ctypes.byref(out)))
     return out.value
 
 
@@ -291,10 +296,10 @@ def _norm(p, u, pn, vars):
         mask |= 1 << (v - 1)
     if not (pn == math.inf or (float(pn).is_integer() and pn >= 1)):
         raise ValueError(f"quadrature_norm takes an integer p >= 1 or Inf, got p = {pn}")
-    p.upload(u, _lib.SLOT_AUX)
+    p.upload(u, _lib.SLOT_ARG)
     lp = ctypes.c_double()
     li = ctypes.c_double()
-    _lib.check(_lib.lib().twoldg_norms(p._h, _lib.SLOT_AUX, -1, mask,
+    _lib.check(_lib.lib().twoldg_norms(p._h, _lib.SLOT_ARG, -1, mask,
                                        2 if pn == math.inf else int(pn), ctypes.byref(lp),
                                        ctypes.byref(li)))
     return li.value if pn == math.inf else lp.value
@@ -321,7 +326,12 @@ def oldg_timestep(p, cfl):
 
 class Integrator:
     """Classic RK4 with truncation as stage and step limiter, state on the
-    device (IMEXRungeKutta's init/step!/solve! semantics, stepping.jl:72-78)."""
+    device (IMEXRungeKutta's init/step!/solve! semantics, stepping.jl:72-78).
+
+    The integrator owns its problem's state slot: the problem's pure entry
+    points leave it alone.  Building another integrator on the same problem
+    takes the slot over, and this one then refuses to go on rather than step
+    the other's state."""
 
     def __init__(self, p, u0, t0, t1, dt):
         if not dt > 0:
@@ -332,6 +342,13 @@ class Integrator:
         self.dt = float(dt)
         self.istep = 0
         p.upload(u0, _lib.SLOT_STATE)
+        p._state_owner = self
+
+    def _own(self):
+        if self.p._state_owner is not self:
+            raise RuntimeError(
+                "another integrator was built on this problem since, and holds its state "
+                "slot: build one problem per concurrent integrator")
 
     @property
     def t(self):
@@ -339,11 +356,13 @@ class Integrator:
 
     @property
     def u(self):
+        self._own()
         return self.p.download(_lib.SLOT_STATE)
 
     def timed(self, n):
         """n steps timed with HIP events on the problem's stream: (total ms,
         summed stage-kernel ms, stage launches)."""
+        self._own()
         ms = ctypes.c_double()
         kms = ctypes.c_double()
         nl = ctypes.c_int64()
@@ -354,6 +373,7 @@ class Integrator:
         return ms.value, kms.value, nl.value
 
     def step(self, n=1):
+        self._own()
         n = min(n, self.nsteps - self.istep)
         if n <= 0:
             return
@@ -365,6 +385,7 @@ class Integrator:
         return self
 
     def energy(self):
+        self._own()
         out = ctypes.c_double()
         _lib.check(_lib.lib().twoldg_energy(self.p._h, _lib.SLOT_STATE, self.t,
                                             ctypes.byref(out)))
