@@ -294,8 +294,11 @@ def test_fused_step_graph_replays_match_eager_steps(case):
 def test_fused_large_forests_each_first_problem():
     """Large forests of small blocks (one wave per workgroup, several
     workgroups per CU), each forest's first problem and a rebuilt one, one step
-    against the oracle.  Guards the table uploads: tables copied by the DMA
-    path into reused device memory were read stale by the first problem."""
+    against the oracle.  Guards the host staging of tables and states: staged
+    in cacheable host memory, a fresh upload was at times read as the previous
+    upload's contents by the first problem of a new forest.  The uploaded state
+    must come back bit for bit before the step, and a mismatch after it names
+    the blocks and variables it lies in."""
     for P, E, roots, regions, refl in [(3, 2, 11, (), False),
                                        (3, 2, 8, O.nested_cubes(3, 1 / 8, 3), True)]:
         forest = T.wave_forest(3, P, E, roots=roots, reflecting=refl,
@@ -308,8 +311,13 @@ def test_fused_large_forests_each_first_problem():
             prob = T.WaveProblem(forest, T.Element(P))
             assert prob.info()["fused"]
             integ = T.wave_integrator(prob, y0, 0.0, dt, dt)
+            assert np.array_equal(integ.u, y0), ("upload", roots, rep)
             integ.solve()
-            assert np.linalg.norm(integ.u - y) <= 1e-13 * np.linalg.norm(y), (roots, rep)
+            got = integ.u
+            bad = np.abs(got - y).reshape(prob.nblocks, 2, -1) > 1e-10 * np.max(np.abs(y))
+            where = np.argwhere(bad.any(axis=2))
+            assert np.linalg.norm(got - y) <= 1e-13 * np.linalg.norm(y), \
+                (roots, rep, f"{len(where)} (block, variable) pairs differ, first {where[:8].tolist()}")
             prob.close()
 
 

@@ -41,14 +41,13 @@ int balloc(twoldg_batch* B, void** p, size_t bytes) {
   return 0;
 }
 
-// Host tables into device memory through pinned staging and a copy kernel on
-// the batch's stream (the copy path left stale values visible to the stage
-// kernels; see h2d in twoldg_impl.h).
+// Host tables into device memory through fine-grained host staging and a copy
+// kernel on the batch's stream (see host_stage_alloc and h2d in twoldg_impl.h).
 int bh2d(twoldg_batch* B, void* dst, const void* src, size_t bytes) {
   if (bytes == 0) return 0;
   const size_t words = (bytes + 3) / 4;
   void* pinned = nullptr;
-  HIP_TRY(hipHostMalloc(&pinned, words * 4, hipHostMallocDefault));
+  HIP_TRY(host_stage_alloc(&pinned, words * 4));
   std::memcpy(pinned, src, bytes);
   if (bytes % 4) std::memset((char*)pinned + bytes, 0, words * 4 - bytes);
   hipLaunchKernelGGL(k_copy_words, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, B->stream,
@@ -410,9 +409,9 @@ int twoldg_batch_destroy(twoldg_batch* B) {
 int twoldg_batch_upload(twoldg_batch* B, int64_t i, const double* host) {
   if (!B || !host || i < 0 || i >= B->nprob) return fail(TWOLDG_EINVAL, "bad batch upload");
   const int64_t n = B->prob_len[i], o = B->prob_off[i];
-  // staged in pinned host memory and converted by a kernel that reads it there
+  // staged in fine-grained host memory and converted by a kernel that reads it there
   double* tmp = nullptr;
-  HIP_TRY(hipHostMalloc((void**)&tmp, n * 8, hipHostMallocDefault));
+  HIP_TRY(host_stage_alloc((void**)&tmp, n * 8));
   std::memcpy(tmp, host, n * 8);
   if (B->precision == TWOLDG_F64)
     hipLaunchKernelGGL((k_from_double<double>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
@@ -430,8 +429,9 @@ int twoldg_batch_upload(twoldg_batch* B, int64_t i, const double* host) {
 int twoldg_batch_download(twoldg_batch* B, int64_t i, double* host) {
   if (!B || !host || i < 0 || i >= B->nprob) return fail(TWOLDG_EINVAL, "bad batch download");
   const int64_t n = B->prob_len[i], o = B->prob_off[i];
+  // converted by a kernel that writes fine-grained host staging directly
   double* tmp = nullptr;
-  HIP_TRY(hipMalloc((void**)&tmp, n * 8));
+  HIP_TRY(host_stage_alloc((void**)&tmp, n * 8));
   if (B->precision == TWOLDG_F64)
     hipLaunchKernelGGL((k_to_double<double>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
                        B->stream, n, (const double*)B->yn + o, (const double*)nullptr, tmp);
@@ -439,9 +439,9 @@ int twoldg_batch_download(twoldg_batch* B, int64_t i, double* host) {
     hipLaunchKernelGGL((k_to_double<float>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
                        B->stream, n, (const float*)B->yn + o,
                        B->lo ? (const float*)B->lo + o : nullptr, tmp);
-  hipError_t e = hipMemcpyAsync(host, tmp, n * 8, hipMemcpyDeviceToHost, B->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(B->stream);
-  (void)hipFree(tmp);
+  hipError_t e = hipStreamSynchronize(B->stream);
+  if (e == hipSuccess) std::memcpy(host, tmp, n * 8);
+  (void)hipHostFree(tmp);
   if (e != hipSuccess) return fail(TWOLDG_EDEVICE, std::string("batch download: ") + hipGetErrorString(e));
   return 0;
 }

@@ -16,6 +16,7 @@ int TW_FN(twoldg_i_trunc)(twoldg_problem* p, int slot) { return dispatch_p<Trunc
 int TW_FN(twoldg_i_energy)(twoldg_problem* p, int slot, double t, double* out) {
   return dispatch_p<EnergyF, TW_T, TW_D>(p, slot, t, out);
 }
-int TW_FN(twoldg_i_norms)(twoldg_problem* p, int a, int b, int mask, int pn, double* lp, double* li) {
-  return dispatch_p<NormsF, TW_T, TW_D>(p, a, b, mask, pn, lp, li);
+int TW_FN(twoldg_i_norms)(twoldg_problem* p, int a, int b, int mask, int pn, double* lp, double* li,
+                          double* part, double* red) {
+  return dispatch_p<NormsF, TW_T, TW_D>(p, a, b, mask, pn, lp, li, part, red);
 }

@@ -37,12 +37,18 @@ int twoldg_i_energy_f64_3(twoldg_problem*, int, double, double*);
 int twoldg_i_energy_f32_1(twoldg_problem*, int, double, double*);
 int twoldg_i_energy_f32_2(twoldg_problem*, int, double, double*);
 int twoldg_i_energy_f32_3(twoldg_problem*, int, double, double*);
-int twoldg_i_norms_f64_1(twoldg_problem*, int, int, int, int, double*, double*);
-int twoldg_i_norms_f64_2(twoldg_problem*, int, int, int, int, double*, double*);
-int twoldg_i_norms_f64_3(twoldg_problem*, int, int, int, int, double*, double*);
-int twoldg_i_norms_f32_1(twoldg_problem*, int, int, int, int, double*, double*);
-int twoldg_i_norms_f32_2(twoldg_problem*, int, int, int, int, double*, double*);
-int twoldg_i_norms_f32_3(twoldg_problem*, int, int, int, int, double*, double*);
+int twoldg_i_norms_f64_1(twoldg_problem*, int, int, int, int, double*, double*, double*,
+                         double*);
+int twoldg_i_norms_f64_2(twoldg_problem*, int, int, int, int, double*, double*, double*,
+                         double*);
+int twoldg_i_norms_f64_3(twoldg_problem*, int, int, int, int, double*, double*, double*,
+                         double*);
+int twoldg_i_norms_f32_1(twoldg_problem*, int, int, int, int, double*, double*, double*,
+                         double*);
+int twoldg_i_norms_f32_2(twoldg_problem*, int, int, int, int, double*, double*, double*,
+                         double*);
+int twoldg_i_norms_f32_3(twoldg_problem*, int, int, int, int, double*, double*, double*,
+                         double*);
 
 namespace {
 
@@ -57,7 +63,7 @@ void destroy(twoldg_problem* p) {
   if (!p) return;
   if (p->stream) (void)hipStreamSynchronize(p->stream);
   if (p->pinned) (void)hipHostFree(p->pinned);
-  for (void* b : {p->slot[0], p->slot[1], p->slot[2], p->slot[3], (void*)p->dstage, p->lo, p->ys, p->acc, p->ug, p->gg,
+  for (void* b : {p->slot[0], p->slot[1], p->slot[2], p->slot[3], p->lo, p->ys, p->acc, p->ug, p->gg,
                   p->grads, p->tables, p->coef, (void*)p->flat, (void*)p->tslot, p->tmodes,
                   p->tweights, p->tginv, (void*)p->spacing, (void*)p->rowptr, (void*)p->cb,
                   (void*)p->cl, p->cval_u, p->cval_g, (void*)p->dir_rows, (void*)p->dir_x,
@@ -303,18 +309,31 @@ int create_buffers(twoldg_problem* p, const twoldg_desc* d, const Csr& csr) {
 
 bool valid_slot(int s) { return s >= 0 && s < TWOLDG_NSLOTS; }
 
+int check_norm_args(twoldg_problem* p, int a, int b, int vars_mask, int pnorm, double* lp,
+                    double* linf) {
+  if (!p || !lp || !linf || !valid_slot(a) || b >= TWOLDG_NSLOTS || b == a)
+    return fail(TWOLDG_EINVAL, "bad norm call");
+  if (!p->uploaded[a] || (b >= 0 && !p->uploaded[b]))
+    return fail(TWOLDG_ESTATE, "a norm of a slot that holds no state");
+  if (pnorm < 1)
+    return fail(TWOLDG_EINVAL, "wave_norm takes an integer p >= 1 or Inf, got p = " +
+                                   std::to_string(pnorm));
+  if ((vars_mask & 3) == 0) return fail(TWOLDG_EINVAL, "no variable selected");
+  return 0;
+}
+
+
 // A slot's device buffer, allocated on its first use (STATE exists from create).
 int ensure_slot(twoldg_problem* p, int s) {
   if (p->slot[s]) return 0;
   return dalloc_bytes(p, &p->slot[s], (size_t)p->state_len * p->elt);
 }
 
-// The staging buffers of state transfers: pinned host doubles, and (for a
-// download) device doubles, allocated once per problem and reused.
-int ensure_staging(twoldg_problem* p, bool device) {
-  const size_t bytes = (size_t)p->state_len * sizeof(double);
-  if (!p->pinned) HIP_TRY(hipHostMalloc((void**)&p->pinned, bytes, hipHostMallocDefault));
-  if (device && !p->dstage) return dalloc(p, &p->dstage, (size_t)p->state_len);
+// The staging buffer of state transfers: fine-grained host doubles, allocated
+// once per problem and reused.
+int ensure_staging(twoldg_problem* p) {
+  if (!p->pinned)
+    HIP_TRY(host_stage_alloc((void**)&p->pinned, (size_t)p->state_len * sizeof(double)));
   return 0;
 }
 
@@ -342,6 +361,7 @@ int twoldg_problem_create(const twoldg_desc* d, twoldg_problem** out) {
   int rc = check_desc(d);
   if (rc) return rc;
   auto* p = new twoldg_problem();
+  (void)hipGetDevice(&p->device);
   p->D = d->D;
   p->P = d->P;
   p->P1 = d->P + 1;
@@ -407,7 +427,7 @@ int twoldg_problem_info(const twoldg_problem* p, int64_t* state_len, int64_t* de
 int twoldg_state_upload(twoldg_problem* p, int slot, const double* host) {
   if (!p || !host || !valid_slot(slot)) return fail(TWOLDG_EINVAL, "bad state upload");
   int rc;
-  if ((rc = ensure_slot(p, slot)) || (rc = ensure_staging(p, false))) return rc;
+  if ((rc = ensure_slot(p, slot)) || (rc = ensure_staging(p))) return rc;
   const int64_t n = p->state_len;
   // staged in pinned host memory and converted by a kernel that reads it there
   std::memcpy(p->pinned, host, n * sizeof(double));
@@ -427,19 +447,18 @@ int twoldg_state_upload(twoldg_problem* p, int slot, const double* host) {
 int twoldg_state_download(twoldg_problem* p, int slot, double* host) {
   if (!p || !host || !valid_slot(slot)) return fail(TWOLDG_EINVAL, "bad state download");
   if (!p->uploaded[slot]) return fail(TWOLDG_ESTATE, "slot " + std::to_string(slot) + " holds no state");
-  int rc = ensure_staging(p, true);
+  int rc = ensure_staging(p);
   if (rc) return rc;
   const int64_t n = p->state_len;
   const void* lo = (slot == TWOLDG_SLOT_STATE) ? p->lo : nullptr;
+  // converted by a kernel that writes the host staging directly
   if (p->precision == TWOLDG_F64)
     hipLaunchKernelGGL((k_to_double<double>), dim3(grid_for(n, 256)), dim3(256), 0, p->stream, n,
-                       (const double*)p->slot[slot], (const double*)nullptr, p->dstage);
+                       (const double*)p->slot[slot], (const double*)nullptr, p->pinned);
   else
     hipLaunchKernelGGL((k_to_double<float>), dim3(grid_for(n, 256)), dim3(256), 0, p->stream, n,
-                       (const float*)p->slot[slot], (const float*)lo, p->dstage);
-  hipError_t e = hipMemcpyAsync(p->pinned, p->dstage, n * sizeof(double), hipMemcpyDeviceToHost,
-                                p->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
+                       (const float*)p->slot[slot], (const float*)lo, p->pinned);
+  hipError_t e = hipStreamSynchronize(p->stream);
   if (e != hipSuccess) return fail(TWOLDG_EDEVICE, std::string("state download: ") + hipGetErrorString(e));
   std::memcpy(host, p->pinned, n * sizeof(double));
   return 0;
@@ -506,15 +525,9 @@ int twoldg_energy(twoldg_problem* p, int slot, double t, double* out) {
 
 int twoldg_norms(twoldg_problem* p, int a, int b, int vars_mask, int pnorm, double* lp,
                  double* linf) {
-  if (!p || !lp || !linf || !valid_slot(a) || b >= TWOLDG_NSLOTS || b == a)
-    return fail(TWOLDG_EINVAL, "bad norm call");
-  if (!p->uploaded[a] || (b >= 0 && !p->uploaded[b]))
-    return fail(TWOLDG_ESTATE, "a norm of a slot that holds no state");
-  if (pnorm < 1)
-    return fail(TWOLDG_EINVAL, "wave_norm takes an integer p >= 1 or Inf, got p = " +
-                                   std::to_string(pnorm));
-  if ((vars_mask & 3) == 0) return fail(TWOLDG_EINVAL, "no variable selected");
-  return TW_DISPATCH(norms, p, a, b, vars_mask & 3, pnorm, lp, linf);
+  int rc = check_norm_args(p, a, b, vars_mask, pnorm, lp, linf);
+  if (rc) return rc;
+  return TW_DISPATCH(norms, p, a, b, vars_mask & 3, pnorm, lp, linf, p->part, p->red);
 }
 
 int twoldg_timestep(const twoldg_problem* p, double cfl, double* out) {

@@ -708,6 +708,7 @@ struct twoldg_problem {
   int64_t state_len = 0;
   int64_t nrows = 0;
   size_t elt = 8;
+  int device = 0;  // the device current at create: the stream's and every buffer's
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // device buffers (void*: element type follows precision)
@@ -742,9 +743,9 @@ struct twoldg_problem {
   int64_t device_bytes = 0;
   bool uploaded[TWOLDG_NSLOTS] = {false, false, false, false};
   // host <-> device staging of state transfers, allocated on the first
-  // transfer and kept: pinned host doubles and device doubles of one state
+  // transfer and kept: fine-grained host doubles of one state, which the
+  // conversion kernels read (upload) and write (download) directly
   double* pinned = nullptr;
-  double* dstage = nullptr;
   // fused one-launch stages (3D, faces all same-level or mirror, flat metric)
   bool fused = false;
   bool no_fused = false;    // the caller asked for the generic stages
@@ -798,11 +799,22 @@ int dalloc_bytes(twoldg_problem* p, void** ptr, size_t bytes) {
   return 0;
 }
 
-// Host data into device memory: staged in pinned host memory and written by a
-// kernel on the problem's stream, so that the device's caches hold the new
-// values.  Tables copied into freshly allocated device memory by the DMA path
-// (hipMemcpy) were at times read stale by the stage kernels of the first
-// problem built on reused memory; a kernel's stores go through the L2.
+// Host memory the GPU reads or writes directly (staging of tables and states).
+// It is allocated fine-grained (coherent): the GPU does not cache it, so a
+// kernel always sees what the host wrote last, and the host what a kernel
+// wrote once the stream is synchronized.  The default hipHostMalloc memory is
+// coarse-grained and may be cached by the GPU; a new allocation can reuse the
+// host pages of a freed one whose lines are still cached, and then part of a
+// fresh upload read as the previous one's contents (seen on the first problem
+// built on a new forest in a process, whose state and tables differ from the
+// last problem's; a rebuilt problem, uploading the same data to the same pages,
+// read right).
+inline hipError_t host_stage_alloc(void** ptr, size_t bytes) {
+  return hipHostMalloc(ptr, bytes, hipHostMallocCoherent);
+}
+
+// Host data into device memory: staged in fine-grained host memory and written
+// by a kernel on the problem's stream (no DMA engine touches the device copy).
 __global__ void k_copy_words(int64_t n, const uint32_t* __restrict__ src, uint32_t* __restrict__ dst) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) dst[i] = src[i];
@@ -812,7 +824,7 @@ int h2d(twoldg_problem* p, void* dst, const void* src, size_t bytes) {
   if (bytes == 0) return 0;
   const size_t words = (bytes + 3) / 4;
   void* pinned = nullptr;
-  HIP_TRY(hipHostMalloc(&pinned, words * 4, hipHostMallocDefault));
+  HIP_TRY(host_stage_alloc(&pinned, words * 4));
   std::memcpy(pinned, src, bytes);
   if (bytes % 4) std::memset((char*)pinned + bytes, 0, words * 4 - bytes);
   // whole words only: the device allocations are at least 4-byte granular
@@ -1098,17 +1110,17 @@ struct Launch {
                        false, p->red);
   }
   static void norms(twoldg_problem* p, const T* a, const T* alo, const T* b, const T* blo,
-                    int mask, int pnorm) {
+                    int mask, int pnorm, double* part, double* red) {
     if (!p->full)
       hipLaunchKernelGGL((k_norm_part<T, D, P1, false>), dim3((unsigned)p->sh.nb),
                          dim3(kRedThreads), 0, p->stream, p->sh, (const Tables<T>*)p->tables,
-                         p->spacing, a, alo, b, blo, (const T*)nullptr, mask, pnorm, p->part);
+                         p->spacing, a, alo, b, blo, (const T*)nullptr, mask, pnorm, part);
     else
       hipLaunchKernelGGL((k_norm_part<T, D, P1, true>), dim3((unsigned)p->sh.nb),
                          dim3(kRedThreads), 0, p->stream, p->sh, (const Tables<T>*)p->tables,
-                         p->spacing, a, alo, b, blo, (const T*)p->coef, mask, pnorm, p->part);
-    hipLaunchKernelGGL((k_final<3>), dim3(1), dim3(kRedThreads), 0, p->stream, p->sh.nb, p->part,
-                       true, p->red);
+                         p->spacing, a, alo, b, blo, (const T*)p->coef, mask, pnorm, part);
+    hipLaunchKernelGGL((k_final<3>), dim3(1), dim3(kRedThreads), 0, p->stream, p->sh.nb, part,
+                       true, red);
   }
 };
 
@@ -1304,15 +1316,18 @@ int energy_impl(twoldg_problem* p, int slot, double t, double* out) {
   return 0;
 }
 
+// part: 3 nb doubles of per-block partials, red: 3 doubles of final sums
+// (the problem's own buffers, or the no-allocation entry point's scratch).
 template <class T, int D, int P1>
-int norms_impl(twoldg_problem* p, int a, int b, int mask, int pnorm, double* lp, double* linf) {
+int norms_impl(twoldg_problem* p, int a, int b, int mask, int pnorm, double* lp, double* linf,
+               double* part, double* red) {
   using L = Launch<T, D, P1>;
   const T* alo = (a == TWOLDG_SLOT_STATE) ? (const T*)p->lo : nullptr;
   const T* blo = (b == TWOLDG_SLOT_STATE) ? (const T*)p->lo : nullptr;
   L::norms(p, (const T*)p->slot[a], alo, b >= 0 ? (const T*)p->slot[b] : nullptr, blo, mask,
-           pnorm);
+           pnorm, part, red);
   double h[3];
-  HIP_TRY(hipMemcpyAsync(h, p->red, 3 * sizeof(double), hipMemcpyDeviceToHost, p->stream));
+  HIP_TRY(hipMemcpyAsync(h, red, 3 * sizeof(double), hipMemcpyDeviceToHost, p->stream));
   HIP_TRY(hipStreamSynchronize(p->stream));
   *lp = h[1] == 0 ? 0 : std::pow(h[0] / h[1], 1.0 / pnorm);
   *linf = h[2];

@@ -110,7 +110,8 @@ def exported_symbols():
     return ["twoldg_abi_version", "twoldg_last_error", "twoldg_device_count",
             "twoldg_problem_create", "twoldg_problem_destroy", "twoldg_problem_info",
             "twoldg_state_upload", "twoldg_state_download", "twoldg_state_copy", "twoldg_rhs",
-            "twoldg_truncate", "twoldg_rk4", "twoldg_energy", "twoldg_norms", "twoldg_timestep",
+            "twoldg_truncate", "twoldg_rk4", "twoldg_energy", "twoldg_norms",
+            "twoldg_timestep",
             "twoldg_synchronize", "twoldg_exchange_plan", "twoldg_node_coordinates",
             "twoldg_rk4_timed",
             "twoldg_batch_create", "twoldg_batch_destroy", "twoldg_batch_upload",
