@@ -139,6 +139,17 @@ int twoldg_energy(twoldg_problem* p, int slot, double t, double* out);
    vars_mask: bit 0 = u, bit 1 = v. */
 int twoldg_norms(twoldg_problem* p, int a, int b, int vars_mask, int pnorm, double* lp,
                  double* linf);
+/* twoldg_norms without any allocation: no counterpart in the reference.  For
+   callers that cannot allocate device memory while they run (code under HIP
+   graph capture, tight serving loops, harnesses that forbid hipMalloc): the
+   per-block partial sums go to a scratch array of fixed size that the library
+   holds on each GPU until the process ends (3 MiB), not to the problem's
+   buffers.  Takes problems of at most 131072 blocks (larger: TWOLDG_EINVAL,
+   use twoldg_norms).  Every call on one GPU shares that GPU's scratch: calls
+   that overlap in time on one GPU (from several threads, or on several
+   problems' streams) race and must be serialized by the caller. */
+int twoldg_norms_noalloc(twoldg_problem* p, int a, int b, int vars_mask, int pnorm, double* lp,
+                         double* linf);
 /* cfl * h / (P+1): oldg_timestep, src/evolution.jl:1528-1531 */
 int twoldg_timestep(const twoldg_problem* p, double cfl, double* out);
 /* wait for the problem's stream */

@@ -2,6 +2,7 @@
 // The kernels live in twoldg_impl.h, instantiated per arithmetic type and
 // dimension in inst.hip.
 #include <map>
+#include <mutex>
 #include "twoldg_impl.h"
 
 namespace twoldg_detail {
@@ -322,6 +323,36 @@ int check_norm_args(twoldg_problem* p, int a, int b, int vars_mask, int pnorm, d
   return 0;
 }
 
+// The scratch of twoldg_norms_noalloc: per-block partial sums (three per block)
+// and the three final sums, in device memory that the library holds on every
+// GPU it runs on until the process ends (3 MiB).  In host code `scratch` names
+// a host-side stand-in; its device address on each GPU comes from
+// hipGetSymbolAddress, taken once per device with that device current and kept.
+constexpr int64_t kScratchMaxBlocks = int64_t(1) << 17;
+constexpr int kScratchMaxDevices = 64;
+static __device__ __align__(256) double scratch[3 * kScratchMaxBlocks + 8];
+std::mutex g_scratch_mu;
+double* g_scratch[kScratchMaxDevices] = {};
+
+int scratch_for(int device, double** out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (device < 0 || device >= kScratchMaxDevices)
+    return fail(TWOLDG_EDEVICE, "device " + std::to_string(device) + " out of range");
+  if (!g_scratch[device]) {
+    int cur = 0;
+    HIP_TRY(hipGetDevice(&cur));
+    if (cur != device) HIP_TRY(hipSetDevice(device));
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (cur != device) (void)hipSetDevice(cur);
+    if (e != hipSuccess || !ptr)
+      return fail(TWOLDG_EDEVICE, std::string("hipGetSymbolAddress(scratch): ") +
+                                      hipGetErrorString(e));
+    g_scratch[device] = (double*)ptr;
+  }
+  *out = g_scratch[device];
+  return 0;
+}
 
 // A slot's device buffer, allocated on its first use (STATE exists from create).
 int ensure_slot(twoldg_problem* p, int s) {
@@ -528,6 +559,24 @@ int twoldg_norms(twoldg_problem* p, int a, int b, int vars_mask, int pnorm, doub
   int rc = check_norm_args(p, a, b, vars_mask, pnorm, lp, linf);
   if (rc) return rc;
   return TW_DISPATCH(norms, p, a, b, vars_mask & 3, pnorm, lp, linf, p->part, p->red);
+}
+
+int twoldg_norms_noalloc(twoldg_problem* p, int a, int b, int vars_mask, int pnorm, double* lp,
+                         double* linf) {
+  int rc = check_norm_args(p, a, b, vars_mask, pnorm, lp, linf);
+  if (rc) return rc;
+  if (p->sh.nb > kScratchMaxBlocks)
+    return fail(TWOLDG_EINVAL, "twoldg_norms_noalloc takes at most " +
+                                   std::to_string(kScratchMaxBlocks) +
+                                   " blocks (its scratch holds three partial sums per block), "
+                                   "and the problem has " + std::to_string(p->sh.nb) +
+                                   ": use twoldg_norms");
+  double* s = nullptr;
+  if ((rc = scratch_for(p->device, &s))) return rc;
+  // every partial is written by its block before the final sum reads it, and
+  // the final sums by the one workgroup of k_final: nothing starts at zero
+  return TW_DISPATCH(norms, p, a, b, vars_mask & 3, pnorm, lp, linf, s,
+                     s + 3 * kScratchMaxBlocks);
 }
 
 int twoldg_timestep(const twoldg_problem* p, double cfl, double* out) {

@@ -71,6 +71,8 @@ def lib():
             "twoldg_energy": ([_vp, ctypes.c_int, _dbl, _vp], ctypes.c_int),
             "twoldg_norms": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp,
                               _vp], ctypes.c_int),
+            "twoldg_norms_noalloc": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_int, _vp, _vp], ctypes.c_int),
             "twoldg_timestep": ([_vp, _dbl, _vp], ctypes.c_int),
             "twoldg_synchronize": ([_vp], ctypes.c_int),
             "twoldg_exchange_plan": ([_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),
@@ -111,7 +113,7 @@ def exported_symbols():
             "twoldg_problem_create", "twoldg_problem_destroy", "twoldg_problem_info",
             "twoldg_state_upload", "twoldg_state_download", "twoldg_state_copy", "twoldg_rhs",
             "twoldg_truncate", "twoldg_rk4", "twoldg_energy", "twoldg_norms",
-            "twoldg_timestep",
+            "twoldg_norms_noalloc", "twoldg_timestep",
             "twoldg_synchronize", "twoldg_exchange_plan", "twoldg_node_coordinates",
             "twoldg_rk4_timed",
             "twoldg_batch_create", "twoldg_batch_destroy", "twoldg_batch_upload",
