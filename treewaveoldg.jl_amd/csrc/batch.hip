@@ -11,12 +11,14 @@ struct twoldg_batch {
   int64_t nprob = 0, nb = 0, total = 0, nfaces = 0;
   std::vector<int64_t> prob_off, prob_len;  // state slice of each problem
   std::vector<int32_t> P1s_host;
+  std::vector<int64_t> off_host, foff_host;  // per block: state and face-buffer offsets
   int tiles[10] = {0};                      // blocks per workgroup by P1
   double tune_ms[10] = {0};                 // first autotune pass: the order's launch alone
   double tune_full_ms[10] = {0};            // second pass: the full launch with the chosen tile
   // device
-  int64_t *off = nullptr, *foff = nullptr;
-  int32_t *nbr = nullptr, *P1s = nullptr, *Ns = nullptr;
+  int64_t *off = nullptr, *foff = nullptr, *nfo = nullptr;
+  void* nrh = nullptr;
+  int32_t *P1s = nullptr, *Ns = nullptr;
   double *spacing = nullptr, *dt = nullptr;
   void* tables = nullptr;
   BatchItem* items = nullptr;
@@ -77,16 +79,21 @@ int batch_tables(twoldg_batch* B) {
   return 0;
 }
 
-// Blocks per workgroup that fit the thread count and the LDS of the kernel.
-int max_tile(int N) {
-  const int NT = N * N, NP = N + 2;
-  const int per = NP * NP + 2 * N * NP + 3 * NT;
-  return std::max(1, std::min(kBatchThreads / NT, kBatchLds / per));
+// Blocks per workgroup that fit the threads' nodes and the LDS of the kernel
+// (elt: bytes per value).
+int max_tile(int N, size_t elt) {
+  const int NT = N * N, per = batch_per_block(N);
+  const int nodes = elt == 4 ? batch_threads<float>() * batch_k<float>()
+                             : batch_threads<double>() * batch_k<double>();
+  const int lds = elt == 4 ? batch_lds<float>() : batch_lds<double>();
+  return std::min(nodes / NT, lds / per);
 }
 
 // The work list with tile[P1] blocks per entry; only orders in `only` (0: all).
 std::vector<BatchItem> make_items(const twoldg_batch* B, const std::vector<int32_t>& Nh,
                                   const int* tile, int only) {
+  const std::vector<int64_t>& offh = B->off_host;
+  const std::vector<int64_t>& foffh = B->foff_host;
   std::vector<BatchItem> items;
   int64_t b = 0;
   while (b < B->nb) {
@@ -96,7 +103,8 @@ std::vector<BatchItem> make_items(const twoldg_batch* B, const std::vector<int32
     if (only == 0 || only == P1) {
       const int t = std::max(1, tile[P1]);
       for (int64_t f = b; f < e; f += t)
-        items.push_back(BatchItem{P1, N, (int32_t)std::min<int64_t>(t, e - f), 0, f});
+        items.push_back(BatchItem{P1, N, (int32_t)std::min<int64_t>(t, e - f), 0, f, offh[f],
+                                  foffh[f]});
     }
     b = e;
   }
@@ -110,7 +118,8 @@ void launch_stage(twoldg_batch* B, const BatchItem* items, int64_t nitems, const
   A.items = items;
   A.off = B->off;
   A.foff = B->foff;
-  A.nbr = B->nbr;
+  A.nfo = B->nfo;
+  A.nrh = (const T*)B->nrh;
   A.spacing = B->spacing;
   A.dt = B->dt;
   A.tables = (const Tables<T>*)B->tables;
@@ -125,7 +134,7 @@ void launch_stage(twoldg_batch* B, const BatchItem* items, int64_t nitems, const
   A.pf_out = (T*)B->pf[1 - in];
   A.coef = coef;
   A.truncate = B->truncate ? 1 : 0;
-  hipLaunchKernelGGL((k_batch2<T, MODE>), dim3((unsigned)nitems), dim3(kBatchThreads), 0,
+  hipLaunchKernelGGL((k_batch2<T, MODE>), dim3((unsigned)nitems), dim3(batch_threads<T>()), 0,
                      B->stream, A);
 }
 
@@ -160,7 +169,7 @@ int autotune(twoldg_batch* B, const std::vector<int32_t>& Nh) {
     int N = 0;
     for (int64_t b = 0; b < B->nb; ++b)
       if (B->P1s_host[b] == P1) N = std::max<int>(N, Nh[b]);
-    const int tmax = max_tile(N);
+    const int tmax = max_tile(N, sizeof(T));
     float best = 1e30f;
     int besttile = 1;
     for (int t = 1; t <= tmax; t = (t == tmax) ? tmax + 1 : std::min(2 * t, tmax)) {
@@ -195,7 +204,7 @@ int autotune(twoldg_batch* B, const std::vector<int32_t>& Nh) {
     int N = 0;
     for (int64_t b = 0; b < B->nb; ++b)
       if (B->P1s_host[b] == P1) N = std::max<int>(N, Nh[b]);
-    const int tmax = max_tile(N);
+    const int tmax = max_tile(N, sizeof(T));
     float best = 1e30f;
     int besttile = B->tiles[P1];
     for (int t = 1; t <= tmax; t = (t == tmax) ? tmax + 1 : std::min(2 * t, tmax)) {
@@ -240,7 +249,7 @@ int batch_build(twoldg_batch* B, const twoldg_desc* const* d, int64_t n, const d
   B->prob_off.assign(n, 0);
   B->prob_len.assign(n, 0);
   std::vector<int64_t> first(n, 0), offh, foffh;
-  std::vector<int32_t> nbrh, Nh;
+  std::vector<int32_t> Nh;
   std::vector<double> hh, dth;
   int64_t nb = 0, total = 0, fsum = 0;
   for (int64_t oi = 0; oi < n; ++oi) {
@@ -262,27 +271,43 @@ int batch_build(twoldg_batch* B, const twoldg_desc* const* d, int64_t n, const d
     }
     nb += p->nblocks;
   }
+  // per block face: the face-buffer index of the neighbour's facing face and
+  // its reciprocal spacing (-1 and 0 for a mirror)
+  std::vector<int64_t> nfoh;
+  std::vector<T> nrhh;
   for (int64_t oi = 0; oi < n; ++oi) {
     const int64_t i = order[oi];
     const twoldg_desc* p = d[i];
+    const int N = p->E * (p->P + 1);
     for (int64_t b = 0; b < p->nblocks; ++b)
       for (int f = 0; f < 4; ++f) {
         const twoldg_face& F = p->faces[b * 4 + f];
-        nbrh.push_back(F.kind == TWOLDG_FACE_SAME ? (int32_t)(first[i] + F.nb[0]) : -1);
+        if (F.kind == TWOLDG_FACE_SAME) {
+          const int64_t nb_ = first[i] + F.nb[0];
+          nfoh.push_back(foffh[nb_] + (int64_t)(f ^ 1) * N);
+          nrhh.push_back((T)(1.0 / p->spacing[F.nb[0]]));
+        } else {
+          nfoh.push_back(-1);
+          nrhh.push_back(T(0));
+        }
       }
   }
   B->nb = nb;
   B->total = total;
   B->nfaces = fsum;
   int rc;
+  B->off_host = offh;
+  B->foff_host = foffh;
   if ((rc = balloc(B, (void**)&B->off, nb * 8)) || (rc = balloc(B, (void**)&B->foff, nb * 8)) ||
-      (rc = balloc(B, (void**)&B->nbr, nb * 16)) || (rc = balloc(B, (void**)&B->P1s, nb * 4)) ||
+      (rc = balloc(B, (void**)&B->nfo, nb * 32)) || (rc = balloc(B, &B->nrh, nb * 4 * sizeof(T))) ||
+      (rc = balloc(B, (void**)&B->P1s, nb * 4)) ||
       (rc = balloc(B, (void**)&B->Ns, nb * 4)) || (rc = balloc(B, (void**)&B->spacing, nb * 8)) ||
       (rc = balloc(B, (void**)&B->dt, nb * 8)))
     return rc;
   if ((rc = bh2d(B, B->off, offh.data(), nb * 8))) return rc;
   if ((rc = bh2d(B, B->foff, foffh.data(), nb * 8))) return rc;
-  if ((rc = bh2d(B, B->nbr, nbrh.data(), nb * 16))) return rc;
+  if ((rc = bh2d(B, B->nfo, nfoh.data(), nb * 32))) return rc;
+  if ((rc = bh2d(B, B->nrh, nrhh.data(), nb * 4 * sizeof(T)))) return rc;
   if ((rc = bh2d(B, B->P1s, B->P1s_host.data(), nb * 4))) return rc;
   if ((rc = bh2d(B, B->Ns, Nh.data(), nb * 4))) return rc;
   if ((rc = bh2d(B, B->spacing, hh.data(), nb * 8))) return rc;
@@ -316,7 +341,7 @@ void batch_destroy(twoldg_batch* B) {
   if (B->stream) (void)hipStreamSynchronize(B->stream);
   if (B->exec) (void)hipGraphExecDestroy(B->exec);
   if (B->graph) (void)hipGraphDestroy(B->graph);
-  for (void* p : {(void*)B->off, (void*)B->foff, (void*)B->nbr, (void*)B->P1s, (void*)B->Ns,
+  for (void* p : {(void*)B->off, (void*)B->foff, (void*)B->nfo, B->nrh, (void*)B->P1s, (void*)B->Ns,
                   (void*)B->spacing, (void*)B->dt, B->tables, (void*)B->items, B->yn, B->ys,
                   B->acc, B->lo, B->uf[0], B->uf[1], B->pf[0], B->pf[1]})
     if (p) (void)hipFree(p);
@@ -362,7 +387,7 @@ int twoldg_batch_create(const twoldg_desc* const* descs, int64_t n, const double
     if (d->nblocks < 1 || !d->spacing || !d->faces)
       return fail(TWOLDG_EINVAL, who + "a problem needs at least one block, its spacings and faces");
     const int N = d->E * (d->P + 1);
-    if (N * N > kBatchThreads || max_tile(N) < 1)
+    if (max_tile(N, d->precision == TWOLDG_F32C ? 4 : 8) < 1)
       return fail(TWOLDG_EINVAL, who + "blocks of " + std::to_string(N) +
                                      "^2 nodes exceed a workgroup");
     if (d->precision != descs[0]->precision || d->truncate != descs[0]->truncate)
