@@ -35,7 +35,7 @@ struct twoldg_batch {
 namespace {
 
 int balloc(twoldg_batch* B, void** p, size_t bytes) {
-  hipError_t e = hipMalloc(p, bytes ? bytes : 8);
+  hipError_t e = pool_alloc(p, bytes ? bytes : 8, false);
   if (e != hipSuccess)
     return fail(TWOLDG_ENOMEM, std::string("hipMalloc of ") + std::to_string(bytes) +
                                    " bytes: " + hipGetErrorString(e));
@@ -55,7 +55,7 @@ int bh2d(twoldg_batch* B, void* dst, const void* src, size_t bytes) {
   hipLaunchKernelGGL(k_copy_words, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, B->stream,
                      (int64_t)words, (const uint32_t*)pinned, (uint32_t*)dst);
   hipError_t e = hipStreamSynchronize(B->stream);
-  (void)hipHostFree(pinned);
+  pool_free(pinned);
   if (e != hipSuccess) return fail(TWOLDG_EDEVICE, std::string("batch table upload: ") + hipGetErrorString(e));
   return 0;
 }
@@ -233,7 +233,7 @@ int autotune(twoldg_batch* B, const std::vector<int32_t>& Nh) {
     B->tiles[P1] = besttile;
     B->tune_full_ms[P1] = best / kReps;
   }
-  (void)hipFree(scratch);
+  pool_free(scratch);
   return 0;
 }
 
@@ -344,7 +344,7 @@ void batch_destroy(twoldg_batch* B) {
   for (void* p : {(void*)B->off, (void*)B->foff, (void*)B->nfo, B->nrh, (void*)B->P1s, (void*)B->Ns,
                   (void*)B->spacing, (void*)B->dt, B->tables, (void*)B->items, B->yn, B->ys,
                   B->acc, B->lo, B->uf[0], B->uf[1], B->pf[0], B->pf[1]})
-    if (p) (void)hipFree(p);
+    if (p) pool_free(p);
   if (B->ev0) (void)hipEventDestroy(B->ev0);
   if (B->ev1) (void)hipEventDestroy(B->ev1);
   if (B->stream) (void)hipStreamDestroy(B->stream);
@@ -446,7 +446,7 @@ int twoldg_batch_upload(twoldg_batch* B, int64_t i, const double* host) {
                        B->stream, n, (const double*)tmp, (float*)B->yn + o,
                        B->lo ? (float*)B->lo + o : nullptr);
   hipError_t e = hipStreamSynchronize(B->stream);
-  (void)hipHostFree(tmp);
+  pool_free(tmp);
   if (e != hipSuccess) return fail(TWOLDG_EDEVICE, std::string("batch upload: ") + hipGetErrorString(e));
   return 0;
 }
@@ -466,7 +466,7 @@ int twoldg_batch_download(twoldg_batch* B, int64_t i, double* host) {
                        B->lo ? (const float*)B->lo + o : nullptr, tmp);
   hipError_t e = hipStreamSynchronize(B->stream);
   if (e == hipSuccess) std::memcpy(host, tmp, n * 8);
-  (void)hipHostFree(tmp);
+  pool_free(tmp);
   if (e != hipSuccess) return fail(TWOLDG_EDEVICE, std::string("batch download: ") + hipGetErrorString(e));
   return 0;
 }

@@ -167,7 +167,7 @@ k_fused3(FusedArgs<T> A) {
     return v;
   };
 
-  const T h = (T)A.spacing[b];
+  const T h = (T)ld_table(A.spacing + b);
   const T rh = T(1) / h;
   const int32_t* nb6 = A.nbr + b * 6;
   // the block's arrays: uniform bases, per-thread 32-bit offsets (z stride N2,
@@ -181,7 +181,8 @@ k_fused3(FusedArgs<T> A) {
   T* lo = A.lo ? A.lo + b * 2 * nn : nullptr;
   T* ufo = A.uf_out + b * 6 * N2;
   T* pfo = A.pf_out + b * 6 * N2;
-  const int nxl = nb6[0], nxh = nb6[1], nyl = nb6[2], nyh = nb6[3], nzl = nb6[4], nzh = nb6[5];
+  const int nxl = ld_table(nb6 + 0), nxh = ld_table(nb6 + 1), nyl = ld_table(nb6 + 2),
+            nyh = ld_table(nb6 + 3), nzl = ld_table(nb6 + 4), nzh = ld_table(nb6 + 5);
   // the reciprocal spacing of a same-level neighbour, read where it is used
   // the reciprocal spacings of the same-level neighbours, per face, through LDS:
   // read there, they wait on the LDS counter, not on the stage's stores
@@ -283,8 +284,8 @@ k_fused3(FusedArgs<T> A) {
   T prev_last = 0, prev_part = 0;  // previous layer: last u, row-P partial sum
   load_layer(0, uc);
   if (tid < 6) {
-    const int nb = nb6[tid];
-    RSP[tid] = nb >= 0 ? T(1) / (T)A.spacing[nb] : T(0);
+    const int nb = ld_table(nb6 + tid);
+    RSP[tid] = nb >= 0 ? T(1) / (T)ld_table(A.spacing + nb) : T(0);
   }
   lds_barrier();
 
@@ -662,12 +663,13 @@ __global__ void k_face_ghost_u(int64_t ntot, FaceGroups G, int64_t N2,
   if (!face_group_row(G, i, e, stride, count)) return;
   T acc = 0;
 #pragma unroll 8
-  for (int j = 0; j < count; ++j) acc += wu[e + j * stride] * uf[src[e + j * stride]];
-  const int64_t r = rows[i];
+  for (int j = 0; j < count; ++j)
+    acc += ld_table(wu + e + j * stride) * uf[ld_table(src + e + j * stride)];
+  const int64_t r = ld_table(rows + i);
   gu[r] = acc;
   const int64_t face = r / N2;  // block * 6 + face
   const T wc = (face % 2) ? tb.W[P1 - 1][P1 + 1] : tb.W[0][0];
-  fg[r] = (pf[r] + wc * acc) * (T(1) / (T)spacing[face / 6]);
+  fg[r] = (pf[r] + wc * acc) * (T(1) / (T)ld_table(spacing + face / 6));
 }
 
 // Then the ghost gradient of every row: the weighted sources' face gradients.
@@ -682,15 +684,17 @@ __global__ void k_face_ghost_g(int64_t ntot, FaceGroups G, const int32_t* __rest
   if (!face_group_row(G, i, e, stride, count)) return;
   T acc = 0;
 #pragma unroll 8
-  for (int j = 0; j < count; ++j) acc += wg[e + j * stride] * fg[src[e + j * stride]];
-  gg[rows[i]] = acc;
+  for (int j = 0; j < count; ++j)
+    acc += ld_table(wg + e + j * stride) * fg[ld_table(src + e + j * stride)];
+  gg[ld_table(rows + i)] = acc;
 }
 
 // The face buffers of a state (before the first fused stage): per block face,
 // the face plane of u and the block-local part of the face gradient.
 template <class T, int P1, int E>
-__global__ void k_faces3(int64_t nb, const T* __restrict__ y, const Tables<T>* __restrict__ tb,
-                         T* __restrict__ uf, T* __restrict__ pf) {
+__global__ void k_faces3(int64_t nb, const T* __restrict__ y, T* __restrict__ uf,
+                         T* __restrict__ pf) {
+  constexpr FusedTab<T, P1> tb = fused_tab<T, P1>();  // as the stage kernel's
   constexpr int N = P1 * E, P = P1 - 1;
   const int64_t N2 = (int64_t)N * N, nn = N2 * N;
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -710,7 +714,7 @@ __global__ void k_faces3(int64_t nb, const T* __restrict__ y, const Tables<T>* _
   const int row = s ? P : 0;
   T acc = 0;
 #pragma unroll
-  for (int j = 0; j < P1; ++j) acc += tb->W[row][j + 1] * u[base + (first + j) * stride];
+  for (int j = 0; j < P1; ++j) acc += tb.W[row][j + 1] * u[base + (first + j) * stride];
   uf[g] = u[base + (s ? N - 1 : 0) * stride];
   pf[g] = acc;
 }

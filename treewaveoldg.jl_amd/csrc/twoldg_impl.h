@@ -16,8 +16,20 @@
 
 namespace twoldg_detail {
 int fail(int code, const std::string& msg);
-}
+// The library's device memory (host = false) and fine-grained host staging
+// (host = true).  Freed blocks are kept, per device and size class, and handed
+// out again: a virtual address range the library has used stays mapped to the
+// same memory for the life of the process.  Giving ranges back to the driver
+// and having new allocations map them to other pages let workgroups of a new
+// problem's first stages read old contents of some pages (wrong values in a
+// few blocks of the first RK4 step after a problem was created, never in a
+// repeat of that step on the same problem).
+hipError_t pool_alloc(void** ptr, size_t bytes, bool host);
+void pool_free(void* ptr);
+}  // namespace twoldg_detail
 using twoldg_detail::fail;
+using twoldg_detail::pool_alloc;
+using twoldg_detail::pool_free;
 
 namespace {
 
@@ -127,12 +139,16 @@ __global__ void k_exchange(Shape sh, ExchangeRows X, const T* __restrict__ coef,
 
 // Dirichlet ghosts of a plane wave sin(k.x - w t + phi): u (MODE 0) or the
 // Cartesian gradient k cos(...) (MODE 2).  rows: u-shaped ghost rows, whose
-// exchange rows are empty.
+// exchange rows are empty.  The time is read from the problem's device clock
+// (t0, s): t = (t0 + s dt) + cdt, cdt the stage's offset, so that a captured
+// RK4 step replays at every step's own time.
 struct DirichletRows {
   int64_t nrows;
   const int64_t* rows;
   const double* x;  // (nrows, D) node positions
   double k0, k1, k2, w, phi;
+  const double* clock;  // device: (t0, s)
+  double dt;
 };
 template <class T, int MODE, int D>
 __device__ __forceinline__ void dirichlet_row(int64_t q, const Shape& sh, const DirichletRows& R,
@@ -157,13 +173,34 @@ __device__ __forceinline__ void dirichlet_row(int64_t q, const Shape& sh, const 
 // The exchange rows and the Dirichlet rows (disjoint ghosts) in one launch.
 template <class T, int MODE, int D>
 __global__ void k_exchange_dirichlet(Shape sh, ExchangeRows X, const T* __restrict__ coef,
-                                     const T* __restrict__ src, DirichletRows R, double t,
+                                     const T* __restrict__ src, DirichletRows R, double cdt,
                                      T* __restrict__ ghost) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r < X.nrows)
+  if (r < X.nrows) {
     exchange_row<T, MODE, D>(r, sh, X, coef, src, ghost);
-  else if (r - X.nrows < R.nrows)
+  } else if (r - X.nrows < R.nrows) {
+    // rounded as the host's t0 + s dt, then + cdt (no fused multiply-add)
+    const double t = __dadd_rn(__dadd_rn(R.clock[0], __dmul_rn(R.clock[1], R.dt)), cdt);
     dirichlet_row<T, MODE, D>(r - X.nrows, sh, R, t, ghost);
+  }
+}
+
+// The device clock of a problem: (t0, step count).
+__global__ void k_set_clock(double* __restrict__ clock, double t0) {
+  clock[0] = t0;
+  clock[1] = 0.0;
+}
+__global__ void k_tick(double* __restrict__ clock) { clock[1] += 1.0; }
+
+// A load of a table the host wrote when the problem was built (neighbour codes,
+// spacings, exchange rows and weights): relaxed, at agent scope, so that it
+// observes the value last written at device level whatever a cache of the
+// workgroup's XCD still holds for that address.  The tables live in recycled
+// device memory; plain loads of them have been seen to return the previous
+// occupant's contents in a few workgroups of a new problem's first stages.
+template <class T>
+__device__ __forceinline__ T ld_table(const T* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // The wide derivative of `read` along d at owned node I, in units of 1/h
@@ -736,6 +773,8 @@ struct twoldg_problem {
   int64_t* dir_rows = nullptr;
   double* dir_x = nullptr;
   double wave_k[3] = {0, 0, 0}, wave_w = 0, wave_phi = 0;
+  double* clock = nullptr;  // device (t0, s): the Dirichlet data's time is t0 + s dt
+  double clock_dt = 0;      // the dt of the clock (a kernel argument)
   double* part = nullptr;   // per-block reduction partials (3 nb)
   double* red = nullptr;    // final reduction (4)
   double hmin = 0, hspeed = 0;
@@ -748,6 +787,7 @@ struct twoldg_problem {
   double* pinned = nullptr;
   // fused one-launch stages (3D, faces all same-level or mirror, flat metric)
   bool fused = false;
+  bool warming = false;     // the throwaway step at creation (fused forms)
   bool no_fused = false;    // the caller asked for the generic stages
   int32_t* nbr6 = nullptr;  // (nb, 6)
   void* uf[2] = {nullptr, nullptr};
@@ -780,7 +820,7 @@ template <class T>
 int dalloc(twoldg_problem* p, T** ptr, size_t count) {
   *ptr = nullptr;
   if (count == 0) return 0;
-  hipError_t e = hipMalloc((void**)ptr, count * sizeof(T));
+  hipError_t e = pool_alloc((void**)ptr, count * sizeof(T), false);
   if (e != hipSuccess)
     return fail(TWOLDG_ENOMEM, std::string("hipMalloc of ") + std::to_string(count * sizeof(T)) +
                                    " bytes: " + hipGetErrorString(e));
@@ -791,7 +831,7 @@ int dalloc(twoldg_problem* p, T** ptr, size_t count) {
 int dalloc_bytes(twoldg_problem* p, void** ptr, size_t bytes) {
   *ptr = nullptr;
   if (bytes == 0) return 0;
-  hipError_t e = hipMalloc(ptr, bytes);
+  hipError_t e = pool_alloc(ptr, bytes, false);
   if (e != hipSuccess)
     return fail(TWOLDG_ENOMEM, std::string("hipMalloc of ") + std::to_string(bytes) +
                                    " bytes: " + hipGetErrorString(e));
@@ -810,7 +850,7 @@ int dalloc_bytes(twoldg_problem* p, void** ptr, size_t bytes) {
 // last problem's; a rebuilt problem, uploading the same data to the same pages,
 // read right).
 inline hipError_t host_stage_alloc(void** ptr, size_t bytes) {
-  return hipHostMalloc(ptr, bytes, hipHostMallocCoherent);
+  return pool_alloc(ptr, bytes, true);
 }
 
 // Host data into device memory: staged in fine-grained host memory and written
@@ -831,13 +871,13 @@ int h2d(twoldg_problem* p, void* dst, const void* src, size_t bytes) {
   hipLaunchKernelGGL(k_copy_words, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, p->stream,
                      (int64_t)words, (const uint32_t*)pinned, (uint32_t*)dst);
   hipError_t e = hipStreamSynchronize(p->stream);
-  (void)hipHostFree(pinned);
+  pool_free(pinned);
   if (e != hipSuccess) return fail(TWOLDG_EDEVICE, std::string("table upload: ") + hipGetErrorString(e));
   return 0;
 }
 
 void dfree(void* ptr) {
-  if (ptr) (void)hipFree(ptr);
+  if (ptr) pool_free(ptr);
 }
 
 inline unsigned grid_for(int64_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
@@ -1022,45 +1062,54 @@ inline void tmark(twoldg_problem* p) {
   (void)hipEventRecord(p->tev[p->ntev++], p->stream);
 }
 
+// The problem's clock at t0 (step count 0), for steps of dt.
+inline void set_clock(twoldg_problem* p, double t0, double dt) {
+  p->clock_dt = dt;
+  hipLaunchKernelGGL(k_set_clock, dim3(1), dim3(1), 0, p->stream, p->clock, t0);
+}
+
 // ---- launch helpers, dispatched on (D, P1) at run time ----------------------
 template <class T, int D, int P1>
 struct Launch {
   static ExchangeRows rows(twoldg_problem* p) { return {p->nrows, p->rowptr, p->cb, p->cl}; }
   static DirichletRows dir(twoldg_problem* p) {
-    return {p->ndir, p->dir_rows, p->dir_x, p->wave_k[0], p->wave_k[1], p->wave_k[2],
-            p->wave_w, p->wave_phi};
+    return {p->ndir,     p->dir_rows,  p->dir_x,     p->wave_k[0], p->wave_k[1],
+            p->wave_k[2], p->wave_w,   p->wave_phi,  p->clock,     p->clock_dt};
   }
-  // the ghosts of src: exchange rows, and the Dirichlet rows at time t
+  // the ghosts of src: exchange rows, and the Dirichlet rows at the clock's
+  // time plus cdt
   template <int MODE>
-  static void ghosts(twoldg_problem* p, const T* coef, const T* src, double t, T* ghost) {
+  static void ghosts(twoldg_problem* p, const T* coef, const T* src, double cdt, T* ghost) {
     if (p->nrows == 0) return;
     if constexpr (MODE != 1) {  // Dirichlet data come with the Cartesian gradients
       if (p->ndir) {
         hipLaunchKernelGGL((k_exchange_dirichlet<T, MODE, D>),
                            dim3(grid_for(p->nrows + p->ndir, 256)), dim3(256), 0, p->stream,
-                           p->sh, rows(p), coef, src, dir(p), t, ghost);
+                           p->sh, rows(p), coef, src, dir(p), cdt, ghost);
         return;
       }
     }
     hipLaunchKernelGGL((k_exchange<T, MODE, D>), dim3(grid_for(p->nrows, 256)), dim3(256), 0,
                        p->stream, p->sh, rows(p), coef, src, ghost);
   }
-  static void exchange_u(twoldg_problem* p, const T* y, double t) {
-    ghosts<0>(p, (const T*)p->cval_u, y, t, (T*)p->ug);
+  static void exchange_u(twoldg_problem* p, const T* y, double cdt) {
+    ghosts<0>(p, (const T*)p->cval_u, y, cdt, (T*)p->ug);
   }
-  static void gradient(twoldg_problem* p, const T* y, double t) {
-    exchange_u(p, y, t);
+  // the gradient (flat, or Cartesian on a full metric) of y and its ghosts, the
+  // Dirichlet data at the clock's time plus cdt
+  static void gradient(twoldg_problem* p, const T* y, double cdt) {
+    exchange_u(p, y, cdt);
     const int64_t n = p->sh.nb * p->sh.nn;
     if (!p->full) {
       hipLaunchKernelGGL((k_gradient<T, D, P1>), dim3(grid_for(n, 256)), dim3(256), 0, p->stream,
                          p->sh, (const Tables<T>*)p->tables, p->spacing, y, (const T*)p->ug,
                          (T*)p->grads);
-      ghosts<1>(p, (const T*)p->cval_g, (const T*)p->grads, t, (T*)p->gg);
+      ghosts<1>(p, (const T*)p->cval_g, (const T*)p->grads, cdt, (T*)p->gg);
     } else {
       hipLaunchKernelGGL((k_gradient_cart<T, D, P1>), dim3(grid_for(n, 256)), dim3(256), 0,
                          p->stream, p->sh, (const Tables<T>*)p->tables, p->spacing, y,
                          (const T*)p->ug, (const T*)p->coef, p->flat, (T*)p->grads);
-      ghosts<2>(p, (const T*)p->cval_u, (const T*)p->grads, t, (T*)p->gg);
+      ghosts<2>(p, (const T*)p->cval_u, (const T*)p->grads, cdt, (T*)p->gg);
     }
   }
   template <int MODE>
@@ -1129,11 +1178,12 @@ int fused_steps(twoldg_problem* p, double dt, int64_t nsteps) {
   constexpr int N = P1 * E;
   constexpr int threads = (N * N + 63) / 64 * 64;
   const int64_t nb = p->sh.nb;
-  T* yn = (T*)p->slot[TWOLDG_SLOT_STATE];
+  // the warm-up at creation steps the stage buffer in place of the state
+  T* yn = (T*)(p->warming ? p->ys : p->slot[TWOLDG_SLOT_STATE]);
   {
     const int64_t n = nb * 6 * (int64_t)N * N;
     hipLaunchKernelGGL((k_faces3<T, P1, E>), dim3(grid_for(n, 256)), dim3(256), 0, p->stream, nb,
-                       yn, (const Tables<T>*)p->tables, (T*)p->uf[0], (T*)p->pf[0]);
+                       yn, (T*)p->uf[0], (T*)p->pf[0]);
   }
   FusedArgs<T> A{};
   A.yn = yn;
@@ -1199,7 +1249,7 @@ int fused_steps(twoldg_problem* p, double dt, int64_t nsteps) {
       hipLaunchKernelGGL((k_fused3<T, P1, E, 3, GHOST>), dim3((unsigned)nb), dim3(threads), 0, p->stream, A);
     tmark(p);
   };
-  if (p->timing || nsteps < 2) {
+  if (p->timing || p->warming || nsteps < 2) {
     for (int64_t s = 0; s < nsteps; ++s) one_step();
   } else {
     // the step's launches replayed as one graph: they are the same every step
@@ -1253,23 +1303,23 @@ int rk4_steps(twoldg_problem* p, double t0, double dt, int64_t nsteps) {
   T* ys = (T*)p->ys;
   T* acc = (T*)p->acc;
   T* lo = (T*)p->lo;
-  for (int64_t s = 0; s < nsteps; ++s) {
-    const double t = t0 + (double)s * dt;
+  // one step at the clock's time t0 + s dt, then the clock's tick
+  auto one_step = [&]() {
     StageIO<T> io{yn, yn, acc, ys, lo, (T)(dt / 2)};
-    L::gradient(p, yn, t);
+    L::gradient(p, yn, 0.0);
     L::template stage<1>(p, io);
     L::truncate(p, ys);
     io.ys = ys;
-    L::gradient(p, ys, t + dt / 2);
+    L::gradient(p, ys, dt / 2);
     L::template stage<2>(p, io);
     L::truncate(p, ys);
     io.a = (T)dt;
-    L::gradient(p, ys, t + dt / 2);
+    L::gradient(p, ys, dt / 2);
     L::template stage<2>(p, io);
     L::truncate(p, ys);
     io.a = (T)(dt / 6);
     io.yo = yn;
-    L::gradient(p, ys, t + dt);
+    L::gradient(p, ys, dt);
     if (p->precision == TWOLDG_F32C) {
       L::template stage<4>(p, io);
       L::truncate(p, yn);
@@ -1278,6 +1328,26 @@ int rk4_steps(twoldg_problem* p, double t0, double dt, int64_t nsteps) {
       L::template stage<3>(p, io);
       L::truncate(p, yn);
     }
+    hipLaunchKernelGGL(k_tick, dim3(1), dim3(1), 0, p->stream, p->clock);
+  };
+  set_clock(p, t0, dt);
+  if (p->timing || nsteps < 2) {
+    for (int64_t s = 0; s < nsteps; ++s) one_step();
+  } else {
+    // the step's launches replayed as one graph; the Dirichlet data read the
+    // step's time from the device clock, which the step's last launch advances
+    if (!p->step_exec || p->step_dt != dt) {
+      if (p->step_exec) (void)hipGraphExecDestroy(p->step_exec);
+      if (p->step_graph) (void)hipGraphDestroy(p->step_graph);
+      p->step_exec = nullptr;
+      p->step_graph = nullptr;
+      HIP_TRY(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
+      one_step();
+      HIP_TRY(hipStreamEndCapture(p->stream, &p->step_graph));
+      HIP_TRY(hipGraphInstantiate(&p->step_exec, p->step_graph, nullptr, nullptr, 0));
+      p->step_dt = dt;
+    }
+    for (int64_t s = 0; s < nsteps; ++s) HIP_TRY(hipGraphLaunch(p->step_exec, p->stream));
   }
   HIP_TRY(hipGetLastError());
   return 0;
@@ -1287,7 +1357,8 @@ template <class T, int D, int P1>
 int rhs_impl(twoldg_problem* p, int src, int dst, double t) {
   using L = Launch<T, D, P1>;
   const T* y = (const T*)p->slot[src];
-  L::gradient(p, y, t);
+  set_clock(p, t, 0.0);
+  L::gradient(p, y, 0.0);
   StageIO<T> io{y, y, nullptr, (T*)p->slot[dst], nullptr, (T)0};
   L::template stage<0>(p, io);
   HIP_TRY(hipGetLastError());
@@ -1307,7 +1378,8 @@ template <class T, int D, int P1>
 int energy_impl(twoldg_problem* p, int slot, double t, double* out) {
   using L = Launch<T, D, P1>;
   const T* y = (const T*)p->slot[slot];
-  L::gradient(p, y, t);
+  set_clock(p, t, 0.0);
+  L::gradient(p, y, 0.0);
   L::energy(p, y);
   double h[2];
   HIP_TRY(hipMemcpyAsync(h, p->red, 2 * sizeof(double), hipMemcpyDeviceToHost, p->stream));

@@ -25,6 +25,7 @@ fallback.
 """
 import ctypes
 import math
+import weakref
 
 import numpy as np
 
@@ -118,7 +119,7 @@ class WaveProblem:
         _lib.check(_lib.lib().twoldg_problem_create(ctypes.byref(desc), ctypes.byref(handle)))
         self._h = handle
         self.state_len = self.nblocks * 2 * self.nn
-        self._state_owner = None  # the integrator whose state the state slot holds
+        self._state_owner = None  # weak reference to the integrator holding the state slot
 
     def close(self):
         if getattr(self, "_h", None):
@@ -342,10 +343,13 @@ class Integrator:
         self.dt = float(dt)
         self.istep = 0
         p.upload(u0, _lib.SLOT_STATE)
-        p._state_owner = self
+        # weak: the problem must not keep its integrator alive (a cycle would
+        # leave both, and the device memory, to the cyclic garbage collector)
+        p._state_owner = weakref.ref(self)
 
     def _own(self):
-        if self.p._state_owner is not self:
+        owner = self.p._state_owner
+        if owner is None or owner() is not self:
             raise RuntimeError(
                 "another integrator was built on this problem since, and holds its state "
                 "slot: build one problem per concurrent integrator")
