@@ -282,6 +282,30 @@ def ball(steps=20):
     return out
 
 
+def annulus(s=8):
+    """The four-patch annulus (diagonal metric, identity glues, walls), P=3, E=2,
+    roots (2s, s) per patch (512 blocks at s=8), the rotating mode m=2, n=1 to a
+    quarter turn; FP64 and FP32c, l2 error and throughput over the run."""
+    import twoldg as T
+    out = {}
+    for prec in ("f64", "f32c"):
+        t0 = time.perf_counter()
+        r = T.annulus_errors(3, 2, roots=(2 * s, s), m=2, precision=prec)
+        el = time.perf_counter() - t0
+        prob = r["problem"]
+        nodes = prob.nblocks * prob.nn
+        u0 = T.initial_state(prob, T.annulus_exact(prob.geometry, prob.forest, 2, r["k"], 0.0))
+        wall, _, _, _ = timed_integrator(T, prob, u0, r["dt"], 50)
+        out[prec] = dict(l2=r["l2"], l2u=r["l2u"], l2v=r["l2v"], nsteps=r["nsteps"],
+                         energy_loss=1 - r["energy"] / r["energy0"],
+                         value=nodes * 50 / wall, driver_wall_s=el)
+        log(f"[bench] annulus s={s} {prec}: {prob.nblocks} blocks, l2 u {r['l2u']:.3e} "
+            f"v {r['l2v']:.3e} ({r['nsteps']} steps), 1 - E1/E0 {1 - r['energy'] / r['energy0']:.2e}, "
+            f"{nodes * 50 / wall:.3e} node-updates/s")
+        prob.close()
+    return out
+
+
 # The side workloads by name, each at the size the default run reports.
 WORKLOADS = {
     "periodic2d": lambda: periodic2d(),
@@ -289,6 +313,7 @@ WORKLOADS = {
     "nested-cubes-large": lambda: nested_cubes(large=True, steps=10),
     "batch2d": lambda: batch2d(),
     "ball": lambda: ball(),
+    "annulus": lambda: annulus(),
 }
 
 

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define TWOLDG_ABI_VERSION 2
+#define TWOLDG_ABI_VERSION 3
 
 /* status codes */
 #define TWOLDG_OK 0
@@ -95,6 +95,12 @@ typedef struct twoldg_desc {
   double hspeed;                /* min over blocks of h_b / max logical speed (0: flat) */
   int32_t flags;                /* TWOLDG_FLAG_* */
   int32_t pad;
+  /* diagonal metric (a Geometry such as the annulus, evolution.jl:67-70); NULL when
+     flat or full.  The gradient sets hold the fluxes F_d = A_d D_d u, the
+     Laplacian divides by sqrt(g), energies and norms weigh by sqrt(g). */
+  const double* dcoef;          /* (nblocks, 1 + D, N^D): sqrt(g), A_1 .. A_D */
+  const double* lmode;          /* (nblocks, D, E, P+1): each line's rank-one truncation */
+  const double* ldual;          /*   pair (MetricTruncation, evolution.jl:556-575) */
 } twoldg_desc;
 
 /* flags */
@@ -150,7 +156,8 @@ int twoldg_norms(twoldg_problem* p, int a, int b, int vars_mask, int pnorm, doub
    problems' streams) race and must be serialized by the caller. */
 int twoldg_norms_noalloc(twoldg_problem* p, int a, int b, int vars_mask, int pnorm, double* lp,
                          double* linf);
-/* cfl * h / (P+1): oldg_timestep, src/evolution.jl:1528-1531 */
+/* cfl * h / (P+1), on a metric (full or diagonal) h = min over blocks of h_b
+   over the block's largest logical speed: oldg_timestep, src/evolution.jl:1528-1531 */
 int twoldg_timestep(const twoldg_problem* p, double cfl, double* out);
 /* wait for the problem's stream */
 int twoldg_synchronize(twoldg_problem* p);

@@ -270,7 +270,7 @@ struct Face {
 struct Problem {
   int D = 2, N = 0, P = 0, E = 0, P1 = 0;
   Element el;
-  int geometry = 0;  // 0 brick, 1 seven-patch ball
+  int geometry = 0;  // 0 brick, 1 seven-patch ball, 2 annulus (diagonal metric)
   Ball ball;
   int npatch = 1;
   int proots[7][3];
@@ -294,6 +294,11 @@ struct Problem {
   double hspeed = 0;
   // Dirichlet plane wave
   double kvec[3] = {0, 0, 0}, omega = 0, phase = 0;
+  // annulus (geometry 2): r0 <= r <= r1, four patches of (n1, n2) roots
+  double an_r0 = 1, an_r1 = 2, an_dth = 0, an_dr = 0;
+  // diagonal metric: (nb, 1 + D, nn) sqrt(g), A_1..A_D; per-line truncation
+  // pairs (nb, D, E, P1)
+  std::vector<double> dcoef, lmode, ldual;
   // work arrays
   std::vector<double> ufs;            // (nb, (N+2)^D)
   std::vector<double> grads[3];       // (nb, prod(N + 2 delta_d))
@@ -490,6 +495,14 @@ static void node_physical(const Problem& pr, int64_t b, const int64_t I[3], doub
     for (int d = 0; d < 3; ++d) x[d] = xl[d];
     return;
   }
+  if (pr.geometry == 2) {  // annulus_radius, annulus_angle (geometry.jl:211-212)
+    const double r = pr.an_r0 + pr.an_dr * xl[1];
+    const double th = pr.leaves[b].patch * (M_PI / 2) + pr.an_dth * xl[0];
+    x[0] = r * std::cos(th);
+    x[1] = r * std::sin(th);
+    x[2] = 0;
+    return;
+  }
   ball_position<double>(pr.ball, pr.leaves[b].patch, xl, x);
 }
 
@@ -522,7 +535,17 @@ static void resolve_faces(Problem& pr) {
         Face& F = pr.faces[(b * D + d) * 2 + s];
         int64_t c[3] = {L.c[0], L.c[1], L.c[2]};
         c[d] += s ? 1 : -1;
-        if (c[d] < 0 || c[d] >= n[d]) {
+        int np = L.patch;
+        if (pr.geometry == 2 && (c[d] < 0 || c[d] >= n[d])) {
+          // the annulus: the angular faces glued by the identity to the next
+          // (previous) patch, the radial faces reflecting walls
+          if (d == 1) {
+            F.kind = G_MIRROR;
+            continue;
+          }
+          np = (L.patch + (s ? 1 : 3)) % 4;
+          c[d] = s ? 0 : n[d] - 1;
+        } else if (c[d] < 0 || c[d] >= n[d]) {
           if (pr.geometry == 0) {
             if (pr.periodic[d]) c[d] = (c[d] + n[d]) % n[d];
             else {
@@ -535,7 +558,7 @@ static void resolve_faces(Problem& pr) {
           }
         }
         int lev;
-        int64_t nb = covering_leaf(pr, L.patch, L.level, c, &lev);
+        int64_t nb = covering_leaf(pr, np, L.level, c, &lev);
         if (nb >= 0 && lev == L.level) {
           F.kind = G_SAME;
           F.nb = nb;
@@ -884,7 +907,9 @@ static void gradient_flat(Problem& pr) {
         for (int e = 0; e < D; ++e) gn[e] = N + (e == d ? 2 : 0);
         int64_t G[3] = {I[0], I[1], I[2]};
         G[d] += 1;
-        pr.grads[d][b * pr.grad_n[d] + hidx(pr, G, gn)] = acc / h;
+        double g = acc / h;
+        if (pr.geometry == 2) g = pr.dcoef[(b * (1 + D) + 1 + d) * nn + l] * g;  // flux A_d x
+        pr.grads[d][b * pr.grad_n[d] + hidx(pr, G, gn)] = g;
       }
     }
   }
@@ -943,7 +968,9 @@ static void laplacian_flat(Problem& pr, const double* y, double* du) {
         acc += term;
       }
       du[b * 2 * nn + l] = y[b * 2 * nn + nn + l];
-      du[b * 2 * nn + nn + l] = acc / h;
+      double lap = acc / h;
+      if (pr.geometry == 2) lap = lap / pr.dcoef[(b * (1 + D)) * nn + l];  // divergence / sqrt(g)
+      du[b * 2 * nn + nn + l] = lap;
     }
   }
 }
@@ -1085,7 +1112,7 @@ static void laplacian_cart(Problem& pr, const double* y, double* du) {
 
 static void fill_gradient(Problem& pr, const double* y, double t) {
   scatter_exchange_u(pr, y, t);
-  if (pr.geometry == 0) {
+  if (pr.geometry != 1) {
     gradient_flat(pr);
     exchange_grads(pr);
   } else {
@@ -1096,7 +1123,7 @@ static void fill_gradient(Problem& pr, const double* y, double t) {
 
 static void rhs(Problem& pr, const double* y, double* du, double t) {
   fill_gradient(pr, y, t);
-  if (pr.geometry == 0) laplacian_flat(pr, y, du);
+  if (pr.geometry != 1) laplacian_flat(pr, y, du);
   else laplacian_cart(pr, y, du);
 }
 
@@ -1126,6 +1153,36 @@ static void truncate_tensor_elem(const Problem& pr, double* a, const int64_t bas
       }
     (void)K1;
     (void)K2;
+  }
+}
+
+// The tensor projector with each line's own rank-one pair (MetricTruncation,
+// evolution.jl:556-575): line along d of element J of block b.
+static void truncate_metric_elem(const Problem& pr, double* a, const int64_t base[3], int64_t b,
+                                 const int64_t J[3]) {
+  const int D = pr.D, P1 = pr.P1, N = pr.N, E = pr.E;
+  for (int d = 0; d < D; ++d) {
+    const double* md = &pr.lmode[((b * D + d) * E + J[d]) * P1];
+    const double* dd = &pr.ldual[((b * D + d) * E + J[d]) * P1];
+    int t[2];
+    int nt = tangential(D, d, t);
+    for (int64_t k2 = 0; k2 < (nt >= 2 ? P1 : 1); ++k2)
+      for (int64_t k1 = 0; k1 < (nt >= 1 ? P1 : 1); ++k1) {
+        int64_t K[2] = {k1, k2};
+        double c = 0;
+        for (int j = 0; j < P1; ++j) {
+          int64_t I[3] = {base[0], base[1], base[2]};
+          I[d] += j;
+          for (int q = 0; q < nt; ++q) I[t[q]] += K[q];
+          c += dd[j] * a[I[0] + N * (I[1] + N * I[2])];
+        }
+        for (int i = 0; i < P1; ++i) {
+          int64_t I[3] = {base[0], base[1], base[2]};
+          I[d] += i;
+          for (int q = 0; q < nt; ++q) I[t[q]] += K[q];
+          a[I[0] + N * (I[1] + N * I[2])] -= md[i] * c;
+        }
+      }
   }
 }
 
@@ -1175,6 +1232,7 @@ static void truncate_state(Problem& pr, double* y) {
       int64_t base[3] = {0, 0, 0};
       for (int d = 0; d < D; ++d) base[d] = J[d] * P1;
       if (pr.geometry == 1 && pr.slot[b] >= 0) truncate_element_chol(pr, a, base, pr.slot[b], e);
+      else if (pr.geometry == 2) truncate_metric_elem(pr, a, base, b, J);
       else truncate_tensor_elem(pr, a, base);
     }
   }
@@ -1199,15 +1257,18 @@ static double energy(Problem& pr, const double* y, double t) {
       int64_t I[3] = {l % N, (l / N) % N, l / ((int64_t)N * N)};
       double w = node_weight(pr, I, pr.h[b]);
       double v = y[b * 2 * nn + nn + l];
-      if (pr.geometry == 0) {
-        k += w * (v * v);
+      if (pr.geometry != 1) {
+        // flat: w v^2 + w g_d^2; diagonal metric: w sqrt(g) v^2 + w F_d^2 / A_d
+        // (evolution.jl:1483-1487, layout.jl:135-161)
+        const double* dc = pr.geometry == 2 ? &pr.dcoef[(b * (1 + D)) * nn] : nullptr;
+        k += (dc ? w * dc[l] : w) * (v * v);
         for (int d = 0; d < D; ++d) {
           int64_t gn[3] = {1, 1, 1};
           for (int e = 0; e < D; ++e) gn[e] = N + (e == d ? 2 : 0);
           int64_t G[3] = {I[0], I[1], I[2]};
           G[d] += 1;
           double g = pr.grads[d][b * pr.grad_n[d] + hidx(pr, G, gn)];
-          p[d] += w * (g * g);
+          p[d] += (dc ? w * (1.0 / dc[(1 + d) * nn + l]) : w) * (g * g);
         }
       } else {
         double c = coefv(pr, b, 0, l);
@@ -1247,6 +1308,7 @@ static void norms(const Problem& pr, const double* err, int vars_mask, int p, do
       int64_t I[3] = {l % N, (l / N) % N, l / ((int64_t)N * N)};
       double w = node_weight(pr, I, pr.h[b]);
       if (pr.geometry == 1) w *= coefv(pr, b, 0, l);
+      if (pr.geometry == 2) w *= pr.dcoef[(b * (1 + pr.D)) * nn + l];
       double s = 0;
       int nv = 0;
       for (int v = 0; v < 2; ++v)
@@ -1280,7 +1342,7 @@ static void allocate_work(Problem& pr) {
   pr.ufs.assign(pr.nb * pr.ufs_n, 0.0);
   for (int d = 0; d < D; ++d) {
     pr.grad_n[d] = ipow(N, D - 1) * (N + 2);
-    if (pr.geometry == 0) pr.grads[d].assign(pr.nb * pr.grad_n[d], 0.0);
+    if (pr.geometry != 1) pr.grads[d].assign(pr.nb * pr.grad_n[d], 0.0);
   }
   if (pr.geometry == 1) pr.cart.assign(pr.nb * D * pr.ufs_n, 0.0);
 }
@@ -1447,6 +1509,89 @@ static void build_ball_metric(Problem& pr) {
 
 }  // namespace
 
+// --- annulus: diagonal metric and per-line truncation (geometry.jl:176-268,
+// evolution.jl:576-622, 698-708) ----------------------------------------------
+// The degree-P polynomial at the element's nodes orthogonal in sum w_i a_i f_i g_i
+// to every lower one: L_P less its projection onto L_0..L_{P-1}, by the normal
+// equations solved in binary128.
+static void projector_pair(const Problem& pr, const qf* a, double* mode, double* dual) {
+  const int P = pr.P, P1 = pr.P1;
+  qf L[9][9];  // L[i][k] = L_k(xi_i)
+  for (int i = 0; i < P1; ++i) legendre_q(P, (qf)pr.el.nodes[i], L[i]);
+  auto inner = [&](int k, int l) {
+    qf s = 0;
+    for (int i = 0; i < P1; ++i) s += (qf)pr.el.weights[i] * a[i] * L[i][k] * L[i][l];
+    return s;
+  };
+  qf G[8][9];
+  for (int k = 0; k < P; ++k) {
+    for (int l = 0; l < P; ++l) G[k][l] = inner(k, l);
+    G[k][P] = inner(P, k);
+  }
+  for (int c = 0; c < P; ++c)  // Gauss-Jordan, the Gram matrix is positive definite
+    for (int r = 0; r < P; ++r) {
+      if (r == c) continue;
+      const qf f = G[r][c] / G[c][c];
+      for (int q = c; q <= P; ++q) G[r][q] -= f * G[c][q];
+    }
+  qf phi[9], gam = 0;
+  for (int i = 0; i < P1; ++i) {
+    phi[i] = L[i][P];
+    for (int k = 0; k < P; ++k) phi[i] -= (G[k][P] / G[k][k]) * L[i][k];
+    gam += (qf)pr.el.weights[i] * a[i] * phi[i] * phi[i];
+  }
+  for (int i = 0; i < P1; ++i) {
+    mode[i] = (double)phi[i];
+    dual[i] = (double)((qf)pr.el.weights[i] * a[i] * phi[i] / gam);
+  }
+}
+
+static void build_annulus_metric(Problem& pr) {
+  const int D = 2, N = pr.N, P1 = pr.P1, E = pr.E;
+  const int64_t nn = pr.nn;
+  pr.dcoef.assign(pr.nb * (1 + D) * nn, 0.0);
+  pr.lmode.assign(pr.nb * D * E * P1, 0.0);
+  pr.ldual.assign(pr.nb * D * E * P1, 0.0);
+  double smax = 0;
+  pr.hspeed = INFINITY;
+  for (int64_t b = 0; b < pr.nb; ++b) {
+    double sb = 0;
+    for (int64_t l = 0; l < nn; ++l) {
+      int64_t I[3] = {l % N, l / N, 0};
+      double x[3];
+      node_logical(pr, b, I, x);
+      // geometry_terms: sqrt(g) = r th' r', A_1 = r'/(r th'), A_2 = r th'/r'
+      const double r = pr.an_r0 + pr.an_dr * x[1];
+      const double J = r * pr.an_dth * pr.an_dr;
+      const double A1 = pr.an_dr / (r * pr.an_dth), A2 = r * pr.an_dth / pr.an_dr;
+      double* c = &pr.dcoef[b * (1 + D) * nn];
+      c[l] = J;
+      c[nn + l] = A1;
+      c[2 * nn + l] = A2;
+      sb = std::max(sb, std::max(std::sqrt(A1 / J), std::sqrt(A2 / J)));
+    }
+    smax = std::max(smax, sb);
+    pr.hspeed = std::min(pr.hspeed, pr.h[b] / sb);
+    // the lines' pairs, from the factors of sqrt(g) along each dimension:
+    // th' along d = 0, r(x_2) r' along d = 1 (jacobian_factor, geometry.jl:220-222)
+    const Leaf& Lf = pr.leaves[b];
+    for (int d = 0; d < D; ++d)
+      for (int e = 0; e < E; ++e) {
+        qf a[9];
+        for (int i = 0; i < P1; ++i) {
+          const qf xi = (qf)pr.plo[Lf.patch][d] +
+                        (qf)pr.h[b] * ((qf)(Lf.c[d] * N + (int64_t)e * P1) + (qf)pr.el.offsets[i]);
+          const qf th = (qf)M_PI / 2 / (qf)pr.proots[Lf.patch][0];
+          const qf dr = ((qf)pr.an_r1 - (qf)pr.an_r0) / (qf)pr.proots[Lf.patch][1];
+          a[i] = d == 0 ? th : ((qf)pr.an_r0 + dr * xi) * dr;
+        }
+        projector_pair(pr, a, &pr.lmode[((b * D + d) * E + e) * P1],
+                       &pr.ldual[((b * D + d) * E + e) * P1]);
+      }
+  }
+  (void)smax;
+}
+
 // ===========================================================================
 // C API (ctypes).
 // ===========================================================================
@@ -1557,6 +1702,53 @@ void* orc_ball(int P, int E, int n, int m, double Lc, double R1, double grading,
   return pr;
 }
 
+// The four-patch annulus r0 <= r <= r1, each patch (n1, n2) roots, the angular
+// faces glued by the identity and the radial ones reflecting (annulus_forest,
+// geometry.jl:240-268).
+void* orc_annulus(int P, int E, int n1, int n2, double r0, double r1, int truncate) {
+  auto* pr = new Problem();
+  pr->D = 2;
+  pr->P = P;
+  pr->P1 = P + 1;
+  pr->E = E;
+  pr->N = E * (P + 1);
+  build_element(P, pr->el);
+  pr->geometry = 2;
+  pr->truncate = truncate != 0;
+  pr->npatch = 4;
+  pr->an_r0 = r0;
+  pr->an_r1 = r1;
+  pr->an_dth = (double)((qf)M_PI / 2 / (qf)n1);
+  pr->an_dr = (double)(((qf)r1 - (qf)r0) / (qf)n2);
+  for (int p = 0; p < 4; ++p) {
+    const int nr[3] = {n1, n2, 1};
+    for (int d = 0; d < 3; ++d) {
+      pr->proots[p][d] = nr[d];
+      pr->plo[p][d] = 0;
+      pr->phi_[p][d] = nr[d];
+      pr->periodic[d] = false;
+    }
+    for (int64_t r = 0; r < (int64_t)n1 * n2; ++r)
+      pr->leaves.push_back(Leaf{p, 0, {r % n1, r / n1, 0}});
+  }
+  finalize_leaves(*pr);
+  resolve_faces(*pr);
+  allocate_work(*pr);
+  build_annulus_metric(*pr);
+  return pr;
+}
+
+// The diagonal metric's coefficients (nb, 1 + D, nn) and the truncation pairs
+// (nb, D, E, P1) of an annulus; 0 for other geometries.
+int orc_dcoef(void* p, double* coef, double* lmode, double* ldual) {
+  Problem* pr = (Problem*)p;
+  if (pr->geometry != 2) return 0;
+  std::memcpy(coef, pr->dcoef.data(), pr->dcoef.size() * sizeof(double));
+  std::memcpy(lmode, pr->lmode.data(), pr->lmode.size() * sizeof(double));
+  std::memcpy(ldual, pr->ldual.data(), pr->ldual.size() * sizeof(double));
+  return 1;
+}
+
 void orc_free(void* p) { delete (Problem*)p; }
 int64_t orc_nblocks(void* p) { return ((Problem*)p)->nb; }
 int orc_N(void* p) { return ((Problem*)p)->N; }
@@ -1599,7 +1791,7 @@ void orc_coords(void* p, double* x) {
 // Coefficients of a full geometry, layout (nb, 1 + D*D, N^D); returns 0 if none.
 int orc_coef(void* p, double* out, uint8_t* flat) {
   Problem* pr = (Problem*)p;
-  if (pr->geometry == 0) return 0;
+  if (pr->geometry != 1) return 0;
   std::memcpy(out, pr->coef.data(), pr->coef.size() * sizeof(double));
   std::memcpy(flat, pr->flat.data(), pr->flat.size());
   return 1;
@@ -1635,6 +1827,24 @@ void orc_fill_exact(void* p, int kind, double t, const double* prm, double* y) {
         double th = sx - pr->omega * t + pr->phase;
         u = std::sin(th);
         v = -pr->omega * std::cos(th);
+      } else if (kind == 4) {
+        // the rotating Neumann mode u = R(kr) cos(m th - k t), v = k R sin(...),
+        // R(z) = s (J_m(z) Y_m'(k r0) - Y_m(z) J_m'(k r0)), s = pi k r0 / 2
+        // (annulus.jl:95-135); prm: m, k
+        const int m = (int)prm[0];
+        const double k = prm[1];
+        double xl[3];
+        node_logical(*pr, b, I, xl);
+        const double r = pr->an_r0 + pr->an_dr * xl[1];
+        const double th = pr->leaves[b].patch * (M_PI / 2) + pr->an_dth * xl[0];
+        const double kr0 = k * pr->an_r0;
+        const double jr0 = (jn(m - 1, kr0) - jn(m + 1, kr0)) / 2;
+        const double yr0 = (yn(m - 1, kr0) - yn(m + 1, kr0)) / 2;
+        const double sc = M_PI * k * pr->an_r0 / 2;
+        const double R = sc * (jn(m, k * r) * yr0 - yn(m, k * r) * jr0);
+        const double ph = m * th - k * t;
+        u = R * std::cos(ph);
+        v = k * R * std::sin(ph);
       } else {
         double r2 = 0;
         for (int d = 0; d < D; ++d) r2 += (x[d] - prm[d]) * (x[d] - prm[d]);
@@ -1659,7 +1869,7 @@ void orc_gradients(void* p, const double* y, double t, double* g) {
     for (int64_t l = 0; l < pr->nn; ++l) {
       int64_t I[3] = {l % N, (l / N) % N, l / ((int64_t)N * N)};
       for (int d = 0; d < D; ++d) {
-        if (pr->geometry == 0) {
+        if (pr->geometry != 1) {
           int64_t gn[3] = {1, 1, 1};
           for (int e = 0; e < D; ++e) gn[e] = N + (e == d ? 2 : 0);
           int64_t G[3] = {I[0], I[1], I[2]};
@@ -1697,7 +1907,7 @@ void orc_ghosts(void* p, const double* y, double t, double* ug, double* gg) {
           S[d] = s ? N + 1 : 0;
           for (int k = 0; k < nt; ++k) S[tt[k]] = T[k] + 1;
           ug[row] = pr->ufs[b * pr->ufs_n + hidx(*pr, S, n)];
-          if (pr->geometry == 0) {
+          if (pr->geometry != 1) {
             int64_t gn[3] = {1, 1, 1};
             for (int e = 0; e < D; ++e) gn[e] = N + (e == d ? 2 : 0);
             int64_t G[3] = {0, 0, 0};
@@ -1747,7 +1957,7 @@ double orc_min_spacing(void* p) {
 
 double orc_timestep(void* p, double cfl) {
   Problem* pr = (Problem*)p;
-  double h = pr->geometry == 0 ? orc_min_spacing(p) : pr->hspeed;
+  double h = pr->geometry == 0 ? orc_min_spacing(p) : pr->hspeed;  // metric: min h_b / speed_b
   return cfl * h / (pr->P + 1);
 }
 

@@ -35,6 +35,9 @@ def lib():
                                                                              ctypes.c_double,
                                                                              ctypes.c_int, _dp]
         L.orc_ball.restype = _dp
+        L.orc_annulus.argtypes = [ctypes.c_int] * 4 + [ctypes.c_double] * 2 + [ctypes.c_int]
+        L.orc_annulus.restype = _dp
+        L.orc_dcoef.argtypes = [_dp, _dp, _dp, _dp]
         L.orc_free.argtypes = [_dp]
         L.orc_nblocks.argtypes = [_dp]
         L.orc_nblocks.restype = ctypes.c_int64
@@ -116,6 +119,23 @@ class Problem:
         p.k = kv
         p.phi = phi
         return p
+
+    @classmethod
+    def annulus(cls, P, E, roots=(2, 1), r0=1.0, r1=2.0, truncate=True):
+        """The four-patch annulus with its diagonal metric (geometry.jl:176-268)."""
+        h = lib().orc_annulus(P, E, roots[0], roots[1], float(r0), float(r1), int(truncate))
+        return cls(h, 2, P, E, 2)
+
+    def dcoef(self):
+        """(sqrt(g), A_1, A_2) at the nodes (nb, 3, nn) and the truncation pairs
+        (nb, D, E, P+1) of an annulus."""
+        c = np.zeros(self.nb * 3 * self.nn)
+        m = np.zeros(self.nb * 2 * self.E * (self.P + 1))
+        d = np.zeros_like(m)
+        if not lib().orc_dcoef(self.h, _ptr(c), _ptr(m), _ptr(d)):
+            return None
+        sh = (self.nb, 2, self.E, self.P + 1)
+        return c.reshape(self.nb, 3, self.nn), m.reshape(sh), d.reshape(sh)
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -216,6 +236,27 @@ class Problem:
         y = np.array(y, dtype=np.float64, copy=True)
         lib().orc_rk4(self.h, _ptr(y), float(t0), float(dt), int(nsteps))
         return y
+
+
+def annulus_errors(P, E, roots=(2, 1), r0=1.0, r1=2.0, m=1, k=None, cfl=0.5, tend=None,
+                   truncate=True, nsteps=None):
+    """The oracle's annulus_errors (annulus.jl:137-162): k the wavenumber of the
+    mode (the caller's, from annulus_wavenumber)."""
+    pr = Problem.annulus(P, E, roots=roots, r0=r0, r1=r1, truncate=truncate)
+    t_end = tend if tend is not None else math.pi / 2 * max(m, 1) / k
+    y0 = pr.truncate(pr.fill_exact(4, 0.0, (m, k)))
+    if nsteps is None:
+        nsteps = math.ceil(t_end / pr.timestep(cfl))
+    dt = t_end / nsteps
+    y = pr.rk4(y0, 0.0, dt, nsteps)
+    err = y - pr.fill_exact(4, t_end, (m, k))
+    l2, linf = pr.norms(err)
+    l2u, _ = pr.norms(err, vars=(1,))
+    l2v, _ = pr.norms(err, vars=(2,))
+    h = pr.min_spacing() * (r1 - r0) / roots[1]
+    return dict(l2=l2, l2u=l2u, l2v=l2v, linf=linf, h=h, nsteps=nsteps, dt=dt, tend=t_end,
+                nblocks=pr.nb, energy0=pr.energy(y0), energy=pr.energy(y), y=y, y0=y0,
+                problem=pr)
 
 
 def nested_cubes(D, w0, levels):

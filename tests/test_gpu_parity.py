@@ -490,3 +490,48 @@ def test_generic_step_graph_replays_match_eager_steps(which):
         ib.timed(n)                     # eager launches, event-timed
         T.wave_energy(b, ib.u, t=ib.t)  # other work on the device between calls
         assert np.array_equal(ia.u, ib.u), n
+
+
+@pytest.mark.parametrize("P", [2, 3, 4])
+def test_annulus_rhs_truncate_energy_norms_match_oracle(P, rough):
+    """The diagonal metric on the four-patch annulus (flux A_d D_d u, divergence
+    over sqrt(g), identity glues in theta, walls in r, per-line truncation pairs,
+    sqrt(g)-weighted energy and norms; evolution.jl:67-110, 556-575, 1472-1512)
+    against the oracle on a scrambled state."""
+    forest, geo = T.annulus_forest(P, 2, roots=(2, 2))
+    prob = T.WaveProblem(forest, T.Element(P), geometry=geo)
+    assert not prob.info()["fused"]
+    ref = O.Problem.annulus(P, 2, roots=(2, 2))
+    u = rough(prob.state_len, seed=13)
+    du = np.empty_like(u)
+    T.wave_rhs(du, u, prob, 0.0)
+    want = ref.rhs(u)
+    assert np.max(np.abs(du - want)) <= 1e-12 * np.max(np.abs(want))
+    got = T.truncate_state(u.copy(), prob)
+    want = ref.truncate(u)
+    assert np.max(np.abs(got - want)) <= 1e-13 * np.max(np.abs(want))
+    assert T.wave_energy(prob, u) == pytest.approx(ref.energy(u), rel=1e-13)
+    assert T.wave_norm(prob, u) == pytest.approx(ref.norms(u)[0], rel=1e-14)
+    assert T.wave_norm(prob, u, math.inf) == ref.norms(u)[1]
+    assert T.oldg_timestep(prob, 0.5) == ref.timestep(0.5)
+
+
+def test_annulus_errors_match_oracle_and_published():
+    """annulus_errors(P=3, E=2, roots=(2s, s)), m=2, n=1, to a quarter turn: the
+    published u 5.06e-3, 6.30e-4 and v 3.35e-3, 4.23e-4 at s = 1, 2 (CODE.md:4194),
+    the oracle's run to sqrt(eps) (state, l2, energy), and FP32c against it."""
+    k = T.annulus_wavenumber(2, 1, 1.0, 2.0)
+    for s, eu, ev in ((1, 5.06e-3, 3.35e-3), (2, 6.30e-4, 4.23e-4)):
+        r = T.annulus_errors(3, 2, roots=(2 * s, s), m=2)
+        o = O.annulus_errors(3, 2, roots=(2 * s, s), m=2, k=k)
+        assert r["k"] == k and r["nsteps"] == o["nsteps"]
+        assert float(f"{r['l2u']:.3g}") == eu and float(f"{r['l2v']:.3g}") == ev
+        assert r["l2"] == pytest.approx(o["l2"], rel=SQRT_EPS)
+        assert r["energy"] == pytest.approx(o["energy"], rel=SQRT_EPS)
+        assert r["energy0"] == pytest.approx(o["energy0"], rel=1e-13)
+        u = r["integrator"].u
+        assert np.linalg.norm(u - o["y"]) <= SQRT_EPS * np.linalg.norm(o["y"])
+        r32 = T.annulus_errors(3, 2, roots=(2 * s, s), m=2, precision="f32c")
+        assert r32["l2"] == pytest.approx(o["l2"], rel=1e-3)
+        u32 = r32["integrator"].u
+        assert np.linalg.norm(u32 - o["y"]) <= 1e-5 * np.linalg.norm(o["y"])

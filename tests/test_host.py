@@ -226,3 +226,44 @@ def test_convergence_rate_closed_form():
     errs = [2.0 * h ** 3 * (1.1 if i % 2 else 1.0) for i, h in enumerate(hs)]
     x, y = np.log(hs), np.log(errs)
     assert T.convergence_rate(hs, errs) == pytest.approx(np.polyfit(x, y, 1)[0], rel=1e-12)
+
+
+@pytest.mark.parametrize("roots", [(2, 1), (4, 2)])
+def test_annulus_forest_metric_tables_match_oracle(roots):
+    """The annulus on the host (leaf order, faces: identity glue in theta, walls in
+    r; sqrt(g), A_d and hspeed; the per-line truncation pairs built in 40-digit
+    arithmetic) equals the oracle's binary128 build bit for bit, and the exact
+    mode equals the oracle's to the Bessel functions' rounding."""
+    P = 3
+    forest, geo = T.annulus_forest(P, 2, roots=roots)
+    ref = O.Problem.annulus(P, 2, roots=roots)
+    assert [(lf[0], lf[2], lf[3]) for lf in forest.leaves] == \
+        [(b[0], b[2][0], b[2][1]) for b in ref.blocks()]
+    kind, nb = ref.faces()
+    F = forest.faces()
+    for b in range(forest.nblocks):
+        for d in range(2):
+            for s in range(2):
+                f = F[(b * 2 + d) * 2 + s]
+                assert f.kind == kind[b, d, s]
+                if f.kind == _lib.FACE_SAME:
+                    assert f.nb[0] == nb[b, d, s]
+    el = T.Element(P)
+    m = geo.metric(forest, el)
+    c, lm, ld = ref.dcoef()
+    assert np.array_equal(m["dcoef"].reshape(c.shape), c)
+    assert m["hspeed"] == O.lib().orc_hspeed(ref.h)
+    tt = geo.truncation_tables(forest, el)
+    assert np.array_equal(tt["lmode"], lm) and np.array_equal(tt["ldual"], ld)
+    k = T.annulus_wavenumber(2, 1, 1.0, 2.0)
+    x = T.problem.fill_at_nodes_forest(forest, T.annulus_exact(geo, forest, 2, k, 0.3), el)
+    assert np.max(np.abs(x - ref.fill_exact(4, 0.3, (2, k)))) < 1e-14
+
+
+def test_annulus_refusals():
+    with pytest.raises(ValueError, match="0 < r0 < r1"):
+        T.annulus_forest(3, 2, r0=2.0, r1=1.0)
+    with pytest.raises(ValueError, match="E must be even"):
+        T.annulus_forest(3, 3)
+    with pytest.raises(ValueError, match="m >= 0 and n >= 1"):
+        T.annulus_wavenumber(1, 0, 1.0, 2.0)

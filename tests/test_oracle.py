@@ -355,3 +355,74 @@ def test_ball_truncation_is_metric_orthogonal_projection(P):
     assert np.max(np.abs(_tensor_truncate(v0, P, 2) - v0)) > 1e-2
     assert np.array_equal(Pa.reshape(ref.nb, 2, nn)[:, 0], a.reshape(ref.nb, 2, nn)[:, 0])
     assert inner(Pa, Pa) <= inner(a, a)
+
+
+# --- the annulus: a diagonal metric (geometry.jl:176-268, CODE.md:4172-4197) ------------
+def test_annulus_published_table():
+    """CODE.md:4188-4197 (the rotating Neumann mode m=2, n=1 on 1 <= r <= 2 to a
+    quarter turn, roots (2s, s), E=2, cfl 1/2): P=3 truncated u 5.06e-3, 6.30e-4
+    and v 3.35e-3, 4.23e-4 at s = 1, 2; P=4 truncated at s=1: u 3.77e-4, v 3.26e-4;
+    P=2 without truncation at s=1: u 5.05e-3, v 3.77e-2; energy loss at most the
+    table's 2.0e-10 (P=3).  And the wavenumber, k = 1.3406."""
+    k = T.annulus_wavenumber(2, 1, 1.0, 2.0)
+    assert float(f"{k:.5g}") == 1.3406
+    rows = [(3, True, 1, 5.06e-3, 3.35e-3), (3, True, 2, 6.30e-4, 4.23e-4),
+            (4, True, 1, 3.77e-4, 3.26e-4), (2, False, 1, 5.05e-3, 3.77e-2)]
+    for P, trunc, s, eu, ev in rows:
+        r = O.annulus_errors(P, 2, roots=(2 * s, s), m=2, k=k, truncate=trunc)
+        assert float(f"{r['l2u']:.3g}") == eu and float(f"{r['l2v']:.3g}") == ev, (P, s, r)
+        if P == 3 and s == 1:
+            assert 0 < 1 - r["energy"] / r["energy0"] < 2.05e-10
+
+
+@pytest.mark.parametrize("P", [2, 3])
+def test_annulus_energy_rate_vanishes(P):
+    """Every glue and wall of the annulus conserves the metric energy
+    1/2 sum w (sqrt(g) v^2 + sum_d F_d^2 / A_d) of the semi-discrete system, with
+    and without the truncation: the rate along f(y) is (E(y+f) - E(y-f))/2."""
+    for trunc in (False, True):
+        ref = O.Problem.annulus(P, 2, roots=(2, 2), truncate=trunc)
+        y = ref.truncate(np.random.default_rng(7).standard_normal(ref.size()))
+        f = ref.rhs(y)
+        rate = (ref.energy(y + f) - ref.energy(y - f)) / 2
+        assert abs(rate) < 1e-12 * (ref.energy(y) + ref.energy(f)), (P, trunc, rate)
+
+
+@pytest.mark.parametrize("P", [2, 3, 4])
+def test_annulus_truncation_is_metric_orthogonal_projection(P):
+    """The per-line pairs make the truncation the sqrt(g)-orthogonal projection
+    onto Q_{P-1} (evolution.jl:556-575): self-adjoint in the sqrt(g)-weighted
+    quadrature, idempotent, u untouched, the flat tensor projector's range (and
+    not its projection) kept, and no longer in that norm."""
+    ref = O.Problem.annulus(P, 2, roots=(2, 2))
+    n, nn = ref.size(), ref.nn
+    rng = np.random.default_rng(32)
+    a, b = rng.standard_normal(n), rng.standard_normal(n)
+    Pa, Pb = ref.truncate(a), ref.truncate(b)
+
+    def nv(x):
+        return ref.norms(x, vars=(2,))[0]
+
+    def inner(x, y):
+        return (nv(x + y) ** 2 - nv(x - y) ** 2) / 4
+
+    scale = math.sqrt(inner(a, a) * inner(b, b))
+    assert abs(inner(Pa, b) - inner(a, Pb)) < 1e-14 * scale
+    assert np.allclose(ref.truncate(Pa), Pa, rtol=0, atol=1e-13 * np.max(np.abs(Pa)))
+    assert np.array_equal(Pa.reshape(-1, 2, nn)[:, 0], a.reshape(-1, 2, nn)[:, 0])
+    assert inner(Pa, Pa) <= inner(a, a)
+    # the range is Q_{P-1} per element: the flat projector leaves it, not a generic state
+    flat = O.Problem.brick(2, P, 2, roots=1)
+    N = ref.N
+    for y, keep in ((Pa, True), (a, False)):
+        blk = y.reshape(-1, 2, nn)[:, 1]
+        z = np.zeros((blk.shape[0], 2, nn))
+        z[:, 1] = blk
+        moved = max(np.max(np.abs(flat.truncate(zi.reshape(-1)) - zi.reshape(-1))) for zi in z)
+        assert (moved < 1e-13 * np.max(np.abs(blk))) == keep, (keep, moved)
+    # the angular lines' pair is the flat one (a constant factor), the radial one is not
+    _, lm, ld = ref.dcoef()
+    el = O.element(P)
+    assert np.allclose(lm[:, 0], el["topmode"], rtol=0, atol=1e-15)
+    assert np.allclose(ld[:, 0], el["topdual"], rtol=0, atol=1e-15)
+    assert np.max(np.abs(lm[:, 1] - el["topmode"])) > 1e-6

@@ -140,7 +140,7 @@ void destroy(twoldg_problem* p) {
                   p->grads, p->tables, p->coef, (void*)p->flat, (void*)p->tslot, p->tmodes,
                   p->tweights, p->tginv, (void*)p->spacing, (void*)p->rowptr, (void*)p->cb,
                   (void*)p->cl, p->cval_u, p->cval_g, (void*)p->dir_rows, (void*)p->dir_x,
-                  (void*)p->part, (void*)p->red, (void*)p->clock, (void*)p->nbr6, p->uf[0], p->uf[1], p->pf[0],
+                  (void*)p->part, (void*)p->red, (void*)p->clock, p->dcoef, p->lmode, p->ldual, (void*)p->nbr6, p->uf[0], p->uf[1], p->pf[0],
                   p->pf[1], (void*)p->frows, (void*)p->fsrc, p->fwu,
                   p->fwg, p->fgu, p->fgg, p->ffg})
     dfree(b);
@@ -177,6 +177,11 @@ int check_desc(const twoldg_desc* d) {
   if (d->nblocks > INT32_MAX) return fail(TWOLDG_EINVAL, "too many blocks");
   if (d->coef && d->D != 3)
     return fail(TWOLDG_EINVAL, "a full metric is supported in three dimensions only");
+  if (d->dcoef && d->coef)
+    return fail(TWOLDG_EINVAL, "a problem has a diagonal or a full metric, not both");
+  if ((d->lmode || d->ldual) && (!d->dcoef || !d->lmode || !d->ldual))
+    return fail(TWOLDG_EINVAL, "per-line truncation pairs need a diagonal metric and both "
+                               "the modes and the duals");
   if (d->tslot && (!d->coef || !d->tmodes || !d->tweights || !d->tchol || d->P > 4))
     return fail(TWOLDG_EINVAL, "per-element truncation tables need a full metric, all "
                                "tables, and P <= 4");
@@ -230,6 +235,15 @@ int create_buffers(twoldg_problem* p, const twoldg_desc* d, const Csr& csr) {
       for (int64_t b = 0; b < p->sh.nb; ++b) fl[b] = d->flat[b] ? 1 : 0;
     if ((rc = h2d(p, p->flat, fl.data(), p->sh.nb))) return rc;
   }
+  if (d->dcoef) {
+    const size_t nc = (size_t)p->sh.nb * (1 + D) * p->sh.nn;
+    if ((rc = upload_as<T>(p, &p->dcoef, d->dcoef, nc))) return rc;
+    if (d->lmode) {
+      const size_t nl = (size_t)p->sh.nb * D * p->E * p->P1;
+      if ((rc = upload_as<T>(p, &p->lmode, d->lmode, nl))) return rc;
+      if ((rc = upload_as<T>(p, &p->ldual, d->ldual, nl))) return rc;
+    }
+  }
   if (d->tslot) {
     p->has_tables = true;
     if ((rc = dalloc(p, &p->tslot, p->sh.nb))) return rc;
@@ -280,7 +294,7 @@ int create_buffers(twoldg_problem* p, const twoldg_desc* d, const Csr& csr) {
   // fused one-launch stages (flat metric, 3D): faces same-level or mirrors read
   // the neighbours' face buffers directly; with coarse-fine faces the face
   // ghosts are formed first by two face-sized launches (the ghosted form)
-  bool eligible = !p->full && fused_supported(D, p->P1, p->E);
+  bool eligible = !p->full && !p->dcoef && fused_supported(D, p->P1, p->E);
   bool nonconforming = false;
   for (int64_t i = 0; eligible && i < p->sh.nb * 2 * D; ++i) {
     const int k = d->faces[i].kind;
@@ -673,7 +687,8 @@ int twoldg_norms_noalloc(twoldg_problem* p, int a, int b, int vars_mask, int pno
 
 int twoldg_timestep(const twoldg_problem* p, double cfl, double* out) {
   if (!p || !out) return fail(TWOLDG_EINVAL, "bad timestep call");
-  const double h = p->full ? p->hspeed : p->hmin;
+  // a metric's step: the smallest h_b over the block's largest logical speed
+  const double h = (p->full || p->dcoef) ? p->hspeed : p->hmin;
   *out = cfl * h / (p->P + 1);
   return 0;
 }

@@ -226,7 +226,8 @@ __device__ __forceinline__ T wide(const Tables<T>& tb, const int I[kMaxD], int d
 template <class T, int D, int P1>
 __global__ void k_gradient(Shape sh, const Tables<T>* __restrict__ tbp,
                            const double* __restrict__ spacing, const T* __restrict__ y,
-                           const T* __restrict__ ug, T* __restrict__ grads) {
+                           const T* __restrict__ ug, const T* __restrict__ dcoef,
+                           T* __restrict__ grads) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= sh.nb * sh.nn) return;
   const int64_t b = g / sh.nn, l = g % sh.nn;
@@ -246,7 +247,9 @@ __global__ void k_gradient(Shape sh, const Tables<T>* __restrict__ tbp,
       if (o >= N) return ug[((b * D + d) * 2 + 1) * sh.Nt + tl];
       return u[base + o * stride];
     };
-    grads[(b * D + d) * sh.nn + l] = wide<T, D, P1>(tb, I, d, read) / h;
+    const T g = wide<T, D, P1>(tb, I, d, read) / h;
+    // on a diagonal metric the flux A_d D_d u (evolution.jl:67-70)
+    grads[(b * D + d) * sh.nn + l] = dcoef ? dcoef[(b * (1 + D) + 1 + d) * sh.nn + l] * g : g;
   }
 }
 
@@ -349,7 +352,8 @@ __device__ __forceinline__ void stage_store(const StageIO<T>& io, int64_t iu, in
 template <class T, int D, int P1, int MODE>
 __global__ void k_stage_flat(Shape sh, const Tables<T>* __restrict__ tbp,
                              const double* __restrict__ spacing, const T* __restrict__ grads,
-                             const T* __restrict__ gg, StageIO<T> io) {
+                             const T* __restrict__ gg, const T* __restrict__ dcoef,
+                             StageIO<T> io) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= sh.nb * sh.nn) return;
   const int64_t b = g / sh.nn, l = g % sh.nn;
@@ -373,7 +377,9 @@ __global__ void k_stage_flat(Shape sh, const Tables<T>* __restrict__ tbp,
   }
   const int64_t iu = b * 2 * sh.nn + l, iv = iu + sh.nn;
   const T ku = io.ys[iv];
-  const T kv = acc / (T)spacing[b];
+  const T lap = acc / (T)spacing[b];
+  // on a diagonal metric the divergence over sqrt(g) (evolution.jl:69-70)
+  const T kv = dcoef ? lap / dcoef[b * (1 + D) * sh.nn + l] : lap;
   stage_store<T, MODE>(io, iu, iv, ku, kv);
 }
 
@@ -434,7 +440,8 @@ constexpr int kTruncThreads = 256;
 template <class T, int D, int P1>
 __global__ void __launch_bounds__(kTruncThreads)
 k_truncate(Shape sh, int E, const Tables<T>* __restrict__ tbp,
-           const int32_t* __restrict__ tslot, T* __restrict__ y) {
+           const int32_t* __restrict__ tslot, const T* __restrict__ lmode,
+           const T* __restrict__ ldual, T* __restrict__ y) {
   const int64_t b = blockIdx.x;
   if (tslot && tslot[b] >= 0) return;  // per-element tables (k_truncate_elem)
   const Tables<T>& tb = *tbp;
@@ -457,6 +464,17 @@ k_truncate(Shape sh, int E, const Tables<T>* __restrict__ tbp,
       T* line = a + base;
       T vals[P1];
       T c = 0;
+      if (lmode) {  // the line's own pair on a diagonal metric (evolution.jl:556-575)
+        const int64_t t = ((b * D + d) * E + ed) * P1;
+#pragma unroll
+        for (int j = 0; j < P1; ++j) {
+          vals[j] = line[j * strides[d]];
+          c += ldual[t + j] * vals[j];
+        }
+#pragma unroll
+        for (int i = 0; i < P1; ++i) line[i * strides[d]] = vals[i] - lmode[t + i] * c;
+        continue;
+      }
 #pragma unroll
       for (int j = 0; j < P1; ++j) {
         vals[j] = line[j * strides[d]];
@@ -605,13 +623,14 @@ __device__ __forceinline__ double node_weight(const int I[kMaxD], double h, int 
   return w;
 }
 
-// Energy partials per block: (kinetic, potential).  flat: sum w v^2 and
-// sum_d w g_d^2; full metric: weights times sqrt(g), |V|^2 (evolution.jl:1472-1489).
+// Energy partials per block: (kinetic, potential) (evolution.jl:1472-1489).
+// flat: sum w v^2 and sum_d w g_d^2; diagonal metric (dcoef): w sqrt(g) v^2 and
+// w F_d^2 / A_d; full metric: weights times sqrt(g), |V|^2.
 template <class T, int D, int P1, bool FULL>
 __global__ void k_energy_part(Shape sh, const Tables<T>* __restrict__ tbp,
                               const double* __restrict__ spacing, const T* __restrict__ y,
                               const T* __restrict__ grads, const T* __restrict__ coef,
-                              double* __restrict__ part) {
+                              const T* __restrict__ dcoef, double* __restrict__ part) {
   const int64_t b = blockIdx.x;
   const double h = spacing[b];
   double v[2] = {0, 0};
@@ -621,6 +640,17 @@ __global__ void k_energy_part(Shape sh, const Tables<T>* __restrict__ tbp,
     double w = node_weight<D>(I, h, P1, tbp->wq);
     const double vv = (double)y[b * 2 * sh.nn + sh.nn + l];
     double pot = 0;
+    if (dcoef) {
+      const T* c = dcoef + b * (1 + D) * sh.nn;
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        const double g = (double)grads[(b * D + d) * sh.nn + l];
+        pot += w * (1.0 / (double)c[(1 + d) * sh.nn + l]) * (g * g);
+      }
+      v[0] += w * (double)c[l] * (vv * vv);
+      v[1] += pot;
+      continue;
+    }
 #pragma unroll
     for (int d = 0; d < D; ++d) {
       const double g = (double)grads[(b * D + d) * sh.nn + l];
@@ -642,8 +672,9 @@ template <class T, int D, int P1, bool FULL>
 __global__ void k_norm_part(Shape sh, const Tables<T>* __restrict__ tbp,
                             const double* __restrict__ spacing, const T* __restrict__ a,
                             const T* __restrict__ alo, const T* __restrict__ bb,
-                            const T* __restrict__ blo, const T* __restrict__ coef, int mask,
-                            int pnorm, double* __restrict__ part) {
+                            const T* __restrict__ blo, const T* __restrict__ coef,
+                            const T* __restrict__ dcoef, int mask, int pnorm,
+                            double* __restrict__ part) {
   const int64_t b = blockIdx.x;
   const double h = spacing[b];
   double v[3] = {0, 0, 0};
@@ -652,6 +683,7 @@ __global__ void k_norm_part(Shape sh, const Tables<T>* __restrict__ tbp,
     unravel<D>(l, sh.N, I);
     double w = node_weight<D>(I, h, P1, tbp->wq);
     if (FULL) w *= (double)coef[b * (1 + D * D) * sh.nn + l];
+    if (dcoef) w *= (double)dcoef[b * (1 + D) * sh.nn + l];  // sqrt(g)
     double s = 0;
     int nv = 0;
     for (int q = 0; q < 2; ++q) {
@@ -758,6 +790,9 @@ struct twoldg_problem {
   void* grads = nullptr;    // (nb, D, nn)
   void* tables = nullptr;   // Tables<T>
   void* coef = nullptr;     // (nb, 1+D*D, nn) in T
+  void* dcoef = nullptr;    // diagonal metric (nb, 1+D, nn): sqrt(g), A_d
+  void* lmode = nullptr;    // per-line truncation pairs (nb, D, E, P1)
+  void* ldual = nullptr;
   uint8_t* flat = nullptr;
   int32_t* tslot = nullptr;
   void* tmodes = nullptr;
@@ -1103,7 +1138,7 @@ struct Launch {
     if (!p->full) {
       hipLaunchKernelGGL((k_gradient<T, D, P1>), dim3(grid_for(n, 256)), dim3(256), 0, p->stream,
                          p->sh, (const Tables<T>*)p->tables, p->spacing, y, (const T*)p->ug,
-                         (T*)p->grads);
+                         (const T*)p->dcoef, (T*)p->grads);
       ghosts<1>(p, (const T*)p->cval_g, (const T*)p->grads, cdt, (T*)p->gg);
     } else {
       hipLaunchKernelGGL((k_gradient_cart<T, D, P1>), dim3(grid_for(n, 256)), dim3(256), 0,
@@ -1124,7 +1159,7 @@ struct Launch {
     if (!p->full)
       hipLaunchKernelGGL((k_stage_flat<T, D, P1, MODE>), dim3(grid_for(n, 256)), dim3(256), 0,
                          p->stream, p->sh, (const Tables<T>*)p->tables, p->spacing,
-                         (const T*)p->grads, (const T*)p->gg, io);
+                         (const T*)p->grads, (const T*)p->gg, (const T*)p->dcoef, io);
     else
       hipLaunchKernelGGL((k_stage_cart<T, D, P1, MODE>), dim3(grid_for(n, 256)), dim3(256), 0,
                          p->stream, p->sh, (const Tables<T>*)p->tables, p->spacing,
@@ -1144,17 +1179,20 @@ struct Launch {
       }
     }
     hipLaunchKernelGGL((k_truncate<T, D, P1>), dim3((unsigned)p->sh.nb), dim3(kTruncThreads), 0,
-                       p->stream, p->sh, p->E, (const Tables<T>*)p->tables, p->tslot, y);
+                       p->stream, p->sh, p->E, (const Tables<T>*)p->tables, p->tslot,
+                       (const T*)p->lmode, (const T*)p->ldual, y);
   }
   static void energy(twoldg_problem* p, const T* y) {
     if (!p->full)
       hipLaunchKernelGGL((k_energy_part<T, D, P1, false>), dim3((unsigned)p->sh.nb),
                          dim3(kRedThreads), 0, p->stream, p->sh, (const Tables<T>*)p->tables,
-                         p->spacing, y, (const T*)p->grads, (const T*)nullptr, p->part);
+                         p->spacing, y, (const T*)p->grads, (const T*)nullptr,
+                         (const T*)p->dcoef, p->part);
     else
       hipLaunchKernelGGL((k_energy_part<T, D, P1, true>), dim3((unsigned)p->sh.nb),
                          dim3(kRedThreads), 0, p->stream, p->sh, (const Tables<T>*)p->tables,
-                         p->spacing, y, (const T*)p->grads, (const T*)p->coef, p->part);
+                         p->spacing, y, (const T*)p->grads, (const T*)p->coef,
+                         (const T*)nullptr, p->part);
     hipLaunchKernelGGL((k_final<2>), dim3(1), dim3(kRedThreads), 0, p->stream, p->sh.nb, p->part,
                        false, p->red);
   }
@@ -1163,11 +1201,13 @@ struct Launch {
     if (!p->full)
       hipLaunchKernelGGL((k_norm_part<T, D, P1, false>), dim3((unsigned)p->sh.nb),
                          dim3(kRedThreads), 0, p->stream, p->sh, (const Tables<T>*)p->tables,
-                         p->spacing, a, alo, b, blo, (const T*)nullptr, mask, pnorm, part);
+                         p->spacing, a, alo, b, blo, (const T*)nullptr, (const T*)p->dcoef,
+                         mask, pnorm, part);
     else
       hipLaunchKernelGGL((k_norm_part<T, D, P1, true>), dim3((unsigned)p->sh.nb),
                          dim3(kRedThreads), 0, p->stream, p->sh, (const Tables<T>*)p->tables,
-                         p->spacing, a, alo, b, blo, (const T*)p->coef, mask, pnorm, part);
+                         p->spacing, a, alo, b, blo, (const T*)p->coef, (const T*)nullptr,
+                         mask, pnorm, part);
     hipLaunchKernelGGL((k_final<3>), dim3(1), dim3(kRedThreads), 0, p->stream, p->sh.nb, part,
                        true, red);
   }

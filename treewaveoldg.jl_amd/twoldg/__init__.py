@@ -13,6 +13,8 @@ from .drivers import (wave_omega, wave_exact, reflect_omega, reflect_exact, gaus
                       PlaneWave, sevenpatch_exact, wave_errors, reflect_errors, sevenpatch_errors,
                       convergence_rate)
 from .batch import Batch, gaussian_pulses
+from .annulus import (PolarAnnulus, annulus_forest, annulus_wavenumber, annulus_exact,
+                      annulus_errors)
 from . import _lib
 
 __all__ = [
@@ -22,6 +24,7 @@ __all__ = [
     "oldg_timestep", "wave_integrator", "Integrator", "exchange_plan", "wave_omega",
     "wave_exact", "reflect_omega", "reflect_exact", "gaussian_pulse", "PlaneWave",
     "sevenpatch_exact", "wave_errors", "reflect_errors", "sevenpatch_errors",
-    "convergence_rate", "Batch",
+    "convergence_rate", "Batch", "PolarAnnulus", "annulus_forest", "annulus_wavenumber",
+    "annulus_exact", "annulus_errors",
     "gaussian_pulses",
 ]

@@ -10,7 +10,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(_HERE, "libtwoldg.so")
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 F64, F32C = 0, 1
 FACE_SAME, FACE_COARSER, FACE_FINER, FACE_MIRROR, FACE_GLUE, FACE_DIRICHLET = 1, 2, 3, 4, 5, 6
 SLOT_STATE, SLOT_DU, SLOT_AUX, SLOT_ARG = 0, 1, 2, 3
@@ -34,7 +34,8 @@ class Desc(ctypes.Structure):
                 ("tchol", ctypes.c_void_p), ("dirichlet_x", ctypes.c_void_p),
                 ("wave_k", ctypes.c_double * 3), ("wave_phi", ctypes.c_double),
                 ("hspeed", ctypes.c_double), ("flags", ctypes.c_int32),
-                ("pad", ctypes.c_int32)]
+                ("pad", ctypes.c_int32), ("dcoef", ctypes.c_void_p),
+                ("lmode", ctypes.c_void_p), ("ldual", ctypes.c_void_p)]
 
 
 class TwoldgError(RuntimeError):

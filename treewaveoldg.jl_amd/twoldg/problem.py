@@ -63,12 +63,13 @@ class WaveProblem:
         if precision not in PRECISIONS:
             raise ValueError(f"precision must be one of {sorted(PRECISIONS)}, got {precision!r}")
         geometry = geometry if geometry is not None else forest.geometry
-        if boundary is not None and geometry is None:
+        full = geometry is not None and getattr(geometry, "kind", "full") == "full"
+        if boundary is not None and not full:
             raise ValueError(
                 "Dirichlet data (`boundary`) are given in Cartesian components and taken by a "
                 "full geometry only: without one the gradient sets hold logical components, "
                 "and the walls are TreeAMR's reflecting faces")
-        if geometry is not None and boundary is None:
+        if full and boundary is None:
             raise ValueError(
                 "the geometry has outer faces, and no data were given for them: pass "
                 "`boundary = PlaneWave(k, phi)`, the Dirichlet data on the outer sphere")
@@ -94,7 +95,18 @@ class WaveProblem:
         desc.faces = ctypes.cast(self._faces, ctypes.c_void_p)
         desc.flags = _lib.FLAG_GENERIC if generic else 0
         self.hspeed = 0.0
-        if geometry is not None:
+        if geometry is not None and not full:
+            # a diagonal metric (the annulus): sqrt(g), A_d and per-line truncation pairs
+            m = geometry.metric(forest, el)
+            self.dcoef = m["dcoef"]
+            self.hspeed = m["hspeed"]
+            desc.dcoef = _ptr(self.dcoef)
+            desc.hspeed = self.hspeed
+            if self.truncate:
+                self._tables = geometry.truncation_tables(forest, el)
+                desc.lmode = _ptr(self._tables["lmode"])
+                desc.ldual = _ptr(self._tables["ldual"])
+        elif geometry is not None:
             m = geometry.metric(forest, el)
             self.coef, self.flat = m["coef"], m["flat"]
             self.hspeed = m["hspeed"]
@@ -161,9 +173,12 @@ class WaveProblem:
 
     # -- positions -------------------------------------------------------------
     def node_coordinates(self):
-        """Physical positions of every owned node, (nblocks, D, N^D)."""
+        """The positions initial data are evaluated at, (nblocks, D, N^D): physical
+        on a full geometry (the seven-patch ball's data take x); logical on a flat
+        forest and on a diagonal geometry, whose data take each patch's logical
+        position (the reference's WithPatch callbacks, annulus.jl:95-135)."""
         x = self.forest.node_logical(self.element.offsets)
-        if self.geometry is not None:
+        if self.geometry is not None and getattr(self.geometry, "kind", "full") == "full":
             x = self.geometry.physical(self.forest, x)
         return x
 
@@ -280,7 +295,7 @@ def initial_state(p, f):
 def wave_energy(p, u, t=None):
     """The discrete energy 1/2 sum w (v^2 + sum_d (D_d u)^2), on a full geometry
     weighted by sqrt(g) with the Cartesian gradient."""
-    if p.geometry is not None and t is None:
+    if p.boundary is not None and t is None:
         raise ValueError("the problem has Dirichlet data, which depend on the time, and none "
                          "was given: pass `t`")
     p.upload(u, _lib.SLOT_ARG)
