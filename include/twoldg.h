@@ -162,6 +162,42 @@ int twoldg_timestep(const twoldg_problem* p, double cfl, double* out);
 /* wait for the problem's stream */
 int twoldg_synchronize(twoldg_problem* p);
 
+/* --- adaptive refinement ----------------------------------------------------- */
+/* The Legendre-mode amplitudes of variable var (0 = u, 1 = v) of a slot, one
+   value per element into tau (nblocks * E^D doubles; e_1 fastest, then e_2 ..
+   e_D, the block slowest): with c_m the coefficient of L_m along a line of the
+   element and A the largest |var| over the problem,
+       tau = post * max over the element's lines in every dimension and
+             q < nmodes of ((weights[q] / A) |c_modes[q]|)^pows[q],
+   zero where A is zero.  mode_kernel! and scaled_modes, src/refinement.jl:56-101;
+   modal_indicator (refinement.jl:148-155) is modes 1..M, weights (2m-1)!!,
+   pows M/m and post 1/(2M-1)!!; mode_amplitude (refinement.jl:168-174) is the
+   single mode with weight, pow and post 1.  1 <= nmodes <= 8, 0 <= modes[q] <= P. */
+int twoldg_modes(twoldg_problem* p, int slot, int var, int nmodes, const int32_t* modes,
+                 const double* weights, const double* pows, double post, double* tau);
+
+#define TWOLDG_XFER_COPY 0
+#define TWOLDG_XFER_REFINE 1
+#define TWOLDG_XFER_COARSEN 2
+/* Where one block of a regridded forest takes its data from in the forest
+   before the regrid. */
+typedef struct twoldg_transfer_block {
+  int32_t kind;    /* TWOLDG_XFER_* */
+  int32_t child;   /* REFINE: the block's offset in its parent, bit d for dimension d */
+  int64_t src[8];  /* COPY, REFINE: the source block src[0]; COARSEN: the 2^D children,
+                      the child at offset c (bit d for dimension d) in src[c] */
+} twoldg_transfer_block;
+/* The state of a regrid: every block of dst's slot dst_slot from src's slot
+   src_slot, map[b] for dst block b (dst's nblocks entries).  A copy keeps the
+   block; a refined block evaluates its parent's element polynomials at its own
+   nodes; a coarsened block is the exact L2 projection R = M_c^-1 I^T M_f of its
+   children.  Both keep the quadrature integral sum w u of each variable.  The
+   element-local operators of TreeAMR's regrid! that blast.jl:336-348 calls;
+   the truncation the reference applies after it is twoldg_truncate.  dst and
+   src must share D, P, E, precision and device; src is not written. */
+int twoldg_transfer(twoldg_problem* dst, int dst_slot, twoldg_problem* src, int src_slot,
+                    const twoldg_transfer_block* map);
+
 /* --- host-only inspection --------------------------------------------------- */
 /* The exchange operator the problem would build, on the host, without touching
    a GPU: rows (b, d, s, t) in order (b*D + d)*2 + s then t; CSR with entries

@@ -66,6 +66,7 @@ struct Element {
   double trunc[9][9];
   double mode[9], dual[9];
   double leg[9][9];  // L_k(xi_i): [i][k]
+  double duals[9][9];  // [m][j]: c_m = sum_j duals[m][j] u_j along a line
 };
 
 static void legendre_q(int P, qf x, qf* L) {
@@ -127,6 +128,14 @@ static void build_element(int P, Element& e) {
     for (int j = 0; j < P1; ++j)
       e.trunc[i][j] = (double)((i == j ? qf(1) : qf(0)) - LP[i] * w[j] * LP[j] / gamma);
     for (int k = 0; k < P1; ++k) e.leg[i][k] = (double)Lall[i][k];
+  }
+  // The Legendre coefficients by discrete orthogonality under the
+  // Gauss-Lobatto rule, sum_i w_i L_m L_k = 0 for m != k <= P
+  // (refinement.jl:45-54): c_m = sum_j w_j L_m(xi_j) u_j / sum_i w_i L_m(xi_i)^2.
+  for (int m = 0; m < P1; ++m) {
+    qf nrm = 0;
+    for (int i = 0; i < P1; ++i) nrm += w[i] * Lall[i][m] * Lall[i][m];
+    for (int j = 0; j < P1; ++j) e.duals[m][j] = (double)(w[j] * Lall[j][m] / nrm);
   }
   e.offsets[0] = 0;
   e.offsets[P] = P1;
@@ -1987,6 +1996,186 @@ void orc_rk4(void* p, double* y, double t0, double dt, int64_t nsteps) {
     for (int64_t i = 0; i < n; ++i) y[i] = y[i] + dt * (b1 * k1[i] + b2 * k2[i] + b3 * k3[i] + b4 * k4[i]);
     truncate_state(*pr, y);
   }
+}
+
+
+// A brick of explicit leaves (level, c0, c1, c2 each), in any order: the forest
+// after a regrid, built here from its keys alone.
+void* orc_brick_leaves(int D, int P, int E, int roots, double L, int reflecting, int truncate,
+                       int64_t nleaves, const int64_t* leaves) {
+  auto* pr = new Problem();
+  pr->D = D;
+  pr->P = P;
+  pr->P1 = P + 1;
+  pr->E = E;
+  pr->N = E * (P + 1);
+  build_element(P, pr->el);
+  pr->geometry = 0;
+  pr->npatch = 1;
+  for (int d = 0; d < 3; ++d) {
+    pr->proots[0][d] = d < D ? roots : 1;
+    pr->plo[0][d] = 0;
+    pr->phi_[0][d] = d < D ? L : 1;
+    pr->periodic[d] = !reflecting;
+  }
+  pr->reflecting = reflecting != 0;
+  pr->truncate = truncate != 0;
+  for (int64_t i = 0; i < nleaves; ++i) {
+    const int64_t* q = leaves + 4 * i;
+    pr->leaves.push_back(Leaf{0, (int)q[0], {q[1], q[2], q[3]}});
+  }
+  finalize_leaves(*pr);
+  resolve_faces(*pr);
+  allocate_work(*pr);
+  return pr;
+}
+
+// The modal amplitudes of variable var of y (refinement.jl:56-101): per element
+// post * max over its lines and the modes q of ((weights[q]/A) |c_modes[q]|)^pows[q],
+// A = max |var| over the forest, zero when A is.  tau: e_1 fastest, block slowest.
+void orc_modes(void* pv, const double* y, int var, int nq, const int* modes,
+               const double* weights, const double* pows, double post, double* tau) {
+  const Problem& pr = *(const Problem*)pv;
+  const int D = pr.D, N = pr.N, P1 = pr.P1, E = pr.E;
+  double peak = 0;
+  for (int64_t b = 0; b < pr.nb; ++b)
+    for (int64_t l = 0; l < pr.nn; ++l) peak = std::max(peak, std::fabs(y[(b * 2 + var) * pr.nn + l]));
+  const double scale = peak == 0 ? 0 : 1 / peak;
+  const int64_t ne = ipow(E, D);
+  for (int64_t b = 0; b < pr.nb; ++b) {
+    const double* a = y + (b * 2 + var) * pr.nn;
+    for (int64_t e = 0; e < ne; ++e) {
+      int64_t J[3] = {0, 0, 0};
+      for (int d = 0, r = (int)e; d < D; ++d, r /= E) J[d] = r % E;
+      double best = 0;
+      for (int d = 0; d < D; ++d) {
+        // every line of the element along d: the other coordinates over its nodes
+        const int64_t nlines = ipow(P1, D - 1);
+        for (int64_t t = 0; t < nlines; ++t) {
+          int64_t I[3] = {0, 0, 0};
+          for (int k = 0, r = (int)t; k < D; ++k) {
+            if (k == d) continue;
+            I[k] = J[k] * P1 + r % P1;
+            r /= P1;
+          }
+          for (int q = 0; q < nq; ++q) {
+            double c = 0;
+            for (int j = 0; j < P1; ++j) {
+              I[d] = J[d] * P1 + j;
+              c += pr.el.duals[modes[q]][j] * a[I[0] + N * (I[1] + (int64_t)N * I[2])];
+            }
+            best = std::max(best, std::pow(weights[q] * scale * std::fabs(c), pows[q]));
+          }
+        }
+      }
+      tau[b * ne + e] = post * best;
+    }
+  }
+}
+
+// The state of a regrid, by keys: every leaf of dst from src, a copy where src
+// holds the same leaf, the parent's polynomial at the leaf's nodes where src
+// holds the parent, and the L2 projection R = M_c^-1 I^T M_f of the children
+// where src holds those (TreeAMR regrid! with ElementLocal operators).
+// Returns the number of dst leaves of each kind in counts[3], or -1 where a
+// leaf has no origin.
+int orc_transfer(void* dv, double* ydst, void* sv, const double* ysrc, int64_t* counts) {
+  const Problem& pd = *(const Problem*)dv;
+  const Problem& ps = *(const Problem*)sv;
+  const int D = pd.D, N = pd.N, P1 = pd.P1, H = pd.E / 2;
+  const int64_t nn = pd.nn;
+  counts[0] = counts[1] = counts[2] = 0;
+  for (int64_t b = 0; b < pd.nb; ++b) {
+    const Leaf& L = pd.leaves[b];
+    int64_t sb;
+    auto node = [&](int64_t l, int64_t I[3]) {
+      I[0] = l % N;
+      I[1] = D >= 2 ? (l / N) % N : 0;
+      I[2] = D >= 3 ? l / ((int64_t)N * N) : 0;
+    };
+    if (is_leaf(ps, L.patch, L.level, L.c, &sb)) {
+      ++counts[0];
+      for (int v = 0; v < 2; ++v)
+        for (int64_t l = 0; l < nn; ++l) ydst[(b * 2 + v) * nn + l] = ysrc[(sb * 2 + v) * nn + l];
+      continue;
+    }
+    int64_t pc[3] = {L.c[0] >> 1, L.c[1] >> 1, L.c[2] >> 1};
+    if (L.level > 0 && is_leaf(ps, L.patch, L.level - 1, pc, &sb)) {
+      ++counts[1];
+      for (int64_t l = 0; l < nn; ++l) {
+        int64_t I[3];
+        node(l, I);
+        int64_t ep[3] = {0, 0, 0};
+        double xr[3] = {0, 0, 0};
+        for (int d = 0; d < D; ++d) {
+          const int64_t ec = I[d] / P1, j = I[d] % P1, o = L.c[d] & 1;
+          ep[d] = o * H + ec / 2;
+          xr[d] = (pd.el.nodes[j] + 2 * (ec % 2) - 1) / 2;
+        }
+        for (int v = 0; v < 2; ++v) {
+          double val = 0;
+          for (int64_t k = 0; k < ipow(P1, D); ++k) {
+            double w = 1;
+            int64_t S[3] = {0, 0, 0};
+            for (int d = 0, r = (int)k; d < D; ++d, r /= P1) {
+              w *= lagrange(pd.el, r % P1, xr[d]);
+              S[d] = ep[d] * P1 + r % P1;
+            }
+            val += w * ysrc[(sb * 2 + v) * nn + S[0] + N * (S[1] + (int64_t)N * S[2])];
+          }
+          ydst[(b * 2 + v) * nn + l] = val;
+        }
+      }
+      continue;
+    }
+    int64_t ch[8];
+    bool all = true;
+    for (int c = 0; c < (1 << D) && all; ++c) {
+      int64_t cc[3] = {0, 0, 0};
+      for (int d = 0; d < D; ++d) cc[d] = 2 * L.c[d] + ((c >> d) & 1);
+      all = is_leaf(ps, L.patch, L.level + 1, cc, &ch[c]);
+    }
+    if (!all) return -1;
+    ++counts[2];
+    for (int64_t l = 0; l < nn; ++l) {
+      int64_t I[3];
+      node(l, I);
+      int64_t ic[3] = {0, 0, 0}, eb[3] = {0, 0, 0};
+      int c = 0;
+      for (int d = 0; d < D; ++d) {
+        const int64_t e = I[d] / P1, o = e / H;
+        ic[d] = I[d] % P1;
+        eb[d] = 2 * (e - o * H);
+        c |= (int)o << d;
+      }
+      for (int v = 0; v < 2; ++v) {
+        double val = 0;
+        for (int64_t k = 0; k < ipow(2 * P1, D); ++k) {
+          double w = 1;
+          int64_t S[3] = {0, 0, 0};
+          for (int d = 0, r = (int)k; d < D; ++d, r /= 2 * P1) {
+            const int fj = r % (2 * P1), f = fj / P1, jj = fj % P1;
+            const double x = (pd.el.nodes[jj] + 2 * f - 1) / 2;
+            w *= lagrange(pd.el, (int)ic[d], x) * pd.el.weights[jj] / (2 * pd.el.weights[ic[d]]);
+            S[d] = (eb[d] + f) * P1 + jj;
+          }
+          val += w * ysrc[(ch[c] * 2 + v) * nn + S[0] + N * (S[1] + (int64_t)N * S[2])];
+        }
+        ydst[(b * 2 + v) * nn + l] = val;
+      }
+    }
+  }
+  return 0;
+}
+
+// The Legendre duals of the element, (P+1, P+1) [mode][node].
+int orc_duals(int P, double* out) {
+  if (P < 1 || P > 8) return -1;
+  Element e;
+  build_element(P, e);
+  for (int m = 0; m <= P; ++m)
+    for (int j = 0; j <= P; ++j) out[m * (P + 1) + j] = e.duals[m][j];
+  return 0;
 }
 
 int orc_threads(void) { return omp_get_max_threads(); }

@@ -63,6 +63,13 @@ def lib():
         L.orc_timestep.restype = ctypes.c_double
         L.orc_rk4.argtypes = [_dp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_int64]
         L.orc_threads.restype = ctypes.c_int
+        L.orc_brick_leaves.argtypes = [ctypes.c_int] * 4 + [ctypes.c_double, ctypes.c_int,
+                                                             ctypes.c_int, ctypes.c_int64, _dp]
+        L.orc_brick_leaves.restype = _dp
+        L.orc_modes.argtypes = [_dp, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp,
+                                ctypes.c_double, _dp]
+        L.orc_transfer.argtypes = [_dp, _dp, _dp, _dp, _dp]
+        L.orc_duals.argtypes = [ctypes.c_int, _dp]
         _LIB = L
     return _LIB
 
@@ -86,6 +93,13 @@ def element(P):
         res[name] = out[k:k + n].reshape(shape).copy()
         k += n
     return res
+
+
+def duals(P):
+    """The Legendre duals (P+1, P+1) [mode][node] in binary128 rounded once."""
+    out = np.zeros((P + 1) * (P + 1))
+    lib().orc_duals(P, _ptr(out))
+    return out.reshape(P + 1, P + 1)
 
 
 class Problem:
@@ -125,6 +139,40 @@ class Problem:
         """The four-patch annulus with its diagonal metric (geometry.jl:176-268)."""
         h = lib().orc_annulus(P, E, roots[0], roots[1], float(r0), float(r1), int(truncate))
         return cls(h, 2, P, E, 2)
+
+    @classmethod
+    def brick_leaves(cls, D, P, E, leaves, roots=4, L=1.0, reflecting=False, truncate=True):
+        """A brick forest of explicit leaves, (level, corner) pairs in any order."""
+        a = np.zeros((len(leaves), 4), dtype=np.int64)
+        for i, (lev, c) in enumerate(leaves):
+            a[i, 0] = lev
+            a[i, 1:1 + len(c)] = c
+        h = lib().orc_brick_leaves(D, P, E, roots, float(L), int(reflecting), int(truncate),
+                                   len(leaves), _ptr(a))
+        return cls(h, D, P, E, 0)
+
+    def modes(self, y, var, modes, weights, pows, post):
+        """Per element post * max over lines and q of ((weights[q]/A) |c_modes[q]|)^pows[q]
+        of variable var (0 = u, 1 = v), shape (nb, E^D) with e_1 fastest
+        (refinement.jl:56-101)."""
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        m = np.asarray(modes, dtype=np.int32)
+        w = np.asarray(weights, dtype=np.float64)
+        pw = np.asarray(pows, dtype=np.float64)
+        tau = np.zeros((self.nb, self.E ** self.D))
+        lib().orc_modes(self.h, _ptr(y), int(var), len(m), _ptr(m), _ptr(w), _ptr(pw),
+                        float(post), _ptr(tau))
+        return tau
+
+    def transfer_from(self, src, ysrc):
+        """This forest's state from src's ysrc by keys (copy, refine, coarsen); returns
+        the state and the number of leaves of each kind."""
+        ysrc = np.ascontiguousarray(ysrc, dtype=np.float64)
+        y = np.zeros(self.size())
+        counts = np.zeros(3, dtype=np.int64)
+        if lib().orc_transfer(self.h, _ptr(y), src.h, _ptr(ysrc), _ptr(counts)):
+            raise ValueError("a leaf of the regridded forest has no origin in the source")
+        return y, tuple(int(c) for c in counts)
 
     def dcoef(self):
         """(sqrt(g), A_1, A_2) at the nodes (nb, 3, nn) and the truncation pairs

@@ -1,0 +1,540 @@
+"""Adaptive refinement: the modal indicator, the block verdict, the regrid of a
+forest and the transfer of a state onto it (refinement.jl, blast.jl:336-348,
+TreeAMR regrid! with ElementLocal operators).
+
+The CPU tests pin the oracle's indicator and transfer against what the
+mathematics fixes and the reference's own measured values
+(adaptive_tests.jl:47-210): Legendre duals inverse to the Legendre table, a
+mode vanishing on data below its degree, the rate h^mode on smooth data, the
+blast Gaussian's published indicator values, sum w u kept by every transfer,
+exact refinement of the element polynomial, and R I = identity.  They also
+cover the host-side verdict and forest logic.  The GPU tests compare
+twoldg_modes and twoldg_transfer with the oracle and run the reference's regrid
+test sequence (adaptive_tests.jl:216-254) on the device.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+import twoldg as T
+from twoldg.problem import fill_at_nodes_forest
+from twoldg.regrid import _unbalanced
+
+gpu = pytest.mark.gpu
+
+
+# -- data (adaptive_tests.jl:11-35) ---------------------------------------------
+def smooth_data(D):
+    def f(x, v):
+        s = np.ones(x.shape[1])
+        for d in range(D):
+            s = s * np.sin(np.pi * (2 * x[d] + 3 * (d + 1) / 10 + v / 7))
+        return s
+    return f
+
+
+def poly_data(D, n):
+    def f(x, v):
+        s = np.ones(x.shape[1])
+        for d in range(D):
+            s = s * ((x[d] - 0.3) ** n + x[d] / 2)
+        return s
+    return f
+
+
+def middle_flags(forest):
+    def fn(b, k):
+        ext = forest.extent(k)
+        return T.Refine if all(0.25 < (e[0] + e[1]) / 2 < 0.75 for e in ext) else T.Keep
+    return T.flag_blocks(forest, fn)
+
+
+def coarsen_all(forest):
+    return T.flag_blocks(forest, lambda b, k: T.Coarsen if T.level(k) > 0 else T.Keep)
+
+
+def node_weights(forest):
+    """The quadrature weight of every node, (nb, N^D) (layout.jl:100-106)."""
+    D, N, P1 = forest.D, forest.N, forest.P + 1
+    wq = T.Element(forest.P).weights
+    w1 = wq[np.arange(N) % P1] * P1 / 2
+    w = np.ones(1)
+    for d in range(D):
+        w = np.multiply.outer(w1, w)     # first dimension fastest
+    w = w.reshape(-1)
+    return np.outer(forest.spacings() ** D, w)
+
+
+def total_mass(forest, y, var):
+    nb, nn = forest.nblocks, forest.N ** forest.D
+    return float(np.sum(node_weights(forest) * y.reshape(nb, 2, nn)[:, var - 1, :]))
+
+
+def oracle_of(forest, truncate=True):
+    leaves = [(lf[1], lf[2:2 + forest.D]) for lf in forest.leaves]
+    roots = forest.patches[0][0][0]
+    o = O.Problem.brick_leaves(forest.D, forest.P, forest.E, leaves, roots=roots,
+                               L=forest.patches[0][2][0] - forest.patches[0][1][0],
+                               reflecting=forest.reflecting, truncate=truncate)
+    # the oracle orders the blocks as the forest does: the state layouts agree
+    assert [(b[0], b[1], b[2]) for b in o.blocks()] == \
+        [(lf[0], lf[1], tuple(lf[2:2 + forest.D])) for lf in forest.leaves]
+    return o
+
+
+def oracle_indicator(o, y, mode, var=1):
+    M = mode
+    ms = list(range(1, M + 1))
+    return o.modes(y, var - 1, ms, [T.doublefactorial(m) for m in ms], [M / m for m in ms],
+                   1 / T.doublefactorial(M))
+
+
+def oracle_amplitude(o, y, mode, var=1):
+    return o.modes(y, var - 1, [mode], [1.0], [1.0], 1.0)
+
+
+# ---------------------------------------------------------------------------
+# The oracle, pinned.
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("P", range(1, 9))
+def test_duals_invert_the_legendre_table(P):
+    d = O.duals(P)
+    x = O.element(P)["nodes"]
+    L = np.polynomial.legendre.legvander(x, P)          # [node][mode]
+    assert np.allclose(d @ L, np.eye(P + 1), atol=1e-13)
+    assert np.array_equal(d[P], O.element(P)["topdual"])
+
+
+def test_oracle_tables_are_the_library_tables():
+    from twoldg.element import _table
+    for P in range(1, 9):
+        assert np.array_equal(O.duals(P), _table(P, "duals"))
+
+
+@pytest.mark.parametrize("D,P", [(1, 3), (1, 4), (2, 3), (2, 4)])
+def test_oracle_mode_vanishes_below_its_degree(D, P):
+    el = T.Element(P)
+    forest = T.wave_forest(D, P, 2, roots=4, refined=True)
+    o = oracle_of(forest)
+    for mode in (P - 1, P):
+        y = fill_at_nodes_forest(forest, poly_data(D, mode - 1), el)
+        assert oracle_amplitude(o, y, mode).max() < 1e-13
+        assert oracle_indicator(o, y, mode).max() > 1e-9
+        y = fill_at_nodes_forest(forest, poly_data(D, mode), el)
+        assert oracle_amplitude(o, y, mode).max() > 1e-9
+        y = fill_at_nodes_forest(forest, lambda x, v: 2.5 + v + 0 * x[0], el)
+        assert oracle_indicator(o, y, mode).max() < 1e-13
+
+
+@pytest.mark.parametrize("D,P", [(1, 3), (1, 4), (2, 3), (2, 4)])
+def test_oracle_indicator_falls_as_h_to_the_mode(D, P):
+    # Measured by the reference: 1D rates 2.00-4.00 for modes 2-4, 2D
+    # 1.95-3.96; the indicator equals the top mode's amplitude to 1e-12.
+    el = T.Element(P)
+    for mode in (P - 1, P):
+        hs, ts, as_ = [], [], []
+        for roots in ((4, 8, 16, 32) if D == 1 else (2, 4, 8)):
+            forest = T.wave_forest(D, P, 2, roots=roots)
+            o = oracle_of(forest)
+            y = fill_at_nodes_forest(forest, smooth_data(D), el)
+            hs.append(forest.minimum_spacing())
+            ts.append(oracle_indicator(o, y, mode).max())
+            as_.append(oracle_amplitude(o, y, mode).max())
+        assert abs(T.convergence_rate(hs, ts) - mode) < 0.06
+        assert abs(T.convergence_rate(hs, as_) - mode) < 0.06
+        assert np.allclose(ts, as_, rtol=1e-12, atol=0)
+
+
+def test_oracle_indicator_scale_and_weights():
+    el = T.Element(3)
+    forest = T.wave_forest(2, 3, 2, roots=2)
+    o = oracle_of(forest)
+    y = fill_at_nodes_forest(forest, smooth_data(2), el)
+    tau = oracle_indicator(o, y, 2)
+    assert np.allclose(oracle_indicator(o, 1000 * y, 2), tau, rtol=1e-14, atol=0)
+    a = [oracle_amplitude(o, y, m) for m in (1, 2, 3)]
+    want = np.maximum(a[2], np.maximum((3 * a[1]) ** 1.5 / 15, a[0] ** 3 / 15))
+    assert np.allclose(oracle_indicator(o, y, 3), want, rtol=1e-13, atol=0)
+    assert not np.any(oracle_indicator(o, 0 * y, 2))
+
+
+def test_oracle_indicator_on_the_blast_gaussian():
+    # adaptive_tests.jl:112-140: the published values at P = 3, roots 2 to 32,
+    # and the orderings at P = 3 and 4.
+    sigma = 0.08
+    for P in (3, 4):
+        el = T.Element(P)
+        ts, as_ = [], []
+        for roots in (2, 4, 8, 16, 32):
+            forest = T.wave_forest(2, P, 2, roots=roots)
+            o = oracle_of(forest)
+            y = fill_at_nodes_forest(forest, T.blast_initial(1.0, (0.5, 0.5), sigma), el)
+            ts.append(oracle_indicator(o, y, P - 1).max())
+            as_.append(oracle_amplitude(o, y, P - 1).max())
+        assert min(ts[:2]) > max(ts[2:])
+        assert all(ts[i] >= ts[i + 1] for i in range(1, 4))
+        if P == 3:
+            assert min(as_[:2]) < max(as_[2:])
+            assert np.allclose(ts, [2.17e-1, 9.58e-2, 5.74e-2, 2.21e-2, 6.14e-3], rtol=5e-3)
+        assert ts[-1] == as_[-1]
+
+
+TRANSFER_CASES = [(1, 3), (2, 3), (3, 2)]
+
+
+@pytest.mark.parametrize("D,P", TRANSFER_CASES)
+def test_oracle_transfer_keeps_the_integral(D, P, rough):
+    forest = T.wave_forest(D, P, 2, roots=4, refined=True)
+    o = oracle_of(forest)
+    y = rough(o.size(), 7)
+    coarse, changed = T.regrid_forest(forest, coarsen_all(forest))
+    assert changed and coarse.maxlevel() == 0
+    oc = oracle_of(coarse)
+    yc, counts = oc.transfer_from(o, y)
+    assert counts[2] > 0 and counts[1] == 0
+    for v in (1, 2):
+        assert math.isclose(total_mass(coarse, yc, v), total_mass(forest, y, v), rel_tol=1e-13)
+    fine, changed = T.regrid_forest(coarse, middle_flags(coarse))
+    assert changed and fine.maxlevel() == 1
+    of = oracle_of(fine)
+    yf, counts = of.transfer_from(oc, yc)
+    assert counts[1] > 0 and counts[2] == 0
+    for v in (1, 2):
+        assert math.isclose(total_mass(fine, yf, v), total_mass(forest, y, v), rel_tol=1e-13)
+
+
+@pytest.mark.parametrize("D,P", TRANSFER_CASES)
+def test_oracle_transfer_is_exact_on_the_element_space(D, P):
+    # Refining samples the parent's polynomial: data of degree <= P in every
+    # coordinate inside each coarse element arrive exactly at the fine nodes.
+    el = T.Element(P)
+    coarse = T.wave_forest(D, P, 2, roots=2)
+    flags = [T.Refine] * coarse.nblocks
+    flags[0] = T.Keep
+    fine, _ = T.regrid_forest(coarse, flags)
+    f = poly_data(D, P)
+    oc, of = oracle_of(coarse), oracle_of(fine)
+    yc = fill_at_nodes_forest(coarse, f, el)
+    yf, counts = of.transfer_from(oc, yc)
+    want = fill_at_nodes_forest(fine, f, el)
+    assert counts[1] > 0
+    assert np.allclose(yf, want, rtol=0, atol=1e-13 * np.abs(want).max())
+    # Coarsening recovers data of degree <= P - 1 per coordinate: there the
+    # Gauss-Lobatto sums in R = M_c^-1 I^T M_f integrate exactly, so R I = 1.
+    # At degree P they do not, which is why v's top mode returns.
+    for n, exact in ((P - 1, True), (P, False)):
+        g = poly_data(D, n)
+        yc = fill_at_nodes_forest(coarse, g, el)
+        yf, _ = of.transfer_from(oc, yc)
+        yb, counts = oc.transfer_from(of, yf)
+        assert counts[2] > 0
+        err = np.abs(yb - yc).max() / np.abs(yc).max()
+        assert (err < 1e-13) == exact
+
+
+def test_oracle_transfer_refuses_an_unrelated_forest():
+    # leaves two levels below every leaf of the source have no origin
+    a = T.wave_forest(2, 3, 2, roots=4)
+    two = T.wave_forest(2, 3, 2, roots=4, refined=[((0.25, 0.75),) * 2, ((0.375, 0.625),) * 2])
+    with pytest.raises(ValueError, match="no origin"):
+        oracle_of(two).transfer_from(oracle_of(a), np.zeros(oracle_of(a).size()))
+    with pytest.raises(ValueError, match="more than one level"):
+        T.transfer_map(a, two)
+
+
+# ---------------------------------------------------------------------------
+# The verdict and the forest (host).
+# ---------------------------------------------------------------------------
+def test_refine_mark_is_the_block_verdict():
+    # adaptive_tests.jl:172-195, with the box in 0-based, end-exclusive ranges.
+    el = T.Element(3)
+    k0 = (0, 0, 0, 0, 0)
+    k1 = (0, 1, 0, 0, 0)
+    tau = np.zeros((2, 2, 2))
+    tau[1, 0, 0] = 0.5
+    tau[0, 1, 1] = 0.005
+    tols = dict(refine_tol=0.01, coarsen_tol=0.001)
+    assert T.refine_mark(tau, el, 0, k0, **tols, maxlevel_cap=1) == (T.Refine, (range(4, 8), range(0, 4)))
+    assert T.refine_mark(tau, el, 0, k1, **tols, maxlevel_cap=1) == (T.Keep, (range(4, 8), range(0, 4)))
+    assert T.refine_mark(tau, el, 1, k1, **tols, maxlevel_cap=2) == (T.Keep, (range(0, 4), range(4, 8)))
+    tau[1, 1, 1] = 0.002
+    assert T.refine_mark(tau, el, 1, k1, **tols, maxlevel_cap=2) == (T.Keep, (range(0, 8), range(4, 8)))
+    tau[:, :, 1] = 0.0005
+    assert T.refine_mark(tau, el, 1, k1, **tols, maxlevel_cap=2) == T.Coarsen
+    assert T.refine_mark(tau, el, 1, k0, **tols, maxlevel_cap=2) == T.Keep
+    with pytest.raises(ValueError, match="dead band"):
+        T.refine_mark(tau, el, 0, k0, refine_tol=0.01, coarsen_tol=0.01, maxlevel_cap=1)
+
+
+def test_refinement_buffer_and_mode_refusals():
+    forest = T.wave_forest(2, 3, 2, roots=4)
+    assert T.refinement_buffer(forest, 2, 0.05) == 8
+    with pytest.raises(ValueError, match="regrid more often"):
+        T.refinement_buffer(forest, 2, 0.1)
+    with pytest.raises(ValueError, match="Mode 0 is the element's mean"):
+        T.check_mode(0, 3)
+    with pytest.raises(ValueError, match="not a Legendre mode"):
+        T.check_mode(4, 3)
+    assert [T.doublefactorial(m) for m in range(5)] == [1, 1, 3, 15, 105]
+
+
+def test_regrid_forest_coarsens_sibling_groups_and_keeps_balance():
+    forest = T.wave_forest(2, 3, 2, roots=4, refined=[((0.25, 0.75),) * 2, ((0.375, 0.625),) * 2])
+    assert forest.maxlevel() == 2
+    # coarsening only the level-1 blocks cannot go through where level-2 blocks
+    # would end up next to level-0 ones
+    fl = T.flag_blocks(forest, lambda b, k: T.Coarsen if T.level(k) == 1 else T.Keep)
+    g, changed = T.regrid_forest(forest, fl)
+    assert not _unbalanced(g)
+    assert g.maxlevel() == 2
+    # coarsening everything takes it down one level per regrid
+    g, changed = T.regrid_forest(forest, coarsen_all(forest))
+    assert changed and g.maxlevel() == 1 and not _unbalanced(g)
+    g, changed = T.regrid_forest(g, coarsen_all(g))
+    assert changed and g.maxlevel() == 0 and g.nblocks == 16
+    # an incomplete sibling group stays
+    fl = coarsen_all(forest)
+    first = next(b for b, k in enumerate(forest.leaves) if T.level(k) == 2)
+    fl[first] = T.Keep
+    g, _ = T.regrid_forest(forest, fl)
+    assert forest.leaves[first] in g.index
+    # nothing flagged: nothing changes
+    g, changed = T.regrid_forest(forest, [T.Keep] * forest.nblocks)
+    assert not changed and g.leaves == forest.leaves
+
+
+def test_regrid_buffer_recruits_neighbours():
+    forest = T.wave_forest(2, 3, 2, roots=4)
+    N = forest.N
+    b = forest.index[(0, 0, 1, 1, 0)]
+    # a box at the block's high-x edge: with a buffer it reaches the block to its
+    # right, without one it does not
+    fl = [T.Keep] * forest.nblocks
+    fl[b] = (T.Refine, (range(N - 2, N), range(2, 4)))
+    g0, _ = T.regrid_forest(forest, fl, buffer=0)
+    g2, _ = T.regrid_forest(forest, fl, buffer=4)
+    right = (0, 0, 2, 1, 0)
+    assert right in g0.index and right not in g2.index
+    # a block at its target level that asked to coarsen is kept by a box
+    fine = T.wave_forest(2, 3, 2, roots=2)
+    fine = fine.with_leaves([(0, 1) + tuple(c) for c in
+                             [(i, j, 0) for i in range(4) for j in range(4)]])
+    fl = [T.Coarsen] * fine.nblocks
+    fl[fine.index[(0, 1, 1, 1, 0)]] = (T.Keep, (range(N - 2, N), range(0, N)))
+    g, _ = T.regrid_forest(fine, fl, buffer=4)
+    assert (0, 1, 2, 1, 0) in g.index      # recruited: its sibling group stays fine
+    with pytest.raises(ValueError, match="one ring"):
+        T.regrid_forest(forest, [T.Keep] * forest.nblocks, buffer=N + 1)
+
+
+def test_transfer_map_matches_the_oracle_by_keys(rough):
+    for D, P in TRANSFER_CASES:
+        forest = T.wave_forest(D, P, 2, roots=4, refined=True)
+        coarse, _ = T.regrid_forest(forest, coarsen_all(forest))
+        m = T.transfer_map(forest, coarse)
+        kinds = [m[i].kind for i in range(coarse.nblocks)]
+        _, counts = oracle_of(coarse).transfer_from(oracle_of(forest),
+                                                    np.zeros(oracle_of(forest).size()))
+        assert tuple(kinds.count(k) for k in (0, 1, 2)) == counts
+
+
+def test_regrid_refuses_multipatch_forests():
+    from twoldg.geometry import sevenpatch_forest
+    for forest in (sevenpatch_forest(2, 2)[0], T.annulus_forest(3, 2)[0]):
+        with pytest.raises(ValueError, match="box forests"):
+            T.regrid_forest(forest, [T.Keep] * forest.nblocks)
+
+
+# ---------------------------------------------------------------------------
+# The device against the oracle.
+# ---------------------------------------------------------------------------
+MODE_CASES = [(1, 3, 4), (2, 3, 4), (2, 4, 2), (3, 2, 2), (3, 3, 2)]
+
+
+@gpu
+@pytest.mark.parametrize("D,P,roots", MODE_CASES)
+@pytest.mark.parametrize("precision", ["f64", "f32c"])
+def test_gpu_modes_match_the_oracle(D, P, roots, precision, rough):
+    forest = T.wave_forest(D, P, 2, roots=roots, refined=roots >= 4)
+    o = oracle_of(forest)
+    p = T.WaveProblem(forest, truncate=False, precision=precision)
+    y = rough(p.state_len, 3)
+    p.upload(y)
+    yo = p.download() if precision == "f32c" else y
+    tol = 1e-13 if precision == "f64" else 1e-12
+    for var in (1, 2):
+        for mode in range(1, P + 1):
+            got = T.modal_indicator(p, mode=mode, var=var)
+            want = oracle_indicator(o, yo, mode, var)
+            assert got.shape == (2,) * D + (forest.nblocks,)
+            assert np.allclose(got.reshape(-1, order="F"), want.reshape(-1), rtol=tol, atol=0)
+        for mode in range(0, P + 1):
+            # f64 through the argument slot (a state vector); f32c from the state
+            # it holds, whose compensated value the download returns exactly
+            got = T.mode_amplitude(p, mode=mode, var=var,
+                                   u=yo if precision == "f64" else None)
+            want = oracle_amplitude(o, yo, mode, var)
+            assert np.allclose(got.reshape(-1, order="F"), want.reshape(-1), rtol=tol,
+                               atol=1e-15)
+    with pytest.raises(ValueError, match="not a Legendre mode"):
+        T.mode_amplitude(p, mode=P + 1)
+    with pytest.raises(ValueError, match="Mode 0"):
+        T.modal_indicator(p, mode=0)
+    p.upload(0 * y)
+    assert not np.any(T.modal_indicator(p, mode=1))
+    p.close()
+
+
+@gpu
+@pytest.mark.parametrize("D,P", TRANSFER_CASES + [(3, 3)])
+@pytest.mark.parametrize("precision", ["f64", "f32c"])
+def test_gpu_transfer_matches_the_oracle(D, P, precision, rough):
+    forest = T.wave_forest(D, P, 2, roots=4, refined=True)
+    p = T.WaveProblem(forest, precision=precision)
+    y = rough(p.state_len, 11)
+    p.upload(y)
+    y = p.download()
+    tol = 1e-13 if precision == "f64" else 4e-7
+    q, changed = T.regrid(p, coarsen_all(forest))
+    assert changed and q.forest.maxlevel() == 0
+    want, _ = oracle_of(q.forest).transfer_from(oracle_of(forest), y)
+    got = q.download()
+    assert np.allclose(got, want, rtol=0, atol=tol * np.abs(want).max())
+    r, changed = T.regrid(q, middle_flags(q.forest))
+    assert changed
+    want2, _ = oracle_of(r.forest).transfer_from(oracle_of(q.forest), got)
+    got2 = r.download()
+    assert np.allclose(got2, want2, rtol=0, atol=tol * np.abs(want2).max())
+    for v in (1, 2):
+        m0 = total_mass(forest, y, v)
+        scale = np.sum(node_weights(forest) * np.abs(y.reshape(forest.nblocks, 2, -1)[:, v - 1]))
+        assert abs(total_mass(r.forest, got2, v) - m0) <= tol * scale
+    for x in (p, q, r):
+        x.close()
+
+
+@gpu
+@pytest.mark.parametrize("D,P", TRANSFER_CASES)
+def test_gpu_regrid_keeps_the_integral_and_the_truncation(D, P, rough):
+    # adaptive_tests.jl:216-254 on the device: coarsening everything keeps
+    # sum w u and brings v's top mode back; the truncation after it removes the
+    # mode and keeps the integral; refining the middle keeps both.
+    forest = T.wave_forest(D, P, 2, roots=4, refined=True)
+    p = T.WaveProblem(forest)
+    u = T.truncate_state(rough(p.state_len, 5), p)
+    p.upload(u)
+    mass = [total_mass(forest, u, v) for v in (1, 2)]
+
+    def conserved(q):
+        y = q.download()
+        return all(math.isclose(total_mass(q.forest, y, v), mass[v - 1], rel_tol=1e-13)
+                   for v in (1, 2))
+
+    q, changed = T.regrid(p, coarsen_all(forest))
+    assert changed and q.forest.maxlevel() == 0
+    assert conserved(q)
+    assert T.mode_amplitude(q, mode=P, var=2).max() > 1e-3
+    q.truncate_state()
+    assert T.mode_amplitude(q, mode=P, var=2).max() < 1e-13
+    assert conserved(q)
+    r, changed = T.regrid(q, middle_flags(q.forest))
+    assert changed and r.forest.maxlevel() == 1
+    assert conserved(r)
+    assert T.mode_amplitude(r, mode=P, var=2).max() < 1e-13
+    # a regrid that changes nothing hands back the same problem and state
+    s, changed = T.regrid(r, [T.Keep] * r.forest.nblocks)
+    assert s is r and not changed
+    for x in (p, q, r):
+        x.close()
+
+
+@gpu
+def test_gpu_indicator_on_the_blast_gaussian():
+    sigma = 0.08
+    ts = []
+    for roots in (2, 4, 8, 16, 32):
+        forest = T.wave_forest(2, 3, 2, roots=roots)
+        p = T.WaveProblem(forest, truncate=False)
+        p.upload(T.fill_at_nodes(T.blast_initial(1.0, (0.5, 0.5), sigma), p))
+        ts.append(T.modal_indicator(p, mode=2).max())
+        p.close()
+    assert np.allclose(ts, [2.17e-1, 9.58e-2, 5.74e-2, 2.21e-2, 6.14e-3], rtol=5e-3)
+
+
+@gpu
+def test_gpu_flags_and_initial_adaptation():
+    # adaptive_tests.jl:142-170, 197-205: refine_flags is refine_mark of every
+    # block; the initial adaptation brings the blast's centre to 0.39 sigma with
+    # the indicator and leaves it at 1.56 sigma with the single mode.
+    sigma = 0.08
+    el = T.Element(3)
+    tols = dict(refine_tol=0.01, coarsen_tol=0.001)
+    forest = T.wave_forest(2, 3, 2, roots=4)
+    p = T.WaveProblem(forest, truncate=False)
+    p.upload(T.fill_at_nodes(T.blast_initial(1.0, (0.5, 0.5), sigma), p))
+    flags = T.refine_flags(p, el, mode=2, **tols, maxlevel_cap=1)
+    tau = T.modal_indicator(p, el, mode=2)
+    assert flags == [T.refine_mark(tau, el, b, k, **tols, maxlevel_cap=1)
+                     for b, k in enumerate(forest.leaves)]
+    assert sum(1 for f in flags if isinstance(f, tuple) and f[0] == T.Refine) == 4
+    p.close()
+
+    def adapted(indicator):
+        P, roots = 3, 4
+
+        def flags(p):
+            tau = indicator(p, el, mode=P - 1)
+            return T.flag_blocks(p.forest, lambda b, k: T.refine_mark(
+                tau, el, b, k, refine_tol=3e-2, coarsen_tol=3e-5, maxlevel_cap=3))
+
+        f0 = T.wave_forest(2, P, 2, roots=roots)
+        f, passes, converged = T.adapt_to_initial_data(
+            f0, el, initial=T.blast_initial(1.0, (0.5, 0.5), sigma), flags=flags, buffer=2,
+            maxpasses=8)
+        centre = [k for k in f.leaves
+                  if all(e[0] <= 0.5 <= e[1] for e in f.extent(k))]
+        return 1 / (roots * 2 ** max(T.level(k) for k in centre) * 2) / sigma
+
+    assert abs(adapted(T.modal_indicator) - 0.39) <= 0.01
+    assert abs(adapted(T.mode_amplitude) - 1.56) <= 0.01
+
+
+@gpu
+@pytest.mark.parametrize("reflecting", [False, True])
+def test_gpu_regridded_problem_is_a_fresh_problem(reflecting, rough):
+    # adaptive_tests.jl:256-285: the problem a regrid returns steps like one
+    # built on its forest from scratch.
+    forest = T.wave_forest(2, 3, 2, roots=4, reflecting=reflecting)
+    p = T.WaveProblem(forest)
+    p.upload(T.truncate_state(rough(p.state_len, 9), p))
+    q, changed = T.regrid(p, middle_flags(forest))
+    assert changed
+    fresh = T.WaveProblem(q.forest)
+    u = q.download()
+    du1, du2 = np.empty_like(u), np.empty_like(u)
+    T.wave_rhs(du1, u, q, 0.0)
+    T.wave_rhs(du2, u, fresh, 0.0)
+    assert np.array_equal(du1, du2)
+    assert T.wave_energy(q, u) == T.wave_energy(fresh, u)
+    for x in (p, q, fresh):
+        x.close()
+
+
+@gpu
+def test_gpu_transfer_refusals(rough):
+    a = T.WaveProblem(T.wave_forest(2, 3, 2, roots=2))
+    b = T.WaveProblem(T.wave_forest(2, 4, 2, roots=2))
+    a.upload(rough(a.state_len, 1))
+    m = T.transfer_map(a.forest, a.forest)
+    import ctypes
+    from twoldg import _lib
+    with pytest.raises(ValueError, match="share D, P, E"):
+        _lib.check(_lib.lib().twoldg_transfer(b._h, 0, a._h, 0, ctypes.cast(m, ctypes.c_void_p)))
+    c = T.WaveProblem(a.forest)
+    m[0].src[0] = a.forest.nblocks
+    with pytest.raises(ValueError, match="out of range"):
+        _lib.check(_lib.lib().twoldg_transfer(c._h, 0, a._h, 0, ctypes.cast(m, ctypes.c_void_p)))
+    for x in (a, b, c):
+        x.close()

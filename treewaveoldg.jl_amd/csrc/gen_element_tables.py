@@ -20,6 +20,8 @@ interp[2*P1][P1]             coarse element's degree-P Lagrange basis at the
                              f in {0,1}, node j sits at (xi_j + 2f - 1)/2
 restrict[P1][2*P1]           R = M_c^-1 I^T M_f, the quadrature adjoint of
                              interp (the coarse ghost of a coarse-fine face)
+duals[P1][P1]                row k: the dual of Legendre mode k (its coefficient
+                             from nodal values); row P is topdual
 
 Run:  python gen_element_tables.py   (writes both files next to this one)
 """
@@ -92,6 +94,11 @@ def element(P):
     trunc = [[(1 if i == j else 0) - LP[i] * w[j] * LP[j] / gamma for j in range(P1)]
              for i in range(P1)]
     topdual = [w[j] * LP[j] / gamma for j in range(P1)]
+    # duals[k][j]: the coefficient of L_k in the element's polynomial through
+    # values u is sum_j duals[k][j] u_j (discrete orthogonality under the
+    # quadrature); the last row is topdual itself (element.jl:154-159)
+    duals = [[w[j] * Lall[j][k] / mp.fsum(w[i] * Lall[i][k] ** 2 for i in range(P1))
+              for j in range(P1)] for k in range(P)] + [topdual]
     offsets = [(P + 1) * (1 + x) / 2 for x in xi]
     offsets[0], offsets[P] = mp.mpf(0), mp.mpf(P1)
     interp = []
@@ -104,7 +111,7 @@ def element(P):
     return dict(nodes=xi, weights=w, offsets=offsets, stencil=W, trunc=trunc,
                 topmode=LP, topdual=topdual, legendre=[[Lall[i][k] for k in range(P1)]
                                                        for i in range(P1)],
-                interp=interp, restrict=restrict)
+                interp=interp, restrict=restrict, duals=duals)
 
 
 def f64hex(x):
@@ -121,7 +128,7 @@ def main():
     here = os.path.dirname(os.path.abspath(__file__))
     els = {P: element(P) for P in range(1, PMAX + 1)}
     names = ["nodes", "weights", "offsets", "stencil", "trunc", "topmode", "topdual",
-             "legendre", "interp", "restrict"]
+             "legendre", "interp", "restrict", "duals"]
     # C++ header: one struct per P, padded to the largest size.
     h = ["// Generated by gen_element_tables.py: OLDG reference-element tables,",
          "// built at 512 bits and rounded once to binary64.  Do not edit.",
@@ -136,6 +143,7 @@ def main():
          "  double legendre[9][9];   // [node][mode]",
          "  double interp[18][9];    // [fine node][coarse node]",
          "  double restrict_[9][18]; // [coarse node][fine node]",
+         "  double duals[9][9];      // [mode][node]",
          "};", "",
          f"inline constexpr ElementTables kElementTables[{PMAX}] = {{"]
     for P in range(1, PMAX + 1):
@@ -150,7 +158,7 @@ def main():
         h.append("  {" + f"{P}, " + ", ".join([
             vec(e["nodes"]), vec(e["weights"]), vec(e["offsets"]), mat(e["stencil"]),
             mat(e["trunc"]), vec(e["topmode"]), vec(e["topdual"]), mat(e["legendre"]),
-            mat(e["interp"]), mat(e["restrict"])]) + "},")
+            mat(e["interp"]), mat(e["restrict"]), mat(e["duals"])]) + "},")
     h += ["};", "", "}  // namespace twoldg", ""]
     with open(os.path.join(here, "element_tables.h"), "w") as fh:
         fh.write("\n".join(h))

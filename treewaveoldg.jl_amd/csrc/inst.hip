@@ -20,3 +20,10 @@ int TW_FN(twoldg_i_norms)(twoldg_problem* p, int a, int b, int mask, int pn, dou
                           double* part, double* red) {
   return dispatch_p<NormsF, TW_T, TW_D>(p, a, b, mask, pn, lp, li, part, red);
 }
+int TW_FN(twoldg_i_modes)(twoldg_problem* p, int slot, int var, ModeArgs M, double* tau) {
+  return dispatch_p<ModesF, TW_T, TW_D>(p, slot, var, M, tau);
+}
+int TW_FN(twoldg_i_transfer)(twoldg_problem* p, int dslot, const twoldg_problem* src, int sslot,
+                             const TransferBlock* map) {
+  return dispatch_p<TransferF, TW_T, TW_D>(p, dslot, src, sslot, map);
+}

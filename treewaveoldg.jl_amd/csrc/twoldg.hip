@@ -122,6 +122,24 @@ int twoldg_i_norms_f32_2(twoldg_problem*, int, int, int, int, double*, double*, 
                          double*);
 int twoldg_i_norms_f32_3(twoldg_problem*, int, int, int, int, double*, double*, double*,
                          double*);
+int twoldg_i_modes_f64_1(twoldg_problem*, int, int, twoldg_mode_args, double*);
+int twoldg_i_transfer_f64_1(twoldg_problem*, int, const twoldg_problem*, int,
+                             const twoldg_transfer_block*);
+int twoldg_i_modes_f64_2(twoldg_problem*, int, int, twoldg_mode_args, double*);
+int twoldg_i_transfer_f64_2(twoldg_problem*, int, const twoldg_problem*, int,
+                             const twoldg_transfer_block*);
+int twoldg_i_modes_f64_3(twoldg_problem*, int, int, twoldg_mode_args, double*);
+int twoldg_i_transfer_f64_3(twoldg_problem*, int, const twoldg_problem*, int,
+                             const twoldg_transfer_block*);
+int twoldg_i_modes_f32_1(twoldg_problem*, int, int, twoldg_mode_args, double*);
+int twoldg_i_transfer_f32_1(twoldg_problem*, int, const twoldg_problem*, int,
+                             const twoldg_transfer_block*);
+int twoldg_i_modes_f32_2(twoldg_problem*, int, int, twoldg_mode_args, double*);
+int twoldg_i_transfer_f32_2(twoldg_problem*, int, const twoldg_problem*, int,
+                             const twoldg_transfer_block*);
+int twoldg_i_modes_f32_3(twoldg_problem*, int, int, twoldg_mode_args, double*);
+int twoldg_i_transfer_f32_3(twoldg_problem*, int, const twoldg_problem*, int,
+                             const twoldg_transfer_block*);
 
 namespace {
 
@@ -131,6 +149,15 @@ namespace {
                     : p->D == 2 ? twoldg_i_##fn##_f64_2(__VA_ARGS__) : twoldg_i_##fn##_f64_3(__VA_ARGS__)) \
        : (p->D == 1 ? twoldg_i_##fn##_f32_1(__VA_ARGS__)                                   \
                     : p->D == 2 ? twoldg_i_##fn##_f32_2(__VA_ARGS__) : twoldg_i_##fn##_f32_3(__VA_ARGS__)))
+
+#define TW_DISPATCH_P(q, fn, ...)                                                         \
+  ((q)->precision == TWOLDG_F64                                                           \
+       ? ((q)->D == 1 ? twoldg_i_##fn##_f64_1(__VA_ARGS__)                                 \
+                      : (q)->D == 2 ? twoldg_i_##fn##_f64_2(__VA_ARGS__)                   \
+                                    : twoldg_i_##fn##_f64_3(__VA_ARGS__))                  \
+       : ((q)->D == 1 ? twoldg_i_##fn##_f32_1(__VA_ARGS__)                                 \
+                      : (q)->D == 2 ? twoldg_i_##fn##_f32_2(__VA_ARGS__)                   \
+                                    : twoldg_i_##fn##_f32_3(__VA_ARGS__)))
 
 void destroy(twoldg_problem* p) {
   if (!p) return;
@@ -696,6 +723,74 @@ int twoldg_timestep(const twoldg_problem* p, double cfl, double* out) {
 int twoldg_synchronize(twoldg_problem* p) {
   if (!p) return fail(TWOLDG_EINVAL, "no problem");
   HIP_TRY(hipStreamSynchronize(p->stream));
+  return 0;
+}
+
+int twoldg_modes(twoldg_problem* p, int slot, int var, int nmodes, const int32_t* modes,
+                 const double* weights, const double* pows, double post, double* tau) {
+  if (!p || !valid_slot(slot)) return fail(TWOLDG_EINVAL, "bad slot");
+  if (!p->uploaded[slot]) return fail(TWOLDG_ESTATE, "slot " + std::to_string(slot) + " holds no state");
+  if (var != 0 && var != 1)
+    return fail(TWOLDG_EINVAL, "var = " + std::to_string(var) +
+                                   " is not a variable of the state, which has 2 (0 = u, 1 = v)");
+  if (nmodes < 1 || nmodes > 8 || !modes || !weights || !pows || !tau)
+    return fail(TWOLDG_EINVAL, "the modes need 1 to 8 entries, their weights, powers and tau");
+  twoldg_mode_args M{};
+  M.nq = nmodes;
+  for (int q = 0; q < nmodes; ++q) {
+    if (modes[q] < 0 || modes[q] > p->P)
+      return fail(TWOLDG_EINVAL, "mode = " + std::to_string(modes[q]) +
+                                     " is not a Legendre mode of an element of order P = " +
+                                     std::to_string(p->P));
+    if (!(pows[q] > 0) || !std::isfinite(weights[q]))
+      return fail(TWOLDG_EINVAL, "a mode's power must be positive and its weight finite");
+    M.m[q] = modes[q];
+    M.wt[q] = weights[q];
+    M.pw[q] = pows[q];
+  }
+  M.post = post;
+  return TW_DISPATCH(modes, p, slot, var, M, tau);
+}
+
+int twoldg_transfer(twoldg_problem* dst, int dst_slot, twoldg_problem* src, int src_slot,
+                    const twoldg_transfer_block* map) {
+  if (!dst || !src || !map || !valid_slot(dst_slot) || !valid_slot(src_slot))
+    return fail(TWOLDG_EINVAL, "a transfer needs two problems, two slots and the map");
+  if (dst == src) return fail(TWOLDG_EINVAL, "a transfer goes between two problems");
+  if (dst->D != src->D || dst->P != src->P || dst->E != src->E ||
+      dst->precision != src->precision || dst->device != src->device)
+    return fail(TWOLDG_EINVAL,
+                "the regridded problem must share D, P, E, precision and device with the "
+                "problem it is regridded from");
+  if (!src->uploaded[src_slot])
+    return fail(TWOLDG_ESTATE, "slot " + std::to_string(src_slot) + " holds no state");
+  const int64_t nb = dst->sh.nb, ns = src->sh.nb, nc = int64_t(1) << dst->D;
+  for (int64_t b = 0; b < nb; ++b) {
+    const twoldg_transfer_block& m = map[b];
+    const int64_t nsrc = m.kind == TWOLDG_XFER_COARSEN ? nc : 1;
+    bool ok = m.kind >= TWOLDG_XFER_COPY && m.kind <= TWOLDG_XFER_COARSEN &&
+              (m.kind != TWOLDG_XFER_REFINE || (m.child >= 0 && m.child < nc));
+    for (int64_t c = 0; ok && c < nsrc; ++c) ok = m.src[c] >= 0 && m.src[c] < ns;
+    if (!ok)
+      return fail(TWOLDG_EINVAL, "transfer entry " + std::to_string(b) +
+                                     " names a kind, child or source block out of range");
+  }
+  int rc = ensure_slot(dst, dst_slot);
+  if (rc) return rc;
+  HIP_TRY(hipSetDevice(dst->device));
+  HIP_TRY(hipStreamSynchronize(src->stream));
+  twoldg_transfer_block* dmap = nullptr;
+  rc = dalloc(dst, &dmap, (size_t)nb);
+  if (rc) return rc;
+  hipError_t e = hipMemcpyAsync(dmap, map, nb * sizeof *map, hipMemcpyHostToDevice, dst->stream);
+  if (e == hipSuccess) {
+    rc = TW_DISPATCH_P(dst, transfer, dst, dst_slot, src, src_slot, dmap);
+    e = hipStreamSynchronize(dst->stream);
+  }
+  dfree(dmap);
+  if (rc) return rc;
+  if (e != hipSuccess) return fail(TWOLDG_EDEVICE, std::string("transfer: ") + hipGetErrorString(e));
+  dst->uploaded[dst_slot] = true;
   return 0;
 }
 

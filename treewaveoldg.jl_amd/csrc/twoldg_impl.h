@@ -31,6 +31,16 @@ using twoldg_detail::fail;
 using twoldg_detail::pool_alloc;
 using twoldg_detail::pool_free;
 
+// The modes a modal indicator reads (regrid.h): per mode q its number m[q], the
+// weight wt[q] and the power pw[q], and the factor post.  Outside the unnamed
+// namespace because it crosses from the ABI unit to the per-type units.
+struct twoldg_mode_args {
+  int nq;
+  int m[8];
+  double wt[8], pw[8];
+  double post;
+};
+
 namespace {
 
 #define HIP_TRY(expr)                                                                       \
@@ -762,6 +772,7 @@ __global__ void k_compensated_copy(int64_t n, const T* __restrict__ hi, const T*
 }
 
 #include "fused3.h"
+#include "regrid.h"
 
 }  // namespace
 
@@ -1446,6 +1457,45 @@ int norms_impl(twoldg_problem* p, int a, int b, int mask, int pnorm, double* lp,
   return 0;
 }
 
+// The modal amplitudes of variable var of a slot, one per element, into the host
+// array tau (blocks slowest, then e_D .. e_1).  The peak A = max |var| is the
+// norms' reduction, left on the device for the element kernel to read.
+template <class T, int D, int P1>
+int modes_impl(twoldg_problem* p, int slot, int var, ModeArgs M, double* tau) {
+  using L = Launch<T, D, P1>;
+  const T* y = (const T*)p->slot[slot];
+  const T* lo = slot == TWOLDG_SLOT_STATE ? (const T*)p->lo : nullptr;
+  L::norms(p, y, lo, nullptr, nullptr, 1 << var, 2, p->part, p->red);
+  int64_t ne = p->sh.nb;
+  for (int d = 0; d < D; ++d) ne *= p->E;
+  double* dtau = nullptr;
+  int rc = dalloc(p, &dtau, (size_t)ne);
+  if (rc) return rc;
+  hipLaunchKernelGGL((k_modes<T, D, P1>), dim3(grid_for(ne, 256)), dim3(256), 0, p->stream,
+                     p->sh, p->E, y, lo, var, M, (const double*)p->red + 2, dtau);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(tau, dtau, ne * sizeof(double), hipMemcpyDeviceToHost,
+                                          p->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
+  dfree(dtau);
+  if (e != hipSuccess) return fail(TWOLDG_EDEVICE, std::string("modes: ") + hipGetErrorString(e));
+  return 0;
+}
+
+// Each block of dst from its origin in src (copy, refine, coarsen): map is a
+// device array of dst->sh.nb entries, checked on the host.
+template <class T, int D, int P1>
+int transfer_impl(twoldg_problem* dst, int dslot, const twoldg_problem* src, int sslot,
+                  const TransferBlock* map) {
+  const T* s = (const T*)src->slot[sslot];
+  const T* slo = sslot == TWOLDG_SLOT_STATE ? (const T*)src->lo : nullptr;
+  T* lo = dslot == TWOLDG_SLOT_STATE ? (T*)dst->lo : nullptr;
+  hipLaunchKernelGGL((k_transfer<T, D, P1>), dim3((unsigned)dst->sh.nb), dim3(256), 0,
+                     dst->stream, dst->sh, dst->E, map, s, slo, (T*)dst->slot[dslot], lo);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
 // Run-time dispatch on (precision, D, P1).
 
 #define TWOLDG_DISPATCH_FN(name, impl)                                                     \
@@ -1461,6 +1511,8 @@ TWOLDG_DISPATCH_FN(Rk4F, rk4_steps)
 TWOLDG_DISPATCH_FN(TruncF, truncate_impl)
 TWOLDG_DISPATCH_FN(EnergyF, energy_impl)
 TWOLDG_DISPATCH_FN(NormsF, norms_impl)
+TWOLDG_DISPATCH_FN(ModesF, modes_impl)
+TWOLDG_DISPATCH_FN(TransferF, transfer_impl)
 
 template <template <class, int, int> class F, class T, int D, class... A>
 int dispatch_p(twoldg_problem* p, A... args) {

@@ -15,6 +15,10 @@ from .drivers import (wave_omega, wave_exact, reflect_omega, reflect_exact, gaus
 from .batch import Batch, gaussian_pulses
 from .annulus import (PolarAnnulus, annulus_forest, annulus_wavenumber, annulus_exact,
                       annulus_errors)
+from .regrid import (RegridFlag, Refine, Keep, Coarsen, level, doublefactorial, check_mode,
+                     check_thresholds, modal_indicator, mode_amplitude, refine_mark, flag_blocks,
+                     refine_flags, refinement_buffer, regrid_forest, transfer_map, regrid,
+                     adapt_to_initial_data, blast_initial)
 from . import _lib
 
 __all__ = [
@@ -26,5 +30,8 @@ __all__ = [
     "sevenpatch_exact", "wave_errors", "reflect_errors", "sevenpatch_errors",
     "convergence_rate", "Batch", "PolarAnnulus", "annulus_forest", "annulus_wavenumber",
     "annulus_exact", "annulus_errors",
-    "gaussian_pulses",
+    "gaussian_pulses", "RegridFlag", "Refine", "Keep", "Coarsen", "level", "doublefactorial",
+    "check_mode", "check_thresholds", "modal_indicator", "mode_amplitude", "refine_mark",
+    "flag_blocks", "refine_flags", "refinement_buffer", "regrid_forest", "transfer_map", "regrid",
+    "adapt_to_initial_data", "blast_initial",
 ]

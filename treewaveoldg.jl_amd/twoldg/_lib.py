@@ -15,6 +15,7 @@ F64, F32C = 0, 1
 FACE_SAME, FACE_COARSER, FACE_FINER, FACE_MIRROR, FACE_GLUE, FACE_DIRICHLET = 1, 2, 3, 4, 5, 6
 SLOT_STATE, SLOT_DU, SLOT_AUX, SLOT_ARG = 0, 1, 2, 3
 FLAG_GENERIC = 1
+XFER_COPY, XFER_REFINE, XFER_COARSEN = 0, 1, 2
 
 
 class Face(ctypes.Structure):
@@ -36,6 +37,11 @@ class Desc(ctypes.Structure):
                 ("hspeed", ctypes.c_double), ("flags", ctypes.c_int32),
                 ("pad", ctypes.c_int32), ("dcoef", ctypes.c_void_p),
                 ("lmode", ctypes.c_void_p), ("ldual", ctypes.c_void_p)]
+
+
+class TransferBlock(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("child", ctypes.c_int32),
+                ("src", ctypes.c_int64 * 8)]
 
 
 class TwoldgError(RuntimeError):
@@ -76,6 +82,9 @@ def lib():
                                       ctypes.c_int, _vp, _vp], ctypes.c_int),
             "twoldg_timestep": ([_vp, _dbl, _vp], ctypes.c_int),
             "twoldg_synchronize": ([_vp], ctypes.c_int),
+            "twoldg_modes": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp,
+                              _dbl, _vp], ctypes.c_int),
+            "twoldg_transfer": ([_vp, ctypes.c_int, _vp, ctypes.c_int, _vp], ctypes.c_int),
             "twoldg_exchange_plan": ([_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),
             "twoldg_node_coordinates": ([_vp, _vp, _vp, _vp], ctypes.c_int),
             "twoldg_rk4_timed": ([_vp, _dbl, _dbl, _i64, ctypes.c_int, _vp, _vp, _vp],
@@ -115,7 +124,7 @@ def exported_symbols():
             "twoldg_state_upload", "twoldg_state_download", "twoldg_state_copy", "twoldg_rhs",
             "twoldg_truncate", "twoldg_rk4", "twoldg_energy", "twoldg_norms",
             "twoldg_norms_noalloc", "twoldg_timestep",
-            "twoldg_synchronize", "twoldg_exchange_plan", "twoldg_node_coordinates",
+            "twoldg_synchronize", "twoldg_modes", "twoldg_transfer", "twoldg_exchange_plan", "twoldg_node_coordinates",
             "twoldg_rk4_timed",
             "twoldg_batch_create", "twoldg_batch_destroy", "twoldg_batch_upload",
             "twoldg_batch_download", "twoldg_batch_rk4", "twoldg_batch_info"]

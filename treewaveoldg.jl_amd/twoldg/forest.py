@@ -18,6 +18,8 @@ which is all the face-only coupling of an element set needs.
 (periodic wrap included), coarser, finer, mirror (reflecting wall), glue
 (seven-patch ball) and Dirichlet (the ball's outer sphere).
 """
+import copy
+
 import numpy as np
 
 from . import _lib
@@ -117,6 +119,20 @@ class Forest:
 
     def minimum_spacing(self):
         return float(self.spacings().min())
+
+    def maxlevel(self):
+        return max(lf[1] for lf in self.leaves)
+
+    def spacing(self, level):
+        """h at `level` in the first patch (TreeAMR's spacing(forest, level))."""
+        roots, lo, hi = self.patches[0]
+        return (hi[0] - lo[0]) / roots[0] / (1 << level) / self.N
+
+    def with_leaves(self, leaves):
+        """A forest over the same patches holding `leaves`."""
+        f = copy.copy(self)
+        f._set(list(leaves))
+        return f
 
     # -- refinement ----------------------------------------------------------
     def _refine(self, targets):

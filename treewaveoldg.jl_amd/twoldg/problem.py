@@ -163,6 +163,12 @@ class WaveProblem:
         _lib.check(_lib.lib().twoldg_state_download(self._h, slot, _ptr(out)))
         return out
 
+    def truncate_state(self):
+        """Truncate the state the problem holds (its state slot) in place; nothing when
+        the problem does not truncate (truncate_state!, evolution.jl:1383-1395)."""
+        if self.truncate:
+            _lib.check(_lib.lib().twoldg_truncate(self._h, _lib.SLOT_STATE))
+
     def info(self):
         n = ctypes.c_int64()
         b = ctypes.c_int64()
