@@ -538,3 +538,111 @@ def test_gpu_transfer_refusals(rough):
         _lib.check(_lib.lib().twoldg_transfer(c._h, 0, a._h, 0, ctypes.cast(m, ctypes.c_void_p)))
     for x in (a, b, c):
         x.close()
+
+
+# ---------------------------------------------------------------------------
+# The blast wave and the adaptive run (blast.jl, adaptive_tests.jl:286-380).
+# ---------------------------------------------------------------------------
+def test_blast_exact_solution_is_the_initial_data_and_its_derivative():
+    # The Hankel table reproduces the Gaussian at t = 0 to its quadrature and
+    # interpolation error, v(0) = 0, and v is the time derivative of u.
+    sigma = 0.08
+    ref = T.blast_reference(1.0, (0.5, 0.5), sigma, rmax=0.4 + 6 * sigma)
+    x = np.random.default_rng(0).random((2, 2000))
+    g = T.blast_initial(1.0, (0.5, 0.5), sigma)
+    f0 = T.blast_exact(ref, 0.0)
+    assert np.abs(f0(x, 1) - g(x, 1)).max() < 1e-5
+    assert not np.any(f0(x, 2))
+    t, dt = 0.13, 1e-4
+    du = (T.blast_exact(ref, t + dt)(x, 1) - T.blast_exact(ref, t - dt)(x, 1)) / (2 * dt)
+    v = T.blast_exact(ref, t)(x, 2)
+    assert np.abs(du - v).max() < 1e-5 * np.abs(v).max() + 1e-9
+
+
+def test_chunks():
+    assert T.chunk_count(0.33, 0.03) == 11
+    assert T.chunk_count(0.2, 0.05) == 4
+    assert T.chunk_count(0.21, 0.05) == 5
+    assert T.chunk_bounds(5, 5, 0.21, 0.05) == (0.2, 0.21)
+    assert T.chunk_bounds(4, 4, 0.2, 0.05)[1] == 0.2
+
+
+ADAPTIVE = dict(P=3, E=2, roots=4, maxlevel_cap=2, refine_tol=1e-2, coarsen_tol=1e-5,
+                chunk=0.05, t_end=0.2)
+
+
+@pytest.fixture(scope="module")
+def adaptive_run():
+    return T.adaptive_blast(**ADAPTIVE)
+
+
+@gpu
+def test_gpu_adaptive_blast_refines_the_ring_up_to_the_cap(adaptive_run):
+    r = adaptive_run
+    sigma, t = 0.08, ADAPTIVE["t_end"]
+    assert r["converged"]
+    assert r["maxlevel"] == ADAPTIVE["maxlevel_cap"]
+    assert r["nregrids"] >= 2
+    f = r["forest"]
+
+    def distance(a, b):
+        return math.hypot(a - 0.5, b - 0.5)
+
+    ring = []
+    for k in f.leaves:
+        ext = f.extent(k)
+        corners = [distance(a, b) for a in ext[0] for b in ext[1]]
+        if min(corners) <= t + sigma and max(corners) >= t - sigma:
+            ring.append(k)
+    assert ring and all(T.level(k) == ADAPTIVE["maxlevel_cap"] for k in ring)
+    coarse = [k for k in f.leaves if T.level(k) < ADAPTIVE["maxlevel_cap"]]
+    assert coarse
+    for k in coarse:
+        ext = f.extent(k)
+        assert distance((ext[0][0] + ext[0][1]) / 2, (ext[1][0] + ext[1][1]) / 2) > t + 2 * sigma
+
+
+@gpu
+def test_gpu_adaptive_blast_energy_changes_only_at_regrids(adaptive_run):
+    r = adaptive_run
+    ch = r["chunks"]
+    for c in ch:
+        assert 0 <= (c["energy_start"] - c["energy_end"]) / c["energy_start"] < 2e-9
+    for a, b in zip(ch[:-1], ch[1:]):
+        if a["regridded"]:
+            assert b["energy_start"] != a["energy_end"]
+            assert abs(b["energy_start"] - a["energy_end"]) < 1e-10 * r["energy0"]
+        else:
+            assert b["energy_start"] == a["energy_end"]
+    assert sum(c["regridded"] for c in ch) == r["nregrids"]
+    assert not ch[-1]["regridded"]
+    assert ch[-1]["t"] == ADAPTIVE["t_end"]
+
+
+@gpu
+def test_gpu_adaptive_blast_matches_the_uniform_fine_run(adaptive_run):
+    r = adaptive_run
+    u = T.uniform_blast(P=3, E=2, t_end=0.2, roots=4 * 2 ** 2)
+    assert r["err"] < 1.01 * u["err"]
+    assert r["l2"] < 1.01 * u["l2"]
+    assert r["node_steps"] < 0.82 * u["nodes"] * u["nsteps"]
+    assert r["h"] == u["h"]
+    deep = dict(ADAPTIVE, maxlevel_cap=3, chunk=0.025)
+    r = T.adaptive_blast(**deep)
+    u = T.uniform_blast(P=3, E=2, t_end=0.2, roots=4 * 2 ** 3)
+    assert r["maxlevel"] == 3
+    assert r["err"] < 1.01 * u["err"]
+    assert r["l2"] < 1.01 * u["l2"]
+    assert r["node_steps"] < 0.65 * u["nodes"] * u["nsteps"]
+
+
+def test_adaptive_blast_refusals():
+    with pytest.raises(ValueError, match="P = 1 has no mode"):
+        T.adaptive_blast(P=1, E=2, roots=4, maxlevel_cap=1, refine_tol=0.01,
+                         coarsen_tol=0.001, chunk=0.05)
+    with pytest.raises(ValueError, match="dead band"):
+        T.adaptive_blast(P=3, E=2, roots=4, maxlevel_cap=1, refine_tol=0.01,
+                         coarsen_tol=0.01, chunk=0.05)
+    with pytest.raises(ValueError, match="must be positive"):
+        T.adaptive_blast(P=3, E=2, roots=4, maxlevel_cap=1, refine_tol=0.01,
+                         coarsen_tol=0.001, chunk=0.0)

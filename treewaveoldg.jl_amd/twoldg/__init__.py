@@ -18,7 +18,9 @@ from .annulus import (PolarAnnulus, annulus_forest, annulus_wavenumber, annulus_
 from .regrid import (RegridFlag, Refine, Keep, Coarsen, level, doublefactorial, check_mode,
                      check_thresholds, modal_indicator, mode_amplitude, refine_mark, flag_blocks,
                      refine_flags, refinement_buffer, regrid_forest, transfer_map, regrid,
-                     adapt_to_initial_data, blast_initial)
+                     adapt_to_initial_data)
+from .blast import (blast_initial, blast_reference, blast_exact, uniform_blast, adaptive_blast,
+                    chunk_count, chunk_bounds)
 from . import _lib
 
 __all__ = [
@@ -33,5 +35,6 @@ __all__ = [
     "gaussian_pulses", "RegridFlag", "Refine", "Keep", "Coarsen", "level", "doublefactorial",
     "check_mode", "check_thresholds", "modal_indicator", "mode_amplitude", "refine_mark",
     "flag_blocks", "refine_flags", "refinement_buffer", "regrid_forest", "transfer_map", "regrid",
-    "adapt_to_initial_data", "blast_initial",
+    "adapt_to_initial_data", "blast_initial", "blast_reference", "blast_exact", "uniform_blast",
+    "adaptive_blast", "chunk_count", "chunk_bounds",
 ]

@@ -360,8 +360,16 @@ class Integrator:
             raise ValueError("dt must be positive")
         self.p = p
         self.t0, self.t1 = float(t0), float(t1)
-        self.nsteps = max(0, int(round((self.t1 - self.t0) / dt)))
-        self.dt = float(dt)
+        # ceil((t1 - t0)/dt) equal steps (stepping.jl:55-61); a quotient within
+        # a few ulp of a whole number m is m steps of the dt given
+        span = self.t1 - self.t0
+        r = span / dt
+        m = round(r)
+        if m >= 1 and abs(r - m) <= 4 * (math.ulp(r) + math.ulp(span) / dt):
+            self.nsteps, self.dt = int(m), float(dt)
+        else:
+            self.nsteps = max(0, math.ceil(r))
+            self.dt = span / self.nsteps if self.nsteps else float(dt)
         self.istep = 0
         p.upload(u0, _lib.SLOT_STATE)
         # weak: the problem must not keep its integrator alive (a cycle would

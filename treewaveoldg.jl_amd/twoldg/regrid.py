@@ -390,17 +390,3 @@ def adapt_to_initial_data(forest, el, *, initial, flags, buffer=0, maxpasses=8,
         forest = nxt
     return forest, passes, converged
 
-
-def blast_initial(L, x0, sigma):
-    """A radial Gaussian at rest, u = exp(-(r/sigma)^2), r the periodically wrapped
-    distance from x0 in a box of side L (blast.jl:14-35)."""
-
-    def f(x, var):
-        if var != 1:
-            return np.zeros(x.shape[1])
-        r2 = np.zeros(x.shape[1])
-        for d in range(2):
-            r2 += (np.mod(x[d] - x0[d] + L / 2, L) - L / 2) ** 2
-        return np.exp(-((np.sqrt(r2) / sigma) ** 2))
-
-    return f
