@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define TWOLDG_ABI_VERSION 3
+#define TWOLDG_ABI_VERSION 4
 
 /* status codes */
 #define TWOLDG_OK 0
@@ -48,6 +48,9 @@ extern "C" {
 #define TWOLDG_FACE_MIRROR 4    /* reflecting wall: parity-signed own face plane */
 #define TWOLDG_FACE_GLUE 5      /* glued patch face: nb[0], its face nface, tangential map */
 #define TWOLDG_FACE_DIRICHLET 6 /* outer face with Dirichlet data */
+#define TWOLDG_FACE_RADIATIVE 7 /* outer face under the radiative condition
+                                   n . grad u = -(rad_kappa du/dt + rad_invR u):
+                                   RadiativeBoundary, evolution.jl:1129-1270 */
 
 /* slots of a problem's device state buffers.  Only twoldg_rk4 advances STATE;
    the pure entry points (rhs, truncate, energy, norms) read and write the slots
@@ -101,6 +104,27 @@ typedef struct twoldg_desc {
   const double* dcoef;          /* (nblocks, 1 + D, N^D): sqrt(g), A_1 .. A_D */
   const double* lmode;          /* (nblocks, D, E, P+1): each line's rank-one truncation */
   const double* ldual;          /*   pair (MetricTruncation, evolution.jl:556-575) */
+  /* factored truncation (FactoredTruncation, evolution.jl:296-380, 788-798, 904-984):
+     on blocks whose sqrt(g) is a radial factor along one dimension k times an
+     angular one across it (the thirteen-patch ball's outer shells), a rank-one
+     pair along k times a sqrt(g)-orthogonal projection on each plane across k.
+     NULL fslot: none.  Needs a full metric, D = 3 and P <= 4. */
+  const int32_t* fslot;         /* [nblocks] factored-table index f, or -1 */
+  int64_t nfactored;
+  const int32_t* fradial;       /* [nf] the radial dimension k, 0-based */
+  const double* frmodes;        /* (nf, E, P+1): the radial pair of each element */
+  const double* frduals;        /*   along k */
+  const int32_t* faslot;        /* [nf] angular table index A */
+  int64_t nangular;
+  const double* famodes;        /* ((P+1)^2, P^2): Legendre basis of a plane, first
+                                   tangential index fastest */
+  const double* faweights;      /* (nangular, E^2, (P+1)^2): quadrature weight times the
+                                   angular factor */
+  const double* fachol;         /* (nangular, E^2, P^4): upper Cholesky factor of
+                                   Psi^T W Psi, column-major */
+  /* TWOLDG_FACE_RADIATIVE faces (RadiativeFaces, evolution.jl:1166-1203) */
+  double rad_kappa;             /* kappa >= 0 */
+  double rad_invR;              /* 1/R of the u/R term, 0 without it */
 } twoldg_desc;
 
 /* flags */
@@ -138,7 +162,9 @@ int twoldg_truncate(twoldg_problem* p, int slot);
 /* nsteps classic RK4 steps of the state slot, truncation as stage and step
    limiter: wave_integrator + solve!, src/stepping.jl:72-78 */
 int twoldg_rk4(twoldg_problem* p, double t0, double dt, int64_t nsteps);
-/* discrete energy: wave_energy, src/evolution.jl:1472-1489 */
+/* discrete energy: wave_energy, src/evolution.jl:1472-1489; under radiative faces
+   with a u/R term it includes the boundary's 1/2 sum w |S[d]| u^2/R over the outer
+   faces' nodes (boundary_energy, evolution.jl:1276-1299) */
 int twoldg_energy(twoldg_problem* p, int slot, double t, double* out);
 /* p-norm (pnorm >= 1) and max norm of slot a, or of a - b when b >= 0:
    quadrature_norm / wave_norm, src/layout.jl:223-235, src/evolution.jl:1498-1512.

@@ -164,6 +164,11 @@ struct Ball {
   double Lc = 1, R1 = 3;
   int n = 2, m = 2;
   double grading = 0;
+  // the thirteen-patch ball (geometry.jl:555-749): inner shells out to Rm,
+  // exactly radial outer shells (patches 7..12) of mo roots out to R1
+  bool thirteen = false;
+  double Rm = 0;
+  int mo = 0;
 };
 
 static inline double exp_(double a) { return std::exp(a); }
@@ -190,6 +195,19 @@ static void ball_position(const Ball& g, int p, const R xi[3], R x[3]) {
     for (int d = 0; d < 3; ++d) x[d] = (R(2 * g.Lc) / R(g.n)) * xi[d] - R(g.Lc);
     return;
   }
+  if (g.thirteen && p >= 7) {
+    // outer shell over (k, s): x = (Rm + (R1 - Rm) rho) c/|c|, c_k = s,
+    // c_j = 2 xi_j/n - 1, rho = xi_k/mo (s = 1), 1 - xi_k/mo (s = -1)
+    int k = (p - 7) / 2, s = ((p - 7) % 2 == 0) ? -1 : 1;
+    R c[3];
+    for (int j = 0; j < 3; ++j) c[j] = (j == k) ? R(s) : R(2) * xi[j] / R(g.n) - R(1);
+    R rho = (s == 1) ? xi[k] / R(g.mo) : R(1) - xi[k] / R(g.mo);
+    R r = R(g.Rm) + (R(g.R1) - R(g.Rm)) * rho;
+    R nc = sqrt_(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+    for (int j = 0; j < 3; ++j) x[j] = (r / nc) * c[j];
+    return;
+  }
+  const double Rout = g.thirteen ? g.Rm : g.R1;  // the inner shells end on Rm
   int k = (p - 1) / 2, s = ((p - 1) % 2 == 0) ? -1 : 1;
   R f[3];
   for (int j = 0; j < 3; ++j)
@@ -201,7 +219,7 @@ static void ball_position(const Ball& g, int p, const R xi[3], R x[3]) {
     rho = (exp_(a * eta) - R(1)) / (exp_(a) - R(1));
   }
   R nf = sqrt_(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
-  for (int j = 0; j < 3; ++j) x[j] = (R(1) - rho) * f[j] + (rho * R(g.R1) / nf) * f[j];
+  for (int j = 0; j < 3; ++j) x[j] = (R(1) - rho) * f[j] + (rho * R(Rout) / nf) * f[j];
 }
 
 
@@ -264,7 +282,8 @@ struct KeyHash {
   }
 };
 
-enum GhostKind { G_NONE = 0, G_SAME, G_COARSER, G_FINER, G_MIRROR, G_GLUE, G_DIRICHLET };
+enum GhostKind { G_NONE = 0, G_SAME, G_COARSER, G_FINER, G_MIRROR, G_GLUE, G_DIRICHLET,
+                 G_RADIATIVE };
 
 struct Face {
   int kind = G_NONE;
@@ -282,8 +301,8 @@ struct Problem {
   int geometry = 0;  // 0 brick, 1 seven-patch ball, 2 annulus (diagonal metric)
   Ball ball;
   int npatch = 1;
-  int proots[7][3];
-  double plo[7][3], phi_[7][3];
+  int proots[13][3];
+  double plo[13][3], phi_[13][3];
   bool periodic[3] = {true, true, true};
   bool reflecting = false;
   bool truncate = true;
@@ -301,6 +320,10 @@ struct Problem {
   std::vector<double> tw, tchol, tmodes;
   int tn = 0, tm = 0;
   double hspeed = 0;
+  // the radiative boundary (evolution.jl:1129-1299) on the outer sphere
+  bool radiative = false;
+  double rad_kappa = 0, rad_invR = 0;
+  const double* ycur = nullptr;  // the state whose gradient is being formed
   // Dirichlet plane wave
   double kvec[3] = {0, 0, 0}, omega = 0, phase = 0;
   // annulus (geometry 2): r0 <= r <= r1, four patches of (n1, n2) roots
@@ -671,10 +694,11 @@ static void build_face_index(Problem& pr) {
 static void resolve_glue(Problem& pr, int64_t b, int d, int s, Face& F) {
   const Leaf& L = pr.leaves[b];
   // the outer sphere: a shell's face away from the cube along its radial dim
-  if (L.patch > 0) {
-    int k = (L.patch - 1) / 2, sg = ((L.patch - 1) % 2 == 0) ? -1 : 1;
+  const int first = pr.ball.thirteen ? 7 : 1;  // the shells that reach the sphere
+  if (L.patch >= first) {
+    int k = (L.patch - first) / 2, sg = ((L.patch - first) % 2 == 0) ? -1 : 1;
     if (d == k && ((sg == 1 && s == 1) || (sg == -1 && s == 0))) {
-      F.kind = G_DIRICHLET;
+      F.kind = pr.radiative ? G_RADIATIVE : G_DIRICHLET;
       return;
     }
   }
@@ -757,6 +781,12 @@ static void fill_face(const Problem& pr, int64_t b, int d, int s, double par, Sr
           I[d] = s ? N - 1 : 0;
           for (int k = 0; k < nt; ++k) I[t[k]] = T[k];
           val = par * src(b, I);
+        } break;
+        case G_RADIATIVE: {  // the face node's own value (radiative_u_kernel!)
+          int64_t I[3] = {0, 0, 0};
+          I[d] = s ? N - 1 : 0;
+          for (int k = 0; k < nt; ++k) I[t[k]] = T[k];
+          val = src(b, I);
         } break;
         case G_GLUE: {
           int64_t l = F.glue_src[i + N * j];
@@ -1065,6 +1095,24 @@ static void exchange_cart(Problem& pr, double t) {
               node_physical(pr, b, I, x);
               plane_wave(pr, x, t, dv);
               v = dv[1 + comp];
+            } else if (F.kind == G_RADIATIVE) {
+              // V_g = V + c S, S . V_g = 2 F* - S . V, F* = -sigma |S| (kappa v + u/R)
+              int64_t I[3] = {0, 0, 0};
+              I[d] = s ? N - 1 : 0;
+              for (int k = 0; k < nt; ++k) I[tt[k]] = T[k];
+              const int64_t lf = lin(pr, I);
+              int64_t Sf[3] = {0, 0, 0};
+              for (int e = 0; e < D; ++e) Sf[e] = I[e] + 1;
+              double Sv[3], V[3], S2 = 0, Fl = 0;
+              for (int q = 0; q < D; ++q) {
+                Sv[q] = coefv(pr, b, 1 + d * D + q, lf);
+                V[q] = g[(b * D + q) * hn + hidx(pr, Sf, n)];
+                S2 += Sv[q] * Sv[q];
+                Fl += Sv[q] * V[q];
+              }
+              const double uu = pr.ycur[b * 2 * pr.nn + lf], vv = pr.ycur[b * 2 * pr.nn + pr.nn + lf];
+              const double Fs = -(s ? 1.0 : -1.0) * std::sqrt(S2) * (pr.rad_kappa * vv + pr.rad_invR * uu);
+              v = V[comp] + 2 * (Fs - Fl) / S2 * Sv[comp];
             }
             g[(b * D + comp) * hn + hidx(pr, S, n)] = v;
           };
@@ -1120,6 +1168,7 @@ static void laplacian_cart(Problem& pr, const double* y, double* du) {
 }
 
 static void fill_gradient(Problem& pr, const double* y, double t) {
+  pr.ycur = y;
   scatter_exchange_u(pr, y, t);
   if (pr.geometry != 1) {
     gradient_flat(pr);
@@ -1303,7 +1352,24 @@ static double energy(Problem& pr, const double* y, double t) {
     for (int d = 0; d < 3; ++d) Pd[d] += pot[b * 3 + d];
   }
   double Pot = Pd[0] + Pd[1] + Pd[2];
-  return (K + Pot) / 2;
+  double bnd = 0;  // the radiative boundary's 1/2 sum w |S[d]| u^2 / R (evolution.jl:1276-1299)
+  if (pr.radiative && pr.rad_invR != 0)
+    for (int64_t b = 0; b < pr.nb; ++b)
+      for (int d = 0; d < D; ++d)
+        for (int s = 0; s < 2; ++s) {
+          if (pr.faces[(b * D + d) * 2 + s].kind != G_RADIATIVE) continue;
+          for (int64_t l = 0; l < nn; ++l) {
+            int64_t I[3] = {l % N, (l / N) % N, l / ((int64_t)N * N)};
+            if (I[d] != (s ? N - 1 : 0)) continue;
+            double w = 1, S2 = 0;
+            for (int e = 0; e < D; ++e)
+              if (e != d) w *= pr.h[b] * pr.P1 / 2 * pr.el.weights[I[e] % pr.P1];
+            for (int q = 0; q < D; ++q) S2 += coefv(pr, b, 1 + d * D + q, l) * coefv(pr, b, 1 + d * D + q, l);
+            const double u = y[b * 2 * nn + l];
+            bnd += w * std::sqrt(S2) * u * u;
+          }
+        }
+  return (K + Pot) / 2 + bnd * pr.rad_invR / 2;
 }
 
 static void norms(const Problem& pr, const double* err, int vars_mask, int p, double* lp, double* linf) {
@@ -1714,6 +1780,66 @@ void* orc_ball(int P, int E, int n, int m, double Lc, double R1, double grading,
 // The four-patch annulus r0 <= r <= r1, each patch (n1, n2) roots, the angular
 // faces glued by the identity and the radial ones reflecting (annulus_forest,
 // geometry.jl:240-268).
+// The seven-patch ball (mo = 0) or the thirteen-patch ball (mo > 0, inner
+// shells to Rm), with Dirichlet data of a plane wave or (radiative != 0) the
+// radiative boundary of kappa and invR on the outer sphere.
+void* orc_ball_ext(int P, int E, int n, int m, int mo, double Lc, double Rm, double R1,
+                   int truncate, const double* kvec, double phase, int radiative, double kappa,
+                   double invR) {
+  auto* pr = new Problem();
+  pr->D = 3;
+  pr->P = P;
+  pr->P1 = P + 1;
+  pr->E = E;
+  pr->N = E * (P + 1);
+  build_element(P, pr->el);
+  pr->geometry = 1;
+  pr->ball.Lc = Lc;
+  pr->ball.R1 = R1;
+  pr->ball.n = n;
+  pr->ball.m = m;
+  pr->ball.thirteen = mo > 0;
+  pr->ball.Rm = Rm;
+  pr->ball.mo = mo;
+  pr->truncate = truncate != 0;
+  pr->radiative = radiative != 0;
+  pr->rad_kappa = kappa;
+  pr->rad_invR = invR;
+  pr->npatch = mo > 0 ? 13 : 7;
+  for (int p = 0; p < pr->npatch; ++p)
+    for (int d = 0; d < 3; ++d) {
+      const bool outer = p >= 7;
+      const int k = p == 0 ? -1 : ((outer ? p - 7 : p - 1) / 2);
+      pr->proots[p][d] = (d == k) ? (outer ? mo : m) : n;
+      pr->plo[p][d] = 0;
+      pr->phi_[p][d] = pr->proots[p][d];
+      pr->periodic[d] = false;
+    }
+  for (int d = 0; d < 3; ++d) pr->kvec[d] = kvec[d];
+  pr->omega = std::sqrt(kvec[0] * kvec[0] + kvec[1] * kvec[1] + kvec[2] * kvec[2]);
+  pr->phase = phase;
+  for (int p = 0; p < pr->npatch; ++p) {
+    int64_t nr = (int64_t)pr->proots[p][0] * pr->proots[p][1] * pr->proots[p][2];
+    for (int64_t r = 0; r < nr; ++r) {
+      Leaf Lf{p, 0, {0, 0, 0}};
+      int64_t x = r;
+      for (int d = 0; d < 3; ++d) {
+        Lf.c[d] = x % pr->proots[p][d];
+        x /= pr->proots[p][d];
+      }
+      pr->leaves.push_back(Lf);
+    }
+  }
+  finalize_leaves(*pr);
+  build_face_index(*pr);
+  resolve_faces(*pr);
+  delete g_faceindex;
+  g_faceindex = nullptr;
+  allocate_work(*pr);
+  build_ball_metric(*pr);
+  return pr;
+}
+
 void* orc_annulus(int P, int E, int n1, int n2, double r0, double r1, int truncate) {
   auto* pr = new Problem();
   pr->D = 2;

@@ -63,6 +63,9 @@ def lib():
         L.orc_timestep.restype = ctypes.c_double
         L.orc_rk4.argtypes = [_dp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_int64]
         L.orc_threads.restype = ctypes.c_int
+        L.orc_ball_ext.argtypes = [ctypes.c_int] * 5 + [ctypes.c_double] * 3 + [
+            ctypes.c_int, _dp, ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_double]
+        L.orc_ball_ext.restype = _dp
         L.orc_brick_leaves.argtypes = [ctypes.c_int] * 4 + [ctypes.c_double, ctypes.c_int,
                                                              ctypes.c_int, ctypes.c_int64, _dp]
         L.orc_brick_leaves.restype = _dp
@@ -129,6 +132,22 @@ class Problem:
         h = lib().orc_ball(P, E, roots[0], roots[1], float(Lc), float(R1), float(grading),
                            int(truncate), _ptr(kv), float(phi), len(regions),
                            _ptr(reg) if len(regions) else None)
+        p = cls(h, 3, P, E, 1)
+        p.k = kv
+        p.phi = phi
+        return p
+
+    @classmethod
+    def ball_ext(cls, P, E, roots=(2, 2), Lc=1.0, Rm=2.0, R1=3.0, truncate=True,
+                 k=(1.1, -0.7, 0.5), phi=0.3, radiative=False, kappa=1.0, invR=0.0):
+        """The seven-patch ball (roots (n, m)) or the thirteen-patch ball (roots
+        (n, m, m'), inner shells to Rm), with a plane wave's Dirichlet data or the
+        radiative boundary on the outer sphere."""
+        kv = np.asarray(k, dtype=np.float64)
+        mo = roots[2] if len(roots) == 3 else 0
+        h = lib().orc_ball_ext(P, E, roots[0], roots[1], mo, float(Lc), float(Rm), float(R1),
+                               int(truncate), _ptr(kv), float(phi), int(radiative),
+                               float(kappa), float(invR))
         p = cls(h, 3, P, E, 1)
         p.k = kv
         p.phi = phi
@@ -339,10 +358,15 @@ def wave_errors(D, P, E, roots=4, L=1.0, m=1, cfl=0.5, periods=0.25, truncate=Tr
 
 
 def sevenpatch_errors(P, E, roots=(2, 2), Lc=1.0, R1=3.0, k=(1.1, -0.7, 0.5), phi=0.3,
-                      cfl=0.5, tend=1.0, truncate=True, grading=0.0, nsteps=None):
-    """The oracle's sevenpatch_errors (sevenpatch.jl:84-117)."""
-    pr = Problem.ball(P, E, roots=roots, Lc=Lc, R1=R1, grading=grading, truncate=truncate,
-                      k=k, phi=phi)
+                      cfl=0.5, tend=1.0, truncate=True, grading=0.0, nsteps=None, Rm=None):
+    """The oracle's sevenpatch_errors (sevenpatch.jl:84-117); with Rm the
+    thirteen-patch ball."""
+    if Rm is None:
+        pr = Problem.ball(P, E, roots=roots, Lc=Lc, R1=R1, grading=grading, truncate=truncate,
+                          k=k, phi=phi)
+    else:
+        pr = Problem.ball_ext(P, E, roots=roots, Lc=Lc, Rm=Rm, R1=R1, truncate=truncate, k=k,
+                              phi=phi)
     y0 = pr.truncate(pr.fill_exact(2, 0.0))
     if nsteps is None:
         nsteps = math.ceil(tend / pr.timestep(cfl))

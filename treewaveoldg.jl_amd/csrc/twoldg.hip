@@ -167,7 +167,9 @@ void destroy(twoldg_problem* p) {
                   p->grads, p->tables, p->coef, (void*)p->flat, (void*)p->tslot, p->tmodes,
                   p->tweights, p->tginv, (void*)p->spacing, (void*)p->rowptr, (void*)p->cb,
                   (void*)p->cl, p->cval_u, p->cval_g, (void*)p->dir_rows, (void*)p->dir_x,
-                  (void*)p->part, (void*)p->red, (void*)p->clock, p->dcoef, p->lmode, p->ldual, (void*)p->nbr6, p->uf[0], p->uf[1], p->pf[0],
+                  (void*)p->part, (void*)p->red, (void*)p->clock, p->dcoef, p->lmode, p->ldual, (void*)p->fslot, (void*)p->fradial,
+                  p->frmodes, p->frduals, (void*)p->faslot, p->famodes, p->faweights, p->faginv,
+                  (void*)p->rad_rows, (void*)p->rpart, (void*)p->nbr6, p->uf[0], p->uf[1], p->pf[0],
                   p->pf[1], (void*)p->frows, (void*)p->fsrc, p->fwu,
                   p->fwg, p->fgu, p->fgg, p->ffg})
     dfree(b);
@@ -212,6 +214,42 @@ int check_desc(const twoldg_desc* d) {
   if (d->tslot && (!d->coef || !d->tmodes || !d->tweights || !d->tchol || d->P > 4))
     return fail(TWOLDG_EINVAL, "per-element truncation tables need a full metric, all "
                                "tables, and P <= 4");
+  if (d->fslot) {
+    if (!d->coef || d->P > 4 || !d->fradial || !d->frmodes || !d->frduals ||
+        !d->faslot || !d->famodes || !d->faweights || !d->fachol || d->nfactored < 1 ||
+        d->nangular < 1)
+      return fail(TWOLDG_EINVAL, "a factored truncation needs a full metric, "
+                                 "P <= 4 and all its tables");
+    for (int64_t b = 0; b < d->nblocks; ++b) {
+      const int32_t f = d->fslot[b];
+      if (f < -1 || f >= d->nfactored)
+        return fail(TWOLDG_EINVAL, "factored-table index out of range");
+      if (f >= 0 && d->tslot && d->tslot[b] >= 0)
+        return fail(TWOLDG_EINVAL, "a block takes per-element or factored tables, not both");
+    }
+    for (int64_t f = 0; f < d->nfactored; ++f)
+      if (d->fradial[f] < 0 || d->fradial[f] > 2 || d->faslot[f] < 0 ||
+          d->faslot[f] >= d->nangular)
+        return fail(TWOLDG_EINVAL, "factored table entry out of range");
+  }
+  if (d->tslot)
+    for (int64_t b = 0; b < d->nblocks; ++b)
+      if (d->tslot[b] < -1 || d->tslot[b] >= d->ntables)
+        return fail(TWOLDG_EINVAL, "per-element table index out of range");
+  for (int64_t i = 0; i < d->nblocks * 2 * d->D; ++i)
+    if (d->faces[i].kind == TWOLDG_FACE_RADIATIVE) {
+      if (!d->coef)
+        return fail(TWOLDG_EINVAL,
+                    "a radiative boundary forms the normal flux from the Cartesian gradient, "
+                    "which a full geometry carries: pass a `FullGeometry` (`geometry`)");
+      if (!(d->rad_kappa >= 0))
+        return fail(TWOLDG_EINVAL,
+                    "κ = " + std::to_string(d->rad_kappa) +
+                        " is negative, and a radiative boundary with κ < 0 feeds energy in: "
+                        "take κ ≥ 0, κ = 1 for Sommerfeld's condition");
+      if (!(d->rad_invR >= 0) || !std::isfinite(d->rad_invR))
+        return fail(TWOLDG_EINVAL, "the radiative boundary's 1/R must be finite and >= 0");
+    }
   for (int64_t b = 0; b < d->nblocks; ++b)
     if (!(d->spacing[b] > 0)) return fail(TWOLDG_EINVAL, "block spacing must be positive");
   for (int64_t i = 0; i < d->nblocks * 2 * d->D; ++i)
@@ -219,6 +257,36 @@ int check_desc(const twoldg_desc* d) {
       return fail(TWOLDG_EINVAL,
                   "Dirichlet data (`boundary`) are given in Cartesian components and taken by a "
                   "full geometry only, with the positions of the outer ghosts");
+  return 0;
+}
+
+// G^-1 = U^-1 U^-T for each of nel Gram matrices of order mq, from their upper
+// Cholesky factors U (column-major): the truncations' two triangular solves
+// become one symmetric mat-vec on the device.
+int gram_inverses(const double* chol, int64_t nel, int64_t mq, std::vector<double>& ginv) {
+  ginv.assign((size_t)(nel * mq * mq), 0.0);
+  std::vector<double> V((size_t)(mq * mq));
+  for (int64_t el = 0; el < nel; ++el) {
+    const double* U = chol + el * mq * mq;
+    // V = U^-1 (upper triangular), column by column: U V[:, j] = e_j
+    std::fill(V.begin(), V.end(), 0.0);
+    for (int64_t j = 0; j < mq; ++j) {
+      for (int64_t i = j; i >= 0; --i) {
+        double r = (i == j) ? 1.0 : 0.0;
+        for (int64_t l = i + 1; l <= j; ++l) r -= U[i + mq * l] * V[l + mq * j];
+        if (U[i + mq * i] == 0.0) return fail(TWOLDG_EINVAL, "singular truncation table");
+        V[i + mq * j] = r / U[i + mq * i];
+      }
+    }
+    double* Gi = ginv.data() + el * mq * mq;
+    for (int64_t q = 0; q < mq; ++q)
+      for (int64_t k = 0; k <= q; ++k) {
+        double g = 0;  // (V V^T)[k][q] = sum over j >= q of V[k][j] V[q][j]
+        for (int64_t j = q; j < mq; ++j) g += V[k + mq * j] * V[q + mq * j];
+        Gi[k + mq * q] = g;
+        Gi[q + mq * k] = g;
+      }
+  }
   return 0;
 }
 
@@ -278,32 +346,36 @@ int create_buffers(twoldg_problem* p, const twoldg_desc* d, const Csr& csr) {
     const int64_t nE = ipow(p->E, D), nq = ipow(p->P1, D), mq = ipow(p->P, D);
     if ((rc = upload_as<T>(p, &p->tmodes, d->tmodes, nq * mq))) return rc;
     if ((rc = upload_as<T>(p, &p->tweights, d->tweights, d->ntables * nE * nq))) return rc;
-    // G^-1 = U^-1 U^-T per element from the upper Cholesky factor U (column-major)
     const int64_t nel = d->ntables * nE;
-    std::vector<double> ginv((size_t)(nel * mq * mq));
-    std::vector<double> V((size_t)(mq * mq));
-    for (int64_t el = 0; el < nel; ++el) {
-      const double* U = d->tchol + el * mq * mq;
-      // V = U^-1 (upper triangular), column by column: U V[:, j] = e_j
-      std::fill(V.begin(), V.end(), 0.0);
-      for (int64_t j = 0; j < mq; ++j) {
-        for (int64_t i = j; i >= 0; --i) {
-          double r = (i == j) ? 1.0 : 0.0;
-          for (int64_t l = i + 1; l <= j; ++l) r -= U[i + mq * l] * V[l + mq * j];
-          if (U[i + mq * i] == 0.0) return fail(TWOLDG_EINVAL, "singular truncation table");
-          V[i + mq * j] = r / U[i + mq * i];
-        }
-      }
-      double* Gi = ginv.data() + el * mq * mq;
-      for (int64_t q = 0; q < mq; ++q)
-        for (int64_t k = 0; k <= q; ++k) {
-          double g = 0;  // (V V^T)[k][q] = sum over j >= q of V[k][j] V[q][j]
-          for (int64_t j = q; j < mq; ++j) g += V[k + mq * j] * V[q + mq * j];
-          Gi[k + mq * q] = g;
-          Gi[q + mq * k] = g;
-        }
-    }
+    std::vector<double> ginv;
+    if ((rc = gram_inverses(d->tchol, nel, mq, ginv))) return rc;
     if ((rc = upload_as<T>(p, &p->tginv, ginv.data(), nel * mq * mq))) return rc;
+  }
+  if (d->fslot) {
+    p->has_tables = true;
+    const int64_t nf = d->nfactored, nA = d->nangular, E = p->E, P1 = p->P1, P = p->P;
+    const int64_t n2 = P1 * P1, m2 = P * P;
+    if ((rc = dalloc(p, &p->fslot, p->sh.nb))) return rc;
+    if ((rc = h2d(p, p->fslot, d->fslot, p->sh.nb * 4))) return rc;
+    if ((rc = dalloc(p, &p->fradial, nf))) return rc;
+    if ((rc = h2d(p, p->fradial, d->fradial, nf * 4))) return rc;
+    if ((rc = dalloc(p, &p->faslot, nf))) return rc;
+    if ((rc = h2d(p, p->faslot, d->faslot, nf * 4))) return rc;
+    if ((rc = upload_as<T>(p, &p->frmodes, d->frmodes, nf * E * P1))) return rc;
+    if ((rc = upload_as<T>(p, &p->frduals, d->frduals, nf * E * P1))) return rc;
+    if ((rc = upload_as<T>(p, &p->famodes, d->famodes, n2 * m2))) return rc;
+    if ((rc = upload_as<T>(p, &p->faweights, d->faweights, nA * E * E * n2))) return rc;
+    std::vector<double> ginv;
+    if ((rc = gram_inverses(d->fachol, nA * E * E, m2, ginv))) return rc;
+    if ((rc = upload_as<T>(p, &p->faginv, ginv.data(), nA * E * E * m2 * m2))) return rc;
+  }
+  if (!csr.rad_rows.empty()) {
+    p->nrad = (int64_t)csr.rad_rows.size();
+    p->rad_kappa = d->rad_kappa;
+    p->rad_invR = d->rad_invR;
+    if ((rc = dalloc(p, &p->rad_rows, p->nrad))) return rc;
+    if ((rc = h2d(p, p->rad_rows, csr.rad_rows.data(), p->nrad * 8))) return rc;
+    if ((rc = dalloc(p, &p->rpart, p->nrad / p->sh.Nt + 1))) return rc;
   }
   if (!csr.dir_rows.empty()) {
     p->ndir = (int64_t)csr.dir_rows.size();

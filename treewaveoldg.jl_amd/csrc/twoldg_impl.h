@@ -515,21 +515,37 @@ __device__ __forceinline__ void lds_barrier() {
 // (and per mode), three LDS barriers.  The elements of blocks without tables
 // (tslot < 0) get the tensor projector here too, direction by direction with
 // k_truncate's arithmetic, so one launch covers the forest.
+// The factored truncation's tables (device): per block its factored index f or
+// -1; per f the radial dimension, the radial pairs (f, e_k, i) and the angular
+// table A; the plane basis Psi ((P+1)^2, P^2); per (A, element across k) the
+// plane's weights and the inverse of its Gram matrix Psi^T W Psi.
+template <class T>
+struct FactoredTabs {
+  const int32_t* slot;
+  const int32_t* radial;
+  const T* rmodes;
+  const T* rduals;
+  const int32_t* aslot;
+  const T* amodes;
+  const T* aweights;
+  const T* aginv;
+};
+
 template <class T, int P1>
 __global__ void __launch_bounds__(((P1 * P1 * P1) + 63) / 64 * 64)
 k_truncate_elem(Shape sh, int E, const Tables<T>* __restrict__ tbp,
                 const int32_t* __restrict__ tslot, const T* __restrict__ tmodes,
                 const T* __restrict__ tweights, const T* __restrict__ tginv,
-                T* __restrict__ y) {
+                FactoredTabs<T> F, T* __restrict__ y) {
   constexpr int P = P1 - 1;
   constexpr int n = P1 * P1 * P1, m = P * P * P;
   __shared__ T sWA[n];
-  __shared__ T sc[m];
-  __shared__ T sz[m];
+  __shared__ T sc[P1 * P * P];  // m <= (P+1) P^2: the element's, or the planes' coefficients
+  __shared__ T sz[P1 * P * P];
   __shared__ T sG[m * m];  // G^-1, column-major (symmetric)
   const int64_t ne = (int64_t)E * E * E;
   const int64_t b = blockIdx.x / ne, e = blockIdx.x % ne;
-  const int s = tslot[b];
+  const int s = tslot ? tslot[b] : -1;
   const int k = threadIdx.x;
   const int N = sh.N;
   const int J0 = (int)(e % E), J1 = (int)((e / E) % E), J2 = (int)(e / ((int64_t)E * E));
@@ -539,6 +555,65 @@ k_truncate_elem(Shape sh, int E, const Tables<T>* __restrict__ tbp,
     const int k0 = q % P1, k1 = (q / P1) % P1, k2 = q / (P1 * P1);
     return base + k0 + (int64_t)N * k1 + (int64_t)N * N * k2;
   };
+  const int fs = F.slot ? F.slot[b] : -1;
+  if (fs >= 0) {
+    // evolution.jl:306-380: along k, a <- a - mode (dual . a) on every line;
+    // then on each plane across k, a <- Psi G^-1 Psi^T W a
+    constexpr int n2 = P1 * P1, m2 = P * P;
+    const int kr = F.radial[fs], A = F.aslot[fs];
+    const int t0 = kr == 0 ? 1 : 0, t1 = kr == 2 ? 1 : 2;
+    const int J[3] = {J0, J1, J2};
+    const int64_t ea = J[t0] + (int64_t)E * J[t1];
+    const int st[3] = {1, P1, P1 * P1};
+    const T* rm = F.rmodes + ((int64_t)fs * E + J[kr]) * P1;
+    const T* rd = F.rduals + ((int64_t)fs * E + J[kr]) * P1;
+    const T* w = F.aweights + ((int64_t)A * E * E + ea) * n2;
+    const T* G = F.aginv + ((int64_t)A * E * E + ea) * m2 * m2;
+    for (int i = k; i < m2 * m2; i += blockDim.x) sG[i] = G[i];
+    int kk[3] = {0, 0, 0};
+    if (k < n) {
+      kk[0] = k % P1;
+      kk[1] = (k / P1) % P1;
+      kk[2] = k / (P1 * P1);
+      sWA[k] = a[at(k)];
+    }
+    __syncthreads();
+    T v = 0;
+    if (k < n) {
+      const int l0 = k - kk[kr] * st[kr];
+      T c = 0;
+#pragma unroll
+      for (int j = 0; j < P1; ++j) c += rd[j] * sWA[l0 + j * st[kr]];
+      v = sWA[k] - rm[kk[kr]] * c;
+    }
+    lds_barrier();
+    const int pn = kk[t0] + P1 * kk[t1];  // the node's index in its plane
+    if (k < n) sWA[k] = w[pn] * v;
+    lds_barrier();
+    if (k < P1 * m2) {  // (plane i, mode q): Psi^T W a
+      const int i = k / m2, q = k % m2;
+      T cq = 0;
+      for (int r = 0; r < n2; ++r) {
+        const int node = i * st[kr] + (r % P1) * st[t0] + (r / P1) * st[t1];
+        cq += F.amodes[r * m2 + q] * sWA[node];
+      }
+      sc[k] = cq;
+    }
+    lds_barrier();
+    if (k < P1 * m2) {
+      const int i = k / m2, q = k % m2;
+      T zq = 0;
+      for (int r = 0; r < m2; ++r) zq += sG[q + m2 * r] * sc[i * m2 + r];
+      sz[k] = zq;
+    }
+    lds_barrier();
+    if (k < n) {
+      T z = 0;
+      for (int q = 0; q < m2; ++q) z += F.amodes[pn * m2 + q] * sz[kk[kr] * m2 + q];
+      a[at(k)] = z;
+    }
+    return;
+  }
   if (s < 0) {  // the tensor projector (evolution.jl:238-264), x, y, then z
     const Tables<T>& tb = *tbp;
     if (k < n) sWA[k] = a[at(k)];
@@ -740,6 +815,94 @@ __global__ void k_final(int64_t nb, const double* __restrict__ part, bool maxlas
     for (int q = 0; q < NV; ++q) out[q] = v[q];
 }
 
+// ---------------------------------------------------------------------------
+// The radiative condition's gradient ghosts (evolution.jl:1236-1270): on each
+// radiative row the ghost holds the face node's Cartesian gradient V (the
+// exchange copies it); its component along S[d] is changed so that
+// S[d] . V_g = 2 F* - F, F = S[d] . V the face node's flux and
+// F* = -sigma |S[d]| (kappa v + invR u), sigma = +1 on a high face.  The
+// Laplacian averages the two, so the face's flux is F*.
+// ---------------------------------------------------------------------------
+template <int D>
+__device__ __forceinline__ int64_t face_node(int64_t face, int64_t t, int N, int* d_out,
+                                             int* s_out) {
+  const int s = (int)(face % 2), d = (int)((face / 2) % D);
+  int I[kMaxD] = {0, 0, 0};
+  I[d] = s ? N - 1 : 0;
+  int64_t rest = t;
+#pragma unroll
+  for (int e = 0; e < D; ++e)
+    if (e != d) {
+      I[e] = (int)(rest % N);
+      rest /= N;
+    }
+  *d_out = d;
+  *s_out = s;
+  return I[0] + (int64_t)N * (D >= 2 ? I[1] : 0) + (int64_t)N * N * (D >= 3 ? I[2] : 0);
+}
+
+template <class T, int D>
+__global__ void k_radiative_grad(Shape sh, int64_t nrad, const int64_t* __restrict__ rows,
+                                 const T* __restrict__ coef, const T* __restrict__ y,
+                                 double kappa, double invR, T* __restrict__ ghost) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nrad) return;
+  const int64_t r = rows[q];
+  const int64_t face = r / sh.Nt, t = r % sh.Nt;
+  const int64_t b = face / (2 * D);
+  int d, s;
+  const int64_t l = face_node<D>(face, t, sh.N, &d, &s);
+  double S[kMaxD], V[kMaxD], S2 = 0, F = 0;
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+    S[i] = (double)coef[(b * (1 + D * D) + 1 + d * D + i) * sh.nn + l];
+    V[i] = (double)ghost[(face * D + i) * sh.Nt + t];
+    S2 += S[i] * S[i];
+    F += S[i] * V[i];
+  }
+  const double u = (double)y[b * 2 * sh.nn + l], v = (double)y[b * 2 * sh.nn + sh.nn + l];
+  const double Fs = -(s ? 1.0 : -1.0) * sqrt(S2) * (kappa * v + invR * u);
+  const double c = 2 * (Fs - F) / S2;
+#pragma unroll
+  for (int i = 0; i < D; ++i) ghost[(face * D + i) * sh.Nt + t] = (T)(V[i] + c * S[i]);
+}
+
+// The radiative boundary's own energy (evolution.jl:1276-1299): per radiative
+// face, sum w |S[d]| u^2 over its nodes, w the face's quadrature weight; one
+// workgroup per face (a face's rows are consecutive).
+template <class T, int D, int P1>
+__global__ void k_rad_energy(Shape sh, const Tables<T>* __restrict__ tbp,
+                             const double* __restrict__ spacing, const int64_t* __restrict__ rows,
+                             const T* __restrict__ coef, const T* __restrict__ y,
+                             const T* __restrict__ ylo, double* __restrict__ part) {
+  const int64_t face = rows[(int64_t)blockIdx.x * sh.Nt] / sh.Nt;
+  const int64_t b = face / (2 * D);
+  const double h = spacing[b];
+  double acc[1] = {0};
+  for (int64_t t = threadIdx.x; t < sh.Nt; t += blockDim.x) {
+    int d, s;
+    const int64_t l = face_node<D>(face, t, sh.N, &d, &s);
+    double w = 1;
+    int64_t rest = t;
+#pragma unroll
+    for (int e = 0; e < D - 1; ++e) {
+      w *= h * P1 / 2 * tbp->wq[(rest % sh.N) % P1];
+      rest /= sh.N;
+    }
+    double S2 = 0;
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      const double Si = (double)coef[(b * (1 + D * D) + 1 + d * D + i) * sh.nn + l];
+      S2 += Si * Si;
+    }
+    double u = (double)y[b * 2 * sh.nn + l];
+    if (ylo) u -= (double)ylo[b * 2 * sh.nn + l];
+    acc[0] += w * sqrt(S2) * u * u;
+  }
+  group_reduce<1>(acc, false);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
+}
+
 // y (double, host layout) <-> device precision
 template <class T>
 __global__ void k_from_double(int64_t n, const double* __restrict__ src, T* __restrict__ hi,
@@ -809,6 +972,21 @@ struct twoldg_problem {
   void* tmodes = nullptr;
   void* tweights = nullptr;
   void* tginv = nullptr;  // per element: the inverse of the truncation's Gram matrix
+  // factored truncation (outer shells): radial pairs, angular tables and the
+  // inverses of their Gram matrices
+  int32_t* fslot = nullptr;
+  int32_t* fradial = nullptr;
+  void* frmodes = nullptr;
+  void* frduals = nullptr;
+  int32_t* faslot = nullptr;
+  void* famodes = nullptr;
+  void* faweights = nullptr;
+  void* faginv = nullptr;
+  // radiative faces: their exchange rows (u-shaped) and the condition's constants
+  int64_t nrad = 0;
+  int64_t* rad_rows = nullptr;
+  double rad_kappa = 0, rad_invR = 0;
+  double* rpart = nullptr;  // per radiative face: the boundary energy's partial sum
   double* spacing = nullptr;
   int32_t* rowptr = nullptr;
   int32_t* cb = nullptr;
@@ -933,6 +1111,7 @@ struct Csr {
   std::vector<int32_t> rowptr, cb, cl;
   std::vector<double> vu, vg;
   std::vector<int64_t> dir_rows;
+  std::vector<int64_t> rad_rows;
 };
 
 int64_t ipow(int64_t a, int b) {
@@ -1057,6 +1236,14 @@ int build_csr(const twoldg_desc* d, Csr& csr) {
             case TWOLDG_FACE_DIRICHLET:
               csr.dir_rows.push_back(r);
               break;
+            case TWOLDG_FACE_RADIATIVE: {
+              // u's ghost is the face node's own u; the gradient's starts as the
+              // face node's and is then corrected by k_radiative_grad
+              I[dd] = s ? N - 1 : 0;
+              for (int k = 0; k < nt; ++k) I[tdim[k]] = T[k];
+              push(b, I, 1.0, 1.0);
+              csr.rad_rows.push_back(r);
+            } break;
             default:
               return fail(TWOLDG_EINVAL, "block " + std::to_string(b) + " face (" +
                                              std::to_string(dd) + "," + std::to_string(s) +
@@ -1156,7 +1343,16 @@ struct Launch {
                          p->stream, p->sh, (const Tables<T>*)p->tables, p->spacing, y,
                          (const T*)p->ug, (const T*)p->coef, p->flat, (T*)p->grads);
       ghosts<2>(p, (const T*)p->cval_u, (const T*)p->grads, cdt, (T*)p->gg);
+      if (p->nrad)
+        hipLaunchKernelGGL((k_radiative_grad<T, D>), dim3(grid_for(p->nrad, 256)), dim3(256), 0,
+                           p->stream, p->sh, p->nrad, p->rad_rows, (const T*)p->coef, y,
+                           p->rad_kappa, p->rad_invR, (T*)p->gg);
     }
+  }
+  static FactoredTabs<T> factored(twoldg_problem* p) {
+    return {p->fslot,           p->fradial,          (const T*)p->frmodes,
+            (const T*)p->frduals, p->faslot,         (const T*)p->famodes,
+            (const T*)p->faweights, (const T*)p->faginv};
   }
   template <int MODE>
   static void stage(twoldg_problem* p, const StageIO<T>& io) {
@@ -1185,7 +1381,7 @@ struct Launch {
         hipLaunchKernelGGL((k_truncate_elem<T, P1>), dim3((unsigned)n),
                            dim3((P1 * P1 * P1 + 63) / 64 * 64), 0, p->stream, p->sh, p->E,
                            (const Tables<T>*)p->tables, p->tslot, (const T*)p->tmodes,
-                           (const T*)p->tweights, (const T*)p->tginv, y);
+                           (const T*)p->tweights, (const T*)p->tginv, factored(p), y);
         return;
       }
     }
@@ -1206,6 +1402,14 @@ struct Launch {
                          (const T*)nullptr, p->part);
     hipLaunchKernelGGL((k_final<2>), dim3(1), dim3(kRedThreads), 0, p->stream, p->sh.nb, p->part,
                        false, p->red);
+    if (p->nrad && p->rad_invR != 0) {  // the radiative boundary's own energy
+      const int64_t nf = p->nrad / p->sh.Nt;
+      hipLaunchKernelGGL((k_rad_energy<T, D, P1>), dim3((unsigned)nf), dim3(kRedThreads), 0,
+                         p->stream, p->sh, (const Tables<T>*)p->tables, p->spacing,
+                         p->rad_rows, (const T*)p->coef, y, (const T*)nullptr, p->rpart);
+      hipLaunchKernelGGL((k_final<1>), dim3(1), dim3(kRedThreads), 0, p->stream, nf, p->rpart,
+                         false, p->red + 2);
+    }
   }
   static void norms(twoldg_problem* p, const T* a, const T* alo, const T* b, const T* blo,
                     int mask, int pnorm, double* part, double* red) {
@@ -1432,10 +1636,13 @@ int energy_impl(twoldg_problem* p, int slot, double t, double* out) {
   set_clock(p, t, 0.0);
   L::gradient(p, y, 0.0);
   L::energy(p, y);
-  double h[2];
-  HIP_TRY(hipMemcpyAsync(h, p->red, 2 * sizeof(double), hipMemcpyDeviceToHost, p->stream));
+  double h[3] = {0, 0, 0};
+  const bool bnd = p->nrad && p->rad_invR != 0;
+  HIP_TRY(hipMemcpyAsync(h, p->red, (bnd ? 3 : 2) * sizeof(double), hipMemcpyDeviceToHost,
+                         p->stream));
   HIP_TRY(hipStreamSynchronize(p->stream));
   *out = (h[0] + h[1]) / 2;
+  if (bnd) *out += h[2] * p->rad_invR / 2;  // boundary_energy, evolution.jl:1276-1299
   return 0;
 }
 

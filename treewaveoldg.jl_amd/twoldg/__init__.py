@@ -13,6 +13,10 @@ from .drivers import (wave_omega, wave_exact, reflect_omega, reflect_exact, gaus
                       PlaneWave, sevenpatch_exact, wave_errors, reflect_errors, sevenpatch_errors,
                       convergence_rate)
 from .batch import Batch, gaussian_pulses
+from .drivers import ball_forest, radiative_errors
+from .geometry import SevenPatchBall, sevenpatch_forest
+from .thirteenpatch import (ThirteenPatchBall, thirteenpatch_forest, RadiativeBoundary,
+                            OutgoingPulse, ZeroData)
 from .annulus import (PolarAnnulus, annulus_forest, annulus_wavenumber, annulus_exact,
                       annulus_errors)
 from .regrid import (RegridFlag, Refine, Keep, Coarsen, level, doublefactorial, check_mode,
@@ -32,7 +36,9 @@ __all__ = [
     "sevenpatch_exact", "wave_errors", "reflect_errors", "sevenpatch_errors",
     "convergence_rate", "Batch", "PolarAnnulus", "annulus_forest", "annulus_wavenumber",
     "annulus_exact", "annulus_errors",
-    "gaussian_pulses", "RegridFlag", "Refine", "Keep", "Coarsen", "level", "doublefactorial",
+    "gaussian_pulses", "ball_forest", "radiative_errors", "SevenPatchBall", "sevenpatch_forest",
+    "ThirteenPatchBall", "thirteenpatch_forest", "RadiativeBoundary", "OutgoingPulse",
+    "ZeroData", "RegridFlag", "Refine", "Keep", "Coarsen", "level", "doublefactorial",
     "check_mode", "check_thresholds", "modal_indicator", "mode_amplitude", "refine_mark",
     "flag_blocks", "refine_flags", "refinement_buffer", "regrid_forest", "transfer_map", "regrid",
     "adapt_to_initial_data", "blast_initial", "blast_reference", "blast_exact", "uniform_blast",

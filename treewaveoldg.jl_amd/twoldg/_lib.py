@@ -10,9 +10,10 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(_HERE, "libtwoldg.so")
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 F64, F32C = 0, 1
 FACE_SAME, FACE_COARSER, FACE_FINER, FACE_MIRROR, FACE_GLUE, FACE_DIRICHLET = 1, 2, 3, 4, 5, 6
+FACE_RADIATIVE = 7
 SLOT_STATE, SLOT_DU, SLOT_AUX, SLOT_ARG = 0, 1, 2, 3
 FLAG_GENERIC = 1
 XFER_COPY, XFER_REFINE, XFER_COARSEN = 0, 1, 2
@@ -36,7 +37,13 @@ class Desc(ctypes.Structure):
                 ("wave_k", ctypes.c_double * 3), ("wave_phi", ctypes.c_double),
                 ("hspeed", ctypes.c_double), ("flags", ctypes.c_int32),
                 ("pad", ctypes.c_int32), ("dcoef", ctypes.c_void_p),
-                ("lmode", ctypes.c_void_p), ("ldual", ctypes.c_void_p)]
+                ("lmode", ctypes.c_void_p), ("ldual", ctypes.c_void_p),
+                ("fslot", ctypes.c_void_p), ("nfactored", ctypes.c_int64),
+                ("fradial", ctypes.c_void_p), ("frmodes", ctypes.c_void_p),
+                ("frduals", ctypes.c_void_p), ("faslot", ctypes.c_void_p),
+                ("nangular", ctypes.c_int64), ("famodes", ctypes.c_void_p),
+                ("faweights", ctypes.c_void_p), ("fachol", ctypes.c_void_p),
+                ("rad_kappa", ctypes.c_double), ("rad_invR", ctypes.c_double)]
 
 
 class TransferBlock(ctypes.Structure):

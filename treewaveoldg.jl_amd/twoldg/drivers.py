@@ -154,13 +154,27 @@ def reflect_errors(D, P, E, **kw):
     return wave_errors(D, P, E, reflecting=True, **kw)
 
 
+def ball_forest(P, E, roots=(2, 2), Lc=1.0, Rm=None, R1=3.0, refined=False, grading=0.0):
+    """The seven-patch ball, or with a middle radius Rm the thirteen-patch one
+    (roots (n, m, m')), whose shells are not graded (sevenpatch.jl:119-129)."""
+    if Rm is None:
+        from .geometry import sevenpatch_forest
+        return sevenpatch_forest(P, E, roots=roots, Lc=Lc, R1=R1, refined=refined,
+                                 grading=grading)
+    if grading != 0:
+        raise ValueError(f"the thirteen-patch ball's shells are not graded, got grading = {grading}")
+    from .thirteenpatch import thirteenpatch_forest
+    return thirteenpatch_forest(P, E, roots=roots, Lc=Lc, Rm=Rm, R1=R1, refined=refined)
+
+
 def sevenpatch_errors(P, E, roots=(2, 2), Lc=1.0, R1=3.0, k=(1.1, -0.7, 0.5), phi=0.3,
                       cfl=0.5, tend=1.0, truncate=True, grading=0.0, precision="f64",
-                      nsteps=None):
-    """A plane wave through the seven-patch ball with the wave itself as the
-    Dirichlet data on the outer sphere (sevenpatch.jl:84-117)."""
-    from .geometry import sevenpatch_forest
-    forest, geo = sevenpatch_forest(P, E, roots=roots, Lc=Lc, R1=R1, grading=grading)
+                      nsteps=None, Rm=None, refined=False):
+    """A plane wave through the seven-patch ball (or, with Rm, the thirteen-patch
+    ball) with the wave itself as the Dirichlet data on the outer sphere
+    (sevenpatch.jl:84-117)."""
+    forest, geo = ball_forest(P, E, roots=roots, Lc=Lc, Rm=Rm, R1=R1, refined=refined,
+                              grading=grading)
     el = Element(P)
     wave = PlaneWave(k, phi)
     problem = WaveProblem(forest, el, truncate=truncate, precision=precision, geometry=geo,
@@ -174,6 +188,39 @@ def sevenpatch_errors(P, E, roots=(2, 2), Lc=1.0, R1=3.0, k=(1.1, -0.7, 0.5), ph
     integ.solve()
     energy = integ.energy()
     uexact = fill_at_nodes(sevenpatch_exact(wave, tend), problem)
+    problem.upload(uexact, _lib.SLOT_AUX)
+    l2, linf = _errors(problem, (1, 2))
+    l2u, _ = _errors(problem, (1,))
+    l2v, _ = _errors(problem, (2,))
+    h = forest.minimum_spacing() * 2 * Lc / roots[0]
+    return dict(l2=l2, l2u=l2u, l2v=l2v, linf=linf, h=h, he=(P + 1) * h, nsteps=nsteps, dt=dt,
+                tend=tend, nblocks=forest.nblocks, energy0=energy0, energy=energy,
+                problem=problem, integrator=integ)
+
+
+def radiative_errors(P, E, roots=(2, 2), Lc=1.0, R1=3.0, w=0.5, kappa=1.0, inverse_r=True,
+                     cfl=0.5, tend=2.0, truncate=True, refined=False, grading=0.0, Rm=None,
+                     precision="f64", nsteps=None):
+    """An OutgoingPulse of width w through a RadiativeBoundary on the seven-patch
+    ball (or, with Rm, the thirteen-patch one) to tend: the sqrt(g)-weighted L2
+    errors together and apart, the L-inf error, the cube's spacing, the steps and
+    the energies, the boundary's own included (sevenpatch.jl:187-236)."""
+    from .thirteenpatch import OutgoingPulse, RadiativeBoundary, pulse_state
+    forest, geo = ball_forest(P, E, roots=roots, Lc=Lc, Rm=Rm, R1=R1, refined=refined,
+                              grading=grading)
+    el = Element(P)
+    pulse = OutgoingPulse(w)
+    problem = WaveProblem(forest, el, truncate=truncate, precision=precision, geometry=geo,
+                          boundary=RadiativeBoundary(kappa, inverse_r))
+    u0 = initial_state(problem, pulse_state(pulse, 0.0))
+    if nsteps is None:
+        nsteps = math.ceil(tend / oldg_timestep(problem, cfl))
+    dt = tend / nsteps
+    energy0 = wave_energy(problem, u0)
+    integ = wave_integrator(problem, u0, 0.0, tend, dt)
+    integ.solve()
+    energy = integ.energy()
+    uexact = fill_at_nodes(pulse_state(pulse, tend), problem)
     problem.upload(uexact, _lib.SLOT_AUX)
     l2, linf = _errors(problem, (1, 2))
     l2u, _ = _errors(problem, (1,))

@@ -67,6 +67,14 @@ class SevenPatchBall:
         k, _ = shell_of(p)
         return tuple(self.m if j == k else self.n for j in range(3))
 
+    def outer_radius(self):
+        """The radius of the outer sphere, which a radiative boundary's u/R reads."""
+        return self.R1
+
+    def _element_blocks(self, forest):
+        """The blocks that take per-element truncation tables: every shell block."""
+        return [b for b, lf in enumerate(forest.leaves) if lf[0] != 0]
+
     # -- the map ------------------------------------------------------------------
     def _rho(self, eta):
         a = self.grading
@@ -180,7 +188,7 @@ class SevenPatchBall:
         wn = el.weights[kk[0]] * el.weights[kk[1]] * el.weights[kk[2]]
         x = forest.node_logical(el.offsets)
         slot = np.full(forest.nblocks, -1, dtype=np.int32)
-        shells = [b for b, lf in enumerate(forest.leaves) if lf[0] != 0]
+        shells = self._element_blocks(forest)
         slot[shells] = np.arange(len(shells), dtype=np.int32)
         ne = E ** 3
         weights = np.empty((len(shells), ne, n))

@@ -64,6 +64,11 @@ class WaveProblem:
             raise ValueError(f"precision must be one of {sorted(PRECISIONS)}, got {precision!r}")
         geometry = geometry if geometry is not None else forest.geometry
         full = geometry is not None and getattr(geometry, "kind", "full") == "full"
+        radiative = boundary is not None and hasattr(boundary, "kappa")
+        if radiative and not full:
+            raise ValueError(
+                "a radiative boundary forms the normal flux from the Cartesian gradient, which "
+                "a full geometry carries: pass a `FullGeometry` (`geometry`)")
         if boundary is not None and not full:
             raise ValueError(
                 "Dirichlet data (`boundary`) are given in Cartesian components and taken by a "
@@ -72,7 +77,8 @@ class WaveProblem:
         if full and boundary is None:
             raise ValueError(
                 "the geometry has outer faces, and no data were given for them: pass "
-                "`boundary = PlaneWave(k, phi)`, the Dirichlet data on the outer sphere")
+                "`boundary = PlaneWave(k, phi)`, the Dirichlet data on the outer sphere, or "
+                "`boundary = RadiativeBoundary()`")
         self.forest = forest
         self.element = el
         self.truncate = bool(truncate)
@@ -116,16 +122,38 @@ class WaveProblem:
             if self.truncate:
                 t = geometry.truncation_tables(forest, el)
                 self._tables = t
-                desc.tslot = _ptr(t["slot"])
-                desc.ntables = t["ntables"]
-                desc.tmodes = _ptr(t["modes"])
-                desc.tweights = _ptr(t["weights"])
-                desc.tchol = _ptr(t["chol"])
-            self.dirichlet_x = geometry.face_positions(forest, el)
-            desc.dirichlet_x = _ptr(self.dirichlet_x)
-            for c in range(3):
-                desc.wave_k[c] = boundary.k[c]
-            desc.wave_phi = boundary.phi
+                if t["ntables"]:
+                    desc.tslot = _ptr(t["slot"])
+                    desc.ntables = t["ntables"]
+                    desc.tmodes = _ptr(t["modes"])
+                    desc.tweights = _ptr(t["weights"])
+                    desc.tchol = _ptr(t["chol"])
+                if "fslot" in t:       # the factored truncation of exactly radial shells
+                    desc.fslot = _ptr(t["fslot"])
+                    desc.nfactored = t["nfactored"]
+                    desc.fradial = _ptr(t["fradial"])
+                    desc.frmodes = _ptr(t["frmodes"])
+                    desc.frduals = _ptr(t["frduals"])
+                    desc.faslot = _ptr(t["faslot"])
+                    desc.nangular = t["nangular"]
+                    desc.famodes = _ptr(t["famodes"])
+                    desc.faweights = _ptr(t["faweights"])
+                    desc.fachol = _ptr(t["fachol"])
+            if radiative:
+                # the outer faces take the radiative ghosts (radiative_faces,
+                # evolution.jl:1191-1203)
+                for i in range(len(self._faces)):
+                    if self._faces[i].kind == _lib.FACE_DIRICHLET:
+                        self._faces[i].kind = _lib.FACE_RADIATIVE
+                desc.rad_kappa = boundary.kappa
+                desc.rad_invR = boundary.invR(geometry)
+                self.rad_invR = desc.rad_invR
+            else:
+                self.dirichlet_x = geometry.face_positions(forest, el)
+                desc.dirichlet_x = _ptr(self.dirichlet_x)
+                for c in range(3):
+                    desc.wave_k[c] = boundary.k[c]
+                desc.wave_phi = boundary.phi
         self._desc = desc
         handle = ctypes.c_void_p()
         _lib.check(_lib.lib().twoldg_problem_create(ctypes.byref(desc), ctypes.byref(handle)))
@@ -301,7 +329,7 @@ def initial_state(p, f):
 def wave_energy(p, u, t=None):
     """The discrete energy 1/2 sum w (v^2 + sum_d (D_d u)^2), on a full geometry
     weighted by sqrt(g) with the Cartesian gradient."""
-    if p.boundary is not None and t is None:
+    if p.boundary is not None and not hasattr(p.boundary, "kappa") and t is None:
         raise ValueError("the problem has Dirichlet data, which depend on the time, and none "
                          "was given: pass `t`")
     p.upload(u, _lib.SLOT_ARG)
