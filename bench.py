@@ -306,6 +306,71 @@ def annulus(s=8):
     return out
 
 
+# The reference's own GPU benchmark figures (CODE.md:5108-5122, one H200,
+# ns per node and RK4 step, the least of five): its flat box and its ball.
+REFERENCE_H200_NS = {"flatbox16": {"f64": 1.61, "f32c": 1.26},
+                     "benchmark-ball": {"f64": 2.84, "f32c": 1.92}}
+
+
+def flatbox16(steps=50):
+    """The reference's flat-box benchmark (bin/benchmark.jl, CODE.md:4998-5007):
+    3D periodic, P = 3, 4096 blocks of 16^3 (E = 4, roots = 16), 16.8M nodes,
+    standing sine mode, truncation on; ns per node and step beside the
+    reference's published H200 figure."""
+    import twoldg as T
+    forest = T.wave_forest(3, 3, 4, roots=16)
+    nodes = forest.nblocks * forest.N ** 3
+    out = {}
+    for prec in ("f64", "f32c"):
+        prob = T.WaveProblem(forest, T.Element(3), precision=prec)
+        u0 = T.initial_state(prob, T.wave_exact(3, 1.0, 1, 0.0))
+        dt = T.oldg_timestep(prob, 0.5)
+        wall, _, _, _ = timed_integrator(T, prob, u0, dt, steps)
+        ns = wall / steps / nodes * 1e9
+        ref = REFERENCE_H200_NS["flatbox16"][prec]
+        out[prec] = dict(value=nodes * steps / wall, ns_per_node_step=ns,
+                         reference_h200_ns=ref, speedup_vs_reference_h200=ref / ns)
+        log(f"[bench] flat box 4096 x 16^3 P=3 {prec}: {ns:.4f} ns per node and step "
+            f"(reference H200 {ref}), {nodes * steps / wall:.3e} node-updates/s")
+        prob.close()
+    out["nodes"] = nodes
+    return out
+
+
+def benchmark_ball(steps=10):
+    """The reference's benchmark ball (CODE.md:5056-5066): the thirteen-patch
+    ball Lc = 4, Rm = 12, R1 = 100, roots (4, 4, 44), P = 3, E = 4 (4672 blocks
+    of 16^3, 1.91e7 nodes), radiative sphere with u/R, truncation on (per-element
+    tables on the inner shells, factored on the outer ones); an outgoing pulse of
+    width 1; ns per node and step beside the reference's published H200 figure."""
+    import twoldg as T
+    t0 = time.perf_counter()
+    forest, geo = T.thirteenpatch_forest(3, 4, roots=(4, 4, 44), Lc=4.0, Rm=12.0, R1=100.0)
+    nodes = forest.nblocks * forest.N ** 3
+    out = {}
+    for prec in ("f64", "f32c"):
+        prob = T.WaveProblem(forest, T.Element(3), geometry=geo, precision=prec,
+                             boundary=T.RadiativeBoundary())
+        build = time.perf_counter() - t0
+        from twoldg.thirteenpatch import pulse_state
+        u0 = T.initial_state(prob, pulse_state(T.OutgoingPulse(1.0), 0.0))
+        dt = T.oldg_timestep(prob, 0.5)
+        wall, _, _, _ = timed_integrator(T, prob, u0, dt, steps)
+        ns = wall / steps / nodes * 1e9
+        ref = REFERENCE_H200_NS["benchmark-ball"][prec]
+        out[prec] = dict(value=nodes * steps / wall, ns_per_node_step=ns, dt=dt,
+                         ms_per_step=wall * 1e3 / steps, reference_h200_ns=ref,
+                         speedup_vs_reference_h200=ref / ns, build_s=build)
+        log(f"[bench] benchmark ball (13 patches, 4672 x 16^3) {prec}: {ns:.4f} ns per node and "
+            f"step (reference H200 {ref}), {wall * 1e3 / steps:.2f} ms per step, dt {dt:.3e}, "
+            f"host build {build:.1f} s")
+        prob.close()
+        t0 = time.perf_counter()
+    out["nodes"] = nodes
+    out["blocks"] = forest.nblocks
+    return out
+
+
 # The side workloads by name, each at the size the default run reports.
 WORKLOADS = {
     "periodic2d": lambda: periodic2d(),
@@ -314,6 +379,8 @@ WORKLOADS = {
     "batch2d": lambda: batch2d(),
     "ball": lambda: ball(),
     "annulus": lambda: annulus(),
+    "flatbox16": lambda: flatbox16(),
+    "benchmark-ball": lambda: benchmark_ball(),
 }
 
 

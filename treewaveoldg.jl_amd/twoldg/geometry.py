@@ -40,6 +40,20 @@ def shell_index(k, s):
     return 1 + 2 * k + (1 if s == 1 else 0)
 
 
+def _largest_eigenvalue(A):
+    """The largest eigenvalue of each symmetric 3x3 matrix of A (n, 3, 3), in
+    closed form: with q = tr A / 3 and B = (A - q I) / p, p^2 = |A - q I|_F^2 / 6,
+    it is q + 2 p cos(acos(det B / 2) / 3)."""
+    q = np.trace(A, axis1=1, axis2=2) / 3
+    off = A[:, 0, 1] ** 2 + A[:, 0, 2] ** 2 + A[:, 1, 2] ** 2
+    p2 = ((A[:, 0, 0] - q) ** 2 + (A[:, 1, 1] - q) ** 2 + (A[:, 2, 2] - q) ** 2 + 2 * off) / 6
+    p = np.sqrt(p2)
+    safe = np.where(p > 0, p, 1.0)
+    B = (A - q[:, None, None] * np.eye(3)) / safe[:, None, None]
+    r = np.clip(np.linalg.det(B) / 2, -1.0, 1.0)
+    return np.where(p > 0, q + 2 * p * np.cos(np.arccos(r) / 3), q)
+
+
 class SevenPatchBall:
     """The ball |x| <= R1 as a cube of half-width Lc and six shells; roots =
     (n, m); grading alpha widens the shells' radial elements outward."""
@@ -137,10 +151,14 @@ class SevenPatchBall:
             for d in range(3):
                 S[:, d, d] = a * a
             return J, S
-        det = np.linalg.det(M)
-        J = np.abs(det)
-        S = J[:, None, None] * np.linalg.inv(M)
-        return J, S
+        # S[d] = J (M^-1)[d, :] = sign(det) (m_{d+1} x m_{d+2}), m_j the columns of M
+        m = [M[:, :, j] for j in range(3)]
+        S = np.empty_like(M)
+        for d in range(3):
+            S[:, d, :] = np.cross(m[(d + 1) % 3], m[(d + 2) % 3])
+        det = np.einsum("ni,ni->n", m[0], S[:, 0, :])
+        sg = np.sign(det)
+        return np.abs(det), S * sg[:, None, None]
 
     def physical(self, forest, x):
         """Physical positions of logical node positions x (nb, 3, nn)."""
@@ -152,17 +170,39 @@ class SevenPatchBall:
     # -- the full metric's tables ---------------------------------------------------
     def metric(self, forest, el):
         """Coefficients (nb, 1 + 9, nn): sqrt(g) then S[d][i] at 1 + 3d + i; flat
-        flags; hspeed = min_b h_b / max speed (block_speeds)."""
+        flags; hspeed = min_b h_b / max speed (block_speeds).  Kept for the forest's
+        leaf set and P, so that problems of several precisions share one build."""
+        key = ("metric", id(forest.leaves), el.P)
+        cache = self.__dict__.setdefault("_built", {})
+        if key not in cache:
+            cache.clear()
+            cache[key] = (forest.leaves, self._metric(forest, el))
+        return cache[key][1]
+
+    def _metric(self, forest, el):
         x = forest.node_logical(el.offsets)
         nb, _, nn = x.shape
         coef = np.empty((nb, 10, nn))
         speed = np.empty(nb)
+        # the frames of many blocks of one patch per call (position and Jacobian
+        # take any number of points)
+        by_patch = {}
         for b, lf in enumerate(forest.leaves):
-            J, S = self.frame(lf[0], x[b])
-            coef[b, 0] = J
-            coef[b, 1:] = S.reshape(nn, 9).T
-            g = np.einsum("nai,nci->nac", S, S) / (J * J)[:, None, None]
-            speed[b] = math.sqrt(np.linalg.eigvalsh(g)[:, -1].max())
+            by_patch.setdefault(lf[0], []).append(b)
+        chunk = max(1, (1 << 20) // nn)
+        for p, blocks in by_patch.items():
+            for i in range(0, len(blocks), chunk):
+                bs = blocks[i:i + chunk]
+                xs = np.moveaxis(x[bs], 1, 0).reshape(3, -1)
+                J, S = self.frame(p, xs)
+                J = J.reshape(len(bs), nn)
+                S = S.reshape(len(bs), nn, 9)
+                coef[bs, 0] = J
+                coef[bs, 1:] = np.moveaxis(S, 2, 1)
+                S3 = S.reshape(-1, 3, 3)
+                g = np.einsum("nai,nci->nac", S3, S3) / (J.reshape(-1) ** 2)[:, None, None]
+                lam = _largest_eigenvalue(g).reshape(len(bs), nn)
+                speed[bs] = np.sqrt(lam.max(axis=1))
         flat = np.zeros(nb, dtype=np.uint8)
         offdiag = [1 + 3 * d + i for d in range(3) for i in range(3) if d != i]
         for b in range(nb):
@@ -244,18 +284,44 @@ class SevenPatchBall:
         return shell_index(d, t), k, 1 if sg == 1 else 0
 
     def _face_corners(self, forest, b, d, s):
+        """The physical corners of block b's face (d, s): (0,0), (1,0), (0,1), (1,1) in
+        its tangential axes, one position call."""
         p, lev, *c = forest.leaves[b]
         ext = forest.extent(forest.leaves[b])
         tdims = [e for e in range(3) if e != d]
-        pts = []
-        for c1 in (0, 1):
-            for c0 in (0, 1):
-                xi = np.zeros((3, 1))
-                xi[d, 0] = ext[d][1] if s else ext[d][0]
-                xi[tdims[0], 0] = ext[tdims[0]][c0]
-                xi[tdims[1], 0] = ext[tdims[1]][c1]
-                pts.append(self.position(p, xi)[:, 0])
-        return np.array(pts)  # corners (0,0), (1,0), (0,1), (1,1)
+        xi = np.zeros((3, 4))
+        xi[d] = ext[d][1] if s else ext[d][0]
+        xi[tdims[0]] = [ext[tdims[0]][0], ext[tdims[0]][1]] * 2
+        xi[tdims[1]] = [ext[tdims[1]][0]] * 2 + [ext[tdims[1]][1]] * 2
+        return self.position(p, xi).T
+
+    def _centre_key(self, corners):
+        return tuple(np.round(corners.mean(axis=0) / (1e-6 * self.R1)).astype(np.int64))
+
+    def _face_blocks(self, forest, q, lev, dq, sq):
+        """The leaves of patch q at level lev on that patch's face (dq, sq), from an
+        index built once per forest leaf set."""
+        return self._face_index(forest)[0].get((q, lev, dq, sq), [])
+
+    def _face_index(self, forest):
+        """Per (patch, level, face) the leaves on that patch face, and per such face
+        its corners keyed by their rounded centre; built once per leaf set."""
+        idx = getattr(forest, "_face_index_cache", None)
+        if idx is None or idx[0] is not forest.leaves:
+            table, centres = {}, {}
+            for cb, lf in enumerate(forest.leaves):
+                p, l, *c = lf
+                n = forest.dims(p, l)
+                for e in range(3):
+                    for side in (0, 1):
+                        if c[e] != (n[e] - 1 if side else 0):
+                            continue
+                        table.setdefault((p, l, e, side), []).append(cb)
+                        corners = self._face_corners(forest, cb, e, side)
+                        centres[(p, l, e, side) + self._centre_key(corners)] = (cb, corners)
+            idx = (forest.leaves, table, centres)
+            forest._face_index_cache = idx
+        return idx[1], idx[2]
 
     def glue_face(self, forest, b, d, s, F):
         p = forest.leaves[b][0]
@@ -267,13 +333,13 @@ class SevenPatchBall:
         mine = self._face_corners(forest, b, d, s)
         tol = 1e-9 * self.R1
         lev = forest.leaves[b][1]
-        for cb, lf in enumerate(forest.leaves):
-            if lf[0] != q or lf[1] != lev:
-                continue
-            n = forest.dims(q, lev)
-            if lf[2 + dq] != (n[dq] - 1 if sq else 0):
-                continue
-            theirs = self._face_corners(forest, cb, dq, sq)
+        table, centres = self._face_index(forest)
+        hit = centres.get((q, lev, dq, sq) + self._centre_key(mine))
+        cands = [hit] if hit is not None else []
+        cands += [(cb, None) for cb in table.get((q, lev, dq, sq), [])]
+        for cb, theirs in cands:
+            if theirs is None:
+                theirs = self._face_corners(forest, cb, dq, sq)
             match = []
             for c in mine:
                 dist = np.linalg.norm(theirs - c, axis=1)
