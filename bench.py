@@ -34,6 +34,9 @@ PEAK_HBM_GBS = 8000.0
 # u and v, read and written at stage 4 (4 more values).
 VALUES_PER_NODE_UPDATE = {"f64": 34, "f32c": 38}
 BYTES_PER_VALUE = {"f64": 8, "f32c": 4}
+# The split form (u in FP64, v in FP32) moves the same 34 values, u's 17 in
+# FP64 and v's 17 in FP32, plus the low word of v read and written at stage 4.
+BYTES_PER_NODE_UPDATE = {"f64": 272, "f32c": 152, "split": 17 * 8 + 17 * 4 + 2 * 4}
 
 
 # HBM bytes per launch of the stage kernel on the default workload, measured with
@@ -83,7 +86,7 @@ def gpu_run(args, precision):
     value = nodes * args.steps / wall
     log(f"[bench] {precision}: {args.steps} steps in {wall * 1e3:.1f} ms (events {ms:.1f} ms), "
         f"stage kernels {kernel_ms:.1f} ms over {launches} launches -> {value:.3e} node-updates/s")
-    alg_bytes = VALUES_PER_NODE_UPDATE[precision] * BYTES_PER_VALUE[precision] * nodes * args.steps
+    alg_bytes = BYTES_PER_NODE_UPDATE[precision] * nodes * args.steps
     achieved = alg_bytes / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else None
     res = dict(value=value, ms_per_step=wall * 1e3 / args.steps, kernel_ms=kernel_ms,
                launches=launches, achieved_gbs=achieved,
@@ -208,7 +211,7 @@ def nested_cubes(large=False, steps=500):
         wall, kms, nl, integ = timed_integrator(T, prob, u0, dt, steps - warm, warmup=warm)
         e1 = integ.energy()
         timed = steps - warm
-        alg = VALUES_PER_NODE_UPDATE[prec] * BYTES_PER_VALUE[prec] * nodes * timed
+        alg = BYTES_PER_NODE_UPDATE[prec] * nodes * timed
         final[prec] = integ.u
         out[prec] = dict(value=nodes * timed / wall, ms_per_step=wall * 1e3 / timed,
                          alg_gbs_whole_step=alg / wall / 1e9, energy_drift=(e1 - e0) / e0,
@@ -244,7 +247,7 @@ def batch2d(steps=20):
         ms = batch.rk4(steps)
         info = batch.info()
         nodes = info["nodes"]
-        alg = VALUES_PER_NODE_UPDATE[prec] * BYTES_PER_VALUE[prec] * nodes * steps
+        alg = BYTES_PER_NODE_UPDATE[prec] * nodes * steps
         out[prec] = dict(value=nodes * steps / (ms * 1e-3), ms_per_step=ms / steps,
                          alg_gbs=alg / (ms * 1e-3) / 1e9, tiles=info["tiles"],
                          workgroups_per_launch=info["workgroups"], setup_s=setup)
@@ -389,7 +392,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--precision", default="f64", choices=["f64", "f32c"])
+    ap.add_argument("--precision", default="f64", choices=["f64", "f32c", "split"])
     ap.add_argument("--accuracy-steps", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--single", action="store_true", help="time only --precision's form")
@@ -406,8 +409,8 @@ def main():
         print(json.dumps(WORKLOADS[args.workload](), default=float))
         return
     main_res = gpu_run(args, args.precision)
-    other = "f32c" if args.precision == "f64" else "f64"
-    other_res = {} if args.single else gpu_run(args, other)
+    others = {} if args.single else {
+        f: gpu_run(args, f) for f in ("f64", "f32c", "split") if f != args.precision}
     configs = {}
     if not args.no_configs:
         for name, run in WORKLOADS.items():
@@ -443,16 +446,19 @@ def main():
                    "global_batch": main_res["nodes"], "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "forms": {prec: {k: main_res.get(k) for k in ("value", "l2", "linf", "energy_drift")},
-                  other: {k: other_res.get(k) for k in ("value", "l2", "linf", "energy_drift")}},
+        "forms": {f: {k: r.get(k) for k in ("value", "l2", "linf", "energy_drift", "achieved_gbs")}
+                  for f, r in [(prec, main_res)] + list(others.items())},
     }
     if configs:
         out["configs"] = configs
-    if main_res.get("l2") and other_res.get("l2"):
-        f64 = main_res if prec == "f64" else other_res
-        f32 = other_res if prec == "f64" else main_res
-        out["forms"]["f32c_over_f64_l2"] = f32["l2"] / f64["l2"]
-        out["forms"]["f32c_over_f64_speed"] = f32["value"] / f64["value"]
+    allres = dict(others)
+    allres[prec] = main_res
+    if "f64" in allres and allres["f64"].get("l2"):
+        f64 = allres["f64"]
+        for f in ("f32c", "split"):
+            if f in allres and allres[f].get("l2"):
+                out["forms"][f + "_over_f64_l2"] = allres[f]["l2"] / f64["l2"]
+                out["forms"][f + "_over_f64_speed"] = allres[f]["value"] / f64["value"]
     print(json.dumps(out), flush=True)
 
 

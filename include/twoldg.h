@@ -40,6 +40,9 @@ extern "C" {
 /* arithmetic forms */
 #define TWOLDG_F64 0   /* FP64 throughout, the reference's structure */
 #define TWOLDG_F32C 1  /* FP32 stencil and storage, FP32 hi+lo (Kahan) RK step, FP64 reductions */
+#define TWOLDG_SPLIT 2 /* u and its stencil in FP64, v in FP32 with its step update added to an
+                          FP32 (hi, lo) pair: the fused 3D stages only (the state crosses the
+                          boundary and every other entry point as FP64) */
 
 /* face kinds: where the ghost plane of a block face comes from */
 #define TWOLDG_FACE_SAME 1      /* same-level neighbour (periodic wrap included); nb[0] */
@@ -76,7 +79,7 @@ typedef struct twoldg_face {
 typedef struct twoldg_desc {
   int32_t abi_version;   /* TWOLDG_ABI_VERSION */
   int32_t D, P, E;       /* N = E (P+1) nodes per block per dimension, E even */
-  int32_t precision;     /* TWOLDG_F64 or TWOLDG_F32C */
+  int32_t precision;     /* TWOLDG_F64, TWOLDG_F32C or TWOLDG_SPLIT */
   int32_t truncate;      /* 1: v held at degree P-1 (the paper's scheme), 0: plain */
   int64_t nblocks;
   const double* spacing;        /* [nblocks] h per block (block extent / N) */

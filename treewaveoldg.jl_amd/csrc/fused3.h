@@ -79,13 +79,22 @@ __device__ __forceinline__ void st_once(T* p, T v) {
     *p = v;
 }
 
-template <class T>
+// The v arrays may be of another type TV and layout than u's: the split form
+// (u in FP64, v in FP32) keeps v in separate (nb, N^3) arrays (vstride N^3);
+// the other forms point them at the v halves of the state arrays (vstride 2 N^3).
+template <class T, class TV = T>
 struct FusedArgs {
   const T* ys;       // stage input (u, v)
   const T* yn;       // step start (the same as ys in mode 1)
   T* acc;            // RK accumulator
   T* yo;             // output state
-  T* lo;             // compensation of yn (final stage, compensated form)
+  T* lo;             // compensation of yn's u (final stage, compensated form)
+  const TV* vs;      // v of ys, of yn (its high word in the split form), of the
+  const TV* vn;      //   accumulator and of the output, block 0's; block b at
+  TV* vacc;          //   + b vstride
+  TV* vo;
+  TV* vlo;           // compensation of yn's v (final stage, compensated and split forms)
+  int64_t vstride;
   const T* uf_in;    // faces of ys: (nb, 6, N^2) u on the face plane
   const T* pf_in;    //              block-local part of the face gradient
   T* uf_out;         // faces of yo
@@ -127,9 +136,13 @@ constexpr FusedTab<T, P1> fused_tab() {
 // Face ghosts are formed here from a same-level neighbour's face buffers (nbr
 // >= 0) or from the own face on a mirror (-1); with GHOST, a coarse-fine face
 // (-2) takes them complete from k_face_ghost_u/_g.
-template <class T, int P1, int E, int MODE, bool GHOST>
+// MODE 1: stage 1 (ys = yn); 2: stages 2 and 3; 3: the final stage; 4: the
+// final stage of the compensated form (u by a Kahan sum into (yn, lo), v's
+// truncated increment likewise into (vn, vlo)); 5: the final stage of the split
+// form (u plain in FP64, v's increment added to the FP32 pair (vn, vlo) in FP64).
+template <class T, int P1, int E, int MODE, bool GHOST, class TV = T>
 __global__ void __launch_bounds__(((P1 * E) * (P1 * E) + 63) / 64 * 64, (fused_waves_per_eu<T, P1, E>()))
-k_fused3(FusedArgs<T> A) {
+k_fused3(FusedArgs<T, TV> A) {
   constexpr FusedTab<T, P1> tb = fused_tab<T, P1>();
   constexpr int N = P1 * E;
   constexpr int NT = N * N;  // = E * N * P1: one element line per thread and role
@@ -174,11 +187,16 @@ k_fused3(FusedArgs<T> A) {
   // v at +nn)
   const int col = y * N + x;
   const T* us = A.ys + b * 2 * nn;
-  const T* vs = us + nn;
   const T* un_ = A.yn + b * 2 * nn;
   T* acc = A.acc + b * 2 * nn;
   T* yo = A.yo + b * 2 * nn;
   T* lo = A.lo ? A.lo + b * 2 * nn : nullptr;
+  // v at the same per-thread offsets as u
+  const TV* vs = A.vs + b * A.vstride;
+  const TV* vn = A.vn + b * A.vstride;
+  TV* vacc = A.vacc + b * A.vstride;
+  TV* vo = A.vo + b * A.vstride;
+  TV* vlo = A.vlo ? A.vlo + b * A.vstride : nullptr;
   T* ufo = A.uf_out + b * 6 * N2;
   T* pfo = A.pf_out + b * 6 * N2;
   const int nxl = ld_table(nb6 + 0), nxh = ld_table(nb6 + 1), nyl = ld_table(nb6 + 2),
@@ -440,19 +458,20 @@ k_fused3(FusedArgs<T> A) {
       T w[P1];
       T plo = 0, phi = 0;  // partial y-face gradients of the next u
       auto ops = [&](int j, T& ku, T& yu, T& yv, T& au, T& av) {
-        const int iu = base + j * N, iv = iu + nn;
-        ku = ld_once(vs + iu);
+        const int iu = base + j * N;
+        ku = (T)ld_once(vs + iu);
         if (MODE == 1) {
           // stage 1: the stage input is the step start (ys = yn), whose u is
           // the layer's plane in LDS (this thread's node, not yet overwritten)
           yu = U[py * NT + (ey * P1 + j) * N + xy];
           yv = ku;
         } else {
-          yu = ld_once(un_ + iu);                                 // compensated form: the high word
-          yv = MODE == 4 ? ld_once(lo + iu) : ld_once(un_ + iv);  // compensated form: the low word
+          yu = ld_once(un_ + iu);  // compensated form: the high word
+          // compensated form: u's low word; split form: unused
+          yv = MODE == 4 ? ld_once(lo + iu) : (MODE == 5 ? T(0) : (T)ld_once(vn + iu));
         }
         au = MODE >= 2 ? ld_once(acc + iu) : T(0);
-        av = MODE >= 2 ? ld_once(acc + iv) : T(0);
+        av = MODE >= 2 ? (T)ld_once(vacc + iu) : T(0);
       };
       // the stage operands one node ahead where registers allow
       constexpr bool ahead = sizeof(T) == 4 || TWOLDG_FUSED_NB_F64 > 0;
@@ -463,38 +482,41 @@ k_fused3(FusedArgs<T> A) {
         T ku1 = 0, yu1 = 0, yv1 = 0, au1 = 0, av1 = 0;
         if (!ahead) ops(j, ku, yu, yv, au, av);
         if (ahead && j + 1 < P1) ops(j + 1, ku1, yu1, yv1, au1, av1);
-        const int iu = base + j * N, iv = iu + nn;
+        const int iu = base + j * N;
         const int l = py * NT + (ey * P1 + j) * N + xy;
         const T lap = ((T(0) + LX[l]) + ly[j]) + LZ[l];
         const T kv = lap * rh;
         T unext;
         // the accumulator (modes 1, 2) or the compensation of u (mode 4): stored
         // now, or left in LX and LZ at l (read above by this thread only)
-        auto out = [&](T* pu, T* pv, T ou, T ov) {
+        auto out = [&](T* pu, bool with_v, T ou, T ov) {
           if (defer) {
             LX[l] = ou;
-            if (pv) LZ[l] = ov;
+            if (with_v) LZ[l] = ov;
           } else {
             st_once(pu + iu, ou);
-            if (pv) st_once(pv + iu, ov);
+            if (with_v) st_once(vacc + iu, (TV)ov);
           }
         };
         if (MODE == 1) {
-          out(acc, acc + nn, ku, kv);
+          out(acc, true, ku, kv);
           unext = yu + A.a * ku;
           w[j] = yv + A.a * kv;
         } else if (MODE == 2) {
-          out(acc, acc + nn, au + T(2) * ku, av + T(2) * kv);
+          out(acc, true, au + T(2) * ku, av + T(2) * kv);
           unext = yu + A.a * ku;
           w[j] = yv + A.a * kv;
         } else if (MODE == 3) {
           unext = yu + A.a * (au + ku);
           w[j] = yv + A.a * (av + kv);
-        } else {  // compensated: u by Kahan now, v's increment truncated first
+        } else if (MODE == 4) {  // compensated: u by Kahan now, v's increment truncated first
           const T du = A.a * (au + ku);
           const T yk = du - yv, tu = yu + yk;
-          out(lo, nullptr, (tu - yu) - yk, T(0));
+          out(lo, false, (tu - yu) - yk, T(0));
           unext = tu;
+          w[j] = A.a * (av + kv);
+        } else {  // split: u plain, v's increment truncated first
+          unext = yu + A.a * (au + ku);
           w[j] = A.a * (av + kv);
         }
         if (!defer) st_once(yo + iu, unext);
@@ -523,13 +545,13 @@ k_fused3(FusedArgs<T> A) {
     lds_barrier();
     // ---- (4) x lines: v's truncation along x, faces of the next u along x;
     // columns: faces of the next u along z ----
-    T lov[P1], hv[P1];  // the compensated form's v operands, ahead of this phase's stores
-    if (act && MODE == 4) {
+    TV lov[P1], hv[P1];  // the final stage's v pair, ahead of this phase's stores
+    if (act && (MODE == 4 || MODE == 5)) {
 #pragma unroll
       for (int j = 0; j < P1; ++j) {
-        const int iv = col + (e * P1 + j) * N2 + nn;
-        lov[j] = ld_once(lo + iv);
-        hv[j] = ld_once(un_ + iv);
+        const int iv = col + (e * P1 + j) * N2;
+        lov[j] = ld_once(vlo + iv);
+        hv[j] = ld_once(vn + iv);
       }
     }
     if (act) {
@@ -586,22 +608,28 @@ k_fused3(FusedArgs<T> A) {
       if (A.truncate) truncate_line(wl);
 #pragma unroll
       for (int j = 0; j < P1; ++j) {
-        const int iv = col + (e * P1 + j) * N2 + nn;
+        const int iu = col + (e * P1 + j) * N2;
         if (defer) {  // the stage update's outputs of the column
           const int li = j * NT + col;
-          st_once(yo + iv - nn, UN[li]);
+          st_once(yo + iu, UN[li]);
           if (MODE == 1 || MODE == 2) {
-            st_once(acc + iv - nn, LX[li]);
-            st_once(acc + iv, LZ[li]);
+            st_once(acc + iu, LX[li]);
+            st_once(vacc + iu, (TV)LZ[li]);
           }
-          if (MODE == 4) st_once(lo + iv - nn, LX[li]);
+          if (MODE == 4) st_once(lo + iu, LX[li]);
         }
         if (MODE == 4) {
-          const T yk = wl[j] - lov[j], tv = hv[j] + yk;
-          st_once(lo + iv, (tv - hv[j]) - yk);
-          st_once(yo + iv, tv);
+          const T yk = wl[j] - (T)lov[j], tv = (T)hv[j] + yk;
+          st_once(vlo + iu, (TV)((tv - (T)hv[j]) - yk));
+          st_once(vo + iu, (TV)tv);
+        } else if (MODE == 5) {
+          // v_n = hi - lo exactly in T; the new pair rounds v_n + increment
+          const T vt = ((T)hv[j] - (T)lov[j]) + wl[j];
+          const TV hi = (TV)vt;
+          st_once(vlo + iu, (TV)((T)hi - vt));
+          st_once(vo + iu, hi);
         } else {
-          st_once(yo + iv, wl[j]);
+          st_once(vo + iu, (TV)wl[j]);
         }
       }
     }

@@ -144,14 +144,14 @@ int twoldg_i_transfer_f32_3(twoldg_problem*, int, const twoldg_problem*, int,
 namespace {
 
 #define TW_DISPATCH(fn, ...)                                                              \
-  (p->precision == TWOLDG_F64                                                             \
+  (p->precision != TWOLDG_F32C                                                            \
        ? (p->D == 1 ? twoldg_i_##fn##_f64_1(__VA_ARGS__)                                   \
                     : p->D == 2 ? twoldg_i_##fn##_f64_2(__VA_ARGS__) : twoldg_i_##fn##_f64_3(__VA_ARGS__)) \
        : (p->D == 1 ? twoldg_i_##fn##_f32_1(__VA_ARGS__)                                   \
                     : p->D == 2 ? twoldg_i_##fn##_f32_2(__VA_ARGS__) : twoldg_i_##fn##_f32_3(__VA_ARGS__)))
 
 #define TW_DISPATCH_P(q, fn, ...)                                                         \
-  ((q)->precision == TWOLDG_F64                                                           \
+  ((q)->precision != TWOLDG_F32C                                                          \
        ? ((q)->D == 1 ? twoldg_i_##fn##_f64_1(__VA_ARGS__)                                 \
                       : (q)->D == 2 ? twoldg_i_##fn##_f64_2(__VA_ARGS__)                   \
                                     : twoldg_i_##fn##_f64_3(__VA_ARGS__))                  \
@@ -169,7 +169,8 @@ void destroy(twoldg_problem* p) {
                   (void*)p->cl, p->cval_u, p->cval_g, (void*)p->dir_rows, (void*)p->dir_x,
                   (void*)p->part, (void*)p->red, (void*)p->clock, p->dcoef, p->lmode, p->ldual, (void*)p->fslot, (void*)p->fradial,
                   p->frmodes, p->frduals, (void*)p->faslot, p->famodes, p->faweights, p->faginv,
-                  (void*)p->rad_rows, (void*)p->rpart, (void*)p->nbr6, p->uf[0], p->uf[1], p->pf[0],
+                  (void*)p->rad_rows, (void*)p->rpart, (void*)p->svn, (void*)p->svlo,
+                  (void*)p->svs, (void*)p->svacc, (void*)p->nbr6, p->uf[0], p->uf[1], p->pf[0],
                   p->pf[1], (void*)p->frows, (void*)p->fsrc, p->fwu,
                   p->fwg, p->fgu, p->fgg, p->ffg})
     dfree(b);
@@ -199,7 +200,7 @@ int check_desc(const twoldg_desc* d) {
                     " per block is odd, and E must be even: a block of a refined forest covers "
                     "half of its parent and has to cover whole elements of it, so N must be a "
                     "multiple of 2(P+1) = " + std::to_string(2 * (d->P + 1)));
-  if (d->precision != TWOLDG_F64 && d->precision != TWOLDG_F32C)
+  if (d->precision != TWOLDG_F64 && d->precision != TWOLDG_F32C && d->precision != TWOLDG_SPLIT)
     return fail(TWOLDG_EINVAL, "unknown precision " + std::to_string(d->precision));
   if (d->nblocks < 1 || !d->spacing || !d->faces)
     return fail(TWOLDG_EINVAL, "a problem needs at least one block, its spacings and faces");
@@ -488,6 +489,21 @@ int create_buffers(twoldg_problem* p, const twoldg_desc* d, const Csr& csr) {
     }
     p->fused = true;
   }
+  if (p->precision == TWOLDG_SPLIT) {
+    // the split form is a form of the fused stages: v of the step start as an
+    // FP32 pair, of the stage state and the accumulator in FP32
+    if (!p->fused)
+      return fail(TWOLDG_EINVAL,
+                  "the split form (u in FP64, v in FP32) runs the fused 3D stages, which take "
+                  "flat forests whose faces are same-level, periodic, reflecting or "
+                  "coarse-fine, with E = 2 (P <= 15) or E = 4 (P <= 5): use FP64 or FP32C "
+                  "for this problem");
+    const size_t nv = (size_t)p->sh.nb * p->sh.nn;
+    if ((rc = dalloc(p, &p->svn, nv))) return rc;
+    if ((rc = dalloc(p, &p->svlo, nv))) return rc;
+    if ((rc = dalloc(p, &p->svs, nv))) return rc;
+    if ((rc = dalloc(p, &p->svacc, nv))) return rc;
+  }
   if ((rc = dalloc(p, &p->part, 3 * p->sh.nb))) return rc;
   if ((rc = dalloc(p, &p->red, 4))) return rc;
   if ((rc = dalloc(p, &p->clock, 2))) return rc;
@@ -588,7 +604,7 @@ int twoldg_problem_create(const twoldg_desc* d, twoldg_problem** out) {
   p->truncate = d->truncate != 0;
   p->no_fused = (d->flags & TWOLDG_FLAG_GENERIC) != 0;
   p->full = d->coef != nullptr;
-  p->elt = d->precision == TWOLDG_F64 ? 8 : 4;
+  p->elt = d->precision == TWOLDG_F32C ? 4 : 8;
   p->sh.N = p->N;
   p->sh.nn = ipow(p->N, p->D);
   p->sh.Nt = ipow(p->N, p->D - 1);
@@ -618,7 +634,7 @@ int twoldg_problem_create(const twoldg_desc* d, twoldg_problem** out) {
   }
   (void)hipEventCreate(&p->ev0);
   (void)hipEventCreate(&p->ev1);
-  rc = p->precision == TWOLDG_F64 ? create_buffers<double>(p, d, csr) : create_buffers<float>(p, d, csr);
+  rc = p->precision != TWOLDG_F32C ? create_buffers<double>(p, d, csr) : create_buffers<float>(p, d, csr);
   if (!rc && p->fused) {
     // One throwaway fused RK4 step over the stage buffers (the state slot is not
     // touched; the compensation is cleared after it).  The first fused step run
@@ -668,7 +684,7 @@ int twoldg_state_upload(twoldg_problem* p, int slot, const double* host) {
   // staged in pinned host memory and converted by a kernel that reads it there
   std::memcpy(p->pinned, host, n * sizeof(double));
   void* lo = (slot == TWOLDG_SLOT_STATE) ? p->lo : nullptr;
-  if (p->precision == TWOLDG_F64)
+  if (p->precision != TWOLDG_F32C)
     hipLaunchKernelGGL((k_from_double<double>), dim3(grid_for(n, 256)), dim3(256), 0, p->stream, n,
                        (const double*)p->pinned, (double*)p->slot[slot], (double*)nullptr);
   else
@@ -688,7 +704,7 @@ int twoldg_state_download(twoldg_problem* p, int slot, double* host) {
   const int64_t n = p->state_len;
   const void* lo = (slot == TWOLDG_SLOT_STATE) ? p->lo : nullptr;
   // converted by a kernel that writes the host staging directly
-  if (p->precision == TWOLDG_F64)
+  if (p->precision != TWOLDG_F32C)
     hipLaunchKernelGGL((k_to_double<double>), dim3(grid_for(n, 256)), dim3(256), 0, p->stream, n,
                        (const double*)p->slot[slot], (const double*)nullptr, p->pinned);
   else

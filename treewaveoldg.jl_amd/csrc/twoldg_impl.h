@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -1028,6 +1029,12 @@ struct twoldg_problem {
   void* fgu = nullptr;
   void* fgg = nullptr;
   void* ffg = nullptr;         // the coarse-fine rows' own face gradients
+  // split form: v of the step start as an FP32 (hi, lo) pair, v of the stage
+  // state and of the accumulator in FP32, (nb, N^3) each
+  float* svn = nullptr;
+  float* svlo = nullptr;
+  float* svs = nullptr;
+  float* svacc = nullptr;
   // the fused RK4 step captured as a graph (replayed for every step of a call)
   hipGraph_t step_graph = nullptr;
   hipGraphExec_t step_exec = nullptr;
@@ -1428,11 +1435,33 @@ struct Launch {
   }
 };
 
-template <class T, int P1, int E, bool GHOST>
-int fused_steps(twoldg_problem* p, double dt, int64_t nsteps) {
+// The split form's v between the FP64 state (v at (2b + 1) N^3 + l) and the FP32
+// pair (hi, lo) with v = hi - lo: hi = fl(v), lo = fl(hi - v).
+__global__ void k_split_v(int64_t nb, int64_t nn, const double* __restrict__ y,
+                          float* __restrict__ hi, float* __restrict__ lo) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb * nn) return;
+  const double v = y[(i / nn * 2 + 1) * nn + i % nn];
+  const float h = (float)v;
+  hi[i] = h;
+  lo[i] = (float)((double)h - v);
+}
+__global__ void k_join_v(int64_t nb, int64_t nn, const float* __restrict__ hi,
+                         const float* __restrict__ lo, double* __restrict__ y) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb * nn) return;
+  y[(i / nn * 2 + 1) * nn + i % nn] = (double)hi[i] - (double)lo[i];
+}
+
+// The fused RK4 steps of a forest, v in TV: T for the FP64 and compensated
+// forms, float under FP64 u for the split form (whose v lives in the FP32 pair
+// (svn, svlo) between the split and join kernels around the steps).
+template <class T, class TV, int P1, int E, bool GHOST>
+int fused_run(twoldg_problem* p, double dt, int64_t nsteps) {
+  constexpr bool split = !std::is_same<T, TV>::value;
   constexpr int N = P1 * E;
   constexpr int threads = (N * N + 63) / 64 * 64;
-  const int64_t nb = p->sh.nb;
+  const int64_t nb = p->sh.nb, nn = p->sh.nn;
   // the warm-up at creation steps the stage buffer in place of the state
   T* yn = (T*)(p->warming ? p->ys : p->slot[TWOLDG_SLOT_STATE]);
   {
@@ -1440,16 +1469,36 @@ int fused_steps(twoldg_problem* p, double dt, int64_t nsteps) {
     hipLaunchKernelGGL((k_faces3<T, P1, E>), dim3(grid_for(n, 256)), dim3(256), 0, p->stream, nb,
                        yn, (T*)p->uf[0], (T*)p->pf[0]);
   }
-  FusedArgs<T> A{};
+  if constexpr (split)
+    hipLaunchKernelGGL(k_split_v, dim3(grid_for(nb * nn, 256)), dim3(256), 0, p->stream, nb, nn,
+                       (const double*)yn, p->svn, p->svlo);
+  FusedArgs<T, TV> A{};
   A.yn = yn;
   A.acc = (T*)p->acc;
-  A.lo = (T*)p->lo;
+  A.lo = split ? nullptr : (T*)p->lo;
   A.nbr = p->nbr6;
   A.spacing = p->spacing;
   A.tables = (const Tables<T>*)p->tables;
   A.truncate = p->truncate ? 1 : 0;
   A.gu = (const T*)p->fgu;
   A.gg = (const T*)p->fgg;
+  A.vstride = split ? nn : 2 * nn;
+  // v of a stage: the v halves of the state arrays, or the split form's own
+  auto vptrs = [&](int stage) {
+    if constexpr (split) {
+      A.vn = (const TV*)p->svn;
+      A.vacc = (TV*)p->svacc;
+      A.vlo = (TV*)p->svlo;
+      A.vs = stage == 1 ? (const TV*)p->svn : (const TV*)p->svs;
+      A.vo = stage == 4 ? (TV*)p->svn : (TV*)p->svs;
+    } else {
+      A.vs = (const TV*)(A.ys + nn);
+      A.vn = (const TV*)(A.yn + nn);
+      A.vacc = (TV*)(A.acc + nn);
+      A.vo = (TV*)(A.yo + nn);
+      A.vlo = A.lo ? (TV*)(A.lo + nn) : nullptr;
+    }
+  };
   // the complete ghosts of the coarse-fine faces from the stage input's face
   // buffers (ghosted form)
   auto ghosts = [&](int in) {
@@ -1476,32 +1525,43 @@ int fused_steps(twoldg_problem* p, double dt, int64_t nsteps) {
     A.ys = yn;  // stage 1 (mode 1) relies on ys = yn
     A.yo = (T*)p->ys;
     A.a = (T)(dt / 2);
+    vptrs(1);
     io(0);
     tmark(p);
     ghosts(0);
-    hipLaunchKernelGGL((k_fused3<T, P1, E, 1, GHOST>), dim3((unsigned)nb), dim3(threads), 0, p->stream, A);
+    hipLaunchKernelGGL((k_fused3<T, P1, E, 1, GHOST, TV>), dim3((unsigned)nb), dim3(threads), 0,
+                       p->stream, A);
     tmark(p);
     A.ys = (T*)p->ys;
+    vptrs(2);
     io(1);
     tmark(p);
     ghosts(1);
-    hipLaunchKernelGGL((k_fused3<T, P1, E, 2, GHOST>), dim3((unsigned)nb), dim3(threads), 0, p->stream, A);
+    hipLaunchKernelGGL((k_fused3<T, P1, E, 2, GHOST, TV>), dim3((unsigned)nb), dim3(threads), 0,
+                       p->stream, A);
     tmark(p);
     A.a = (T)dt;
     io(0);
     tmark(p);
     ghosts(0);
-    hipLaunchKernelGGL((k_fused3<T, P1, E, 2, GHOST>), dim3((unsigned)nb), dim3(threads), 0, p->stream, A);
+    hipLaunchKernelGGL((k_fused3<T, P1, E, 2, GHOST, TV>), dim3((unsigned)nb), dim3(threads), 0,
+                       p->stream, A);
     tmark(p);
     A.a = (T)(dt / 6);
     A.yo = yn;
+    vptrs(4);
     io(1);
     tmark(p);
     ghosts(1);
-    if (p->precision == TWOLDG_F32C)
-      hipLaunchKernelGGL((k_fused3<T, P1, E, 4, GHOST>), dim3((unsigned)nb), dim3(threads), 0, p->stream, A);
+    if constexpr (split)
+      hipLaunchKernelGGL((k_fused3<T, P1, E, 5, GHOST, TV>), dim3((unsigned)nb), dim3(threads), 0,
+                         p->stream, A);
+    else if (p->precision == TWOLDG_F32C)
+      hipLaunchKernelGGL((k_fused3<T, P1, E, 4, GHOST, TV>), dim3((unsigned)nb), dim3(threads), 0,
+                         p->stream, A);
     else
-      hipLaunchKernelGGL((k_fused3<T, P1, E, 3, GHOST>), dim3((unsigned)nb), dim3(threads), 0, p->stream, A);
+      hipLaunchKernelGGL((k_fused3<T, P1, E, 3, GHOST, TV>), dim3((unsigned)nb), dim3(threads), 0,
+                         p->stream, A);
     tmark(p);
   };
   if (p->timing || p->warming || nsteps < 2) {
@@ -1522,8 +1582,18 @@ int fused_steps(twoldg_problem* p, double dt, int64_t nsteps) {
     }
     for (int64_t s = 0; s < nsteps; ++s) HIP_TRY(hipGraphLaunch(p->step_exec, p->stream));
   }
+  if constexpr (split)
+    hipLaunchKernelGGL(k_join_v, dim3(grid_for(nb * nn, 256)), dim3(256), 0, p->stream, nb, nn,
+                       (const float*)p->svn, (const float*)p->svlo, (double*)yn);
   HIP_TRY(hipGetLastError());
   return 0;
+}
+
+template <class T, int P1, int E, bool GHOST>
+int fused_steps(twoldg_problem* p, double dt, int64_t nsteps) {
+  if constexpr (std::is_same<T, double>::value)
+    if (p->precision == TWOLDG_SPLIT) return fused_run<double, float, P1, E, GHOST>(p, dt, nsteps);
+  return fused_run<T, T, P1, E, GHOST>(p, dt, nsteps);
 }
 
 // (P+1, E) pairs the fused stage is compiled for.

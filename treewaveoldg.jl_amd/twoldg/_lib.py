@@ -11,7 +11,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(_HERE, "libtwoldg.so")
 
 ABI_VERSION = 4
-F64, F32C = 0, 1
+F64, F32C, SPLIT = 0, 1, 2
 FACE_SAME, FACE_COARSER, FACE_FINER, FACE_MIRROR, FACE_GLUE, FACE_DIRICHLET = 1, 2, 3, 4, 5, 6
 FACE_RADIATIVE = 7
 SLOT_STATE, SLOT_DU, SLOT_AUX, SLOT_ARG = 0, 1, 2, 3

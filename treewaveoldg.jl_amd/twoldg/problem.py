@@ -37,16 +37,17 @@ def _ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p)
 
 
-PRECISIONS = {"f64": _lib.F64, "f32c": _lib.F32C}
+PRECISIONS = {"f64": _lib.F64, "f32c": _lib.F32C, "split": _lib.SPLIT}
 
 
 class WaveProblem:
     """Everything the right-hand side and the truncation need, built once and
     held on the device.
 
-    `precision`: "f64" (the reference's structure, FP64 throughout) or "f32c"
+    `precision`: "f64" (the reference's structure, FP64 throughout), "f32c"
     (FP32 stencil and storage, compensated FP32 hi+lo RK update, FP64
-    reductions).  `geometry`: None (flat, logical coordinates) or a
+    reductions) or "split" (u and its stencil in FP64, v in FP32 with a
+    compensated step update; the fused 3D stages only).  `geometry`: None (flat, logical coordinates) or a
     SevenPatchBall (full metric).  `boundary`: Dirichlet data on a full
     geometry's outer sphere, a PlaneWave.
     """
