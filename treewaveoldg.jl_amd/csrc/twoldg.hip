@@ -170,7 +170,7 @@ void destroy(twoldg_problem* p) {
                   (void*)p->part, (void*)p->red, (void*)p->clock, p->dcoef, p->lmode, p->ldual, (void*)p->fslot, (void*)p->fradial,
                   p->frmodes, p->frduals, (void*)p->faslot, p->famodes, p->faweights, p->faginv,
                   (void*)p->rad_rows, (void*)p->rpart, (void*)p->svn, (void*)p->svlo,
-                  (void*)p->svs, (void*)p->svacc, (void*)p->nbr6, p->uf[0], p->uf[1], p->pf[0],
+                  (void*)p->svs, (void*)p->svacc, p->flux, (void*)p->nbr6, p->uf[0], p->uf[1], p->pf[0],
                   p->pf[1], (void*)p->frows, (void*)p->fsrc, p->fwu,
                   p->fwg, p->fgu, p->fgg, p->ffg})
     dfree(b);
@@ -308,6 +308,7 @@ int create_buffers(twoldg_problem* p, const twoldg_desc* d, const Csr& csr) {
   HIP_TRY(hipMemsetAsync(p->ug, 0, p->nrows * sizeof(T), p->stream));
   HIP_TRY(hipMemsetAsync(p->gg, 0, p->nrows * (p->full ? D : 1) * sizeof(T), p->stream));
   if ((rc = dalloc_bytes(p, &p->grads, p->sh.nb * D * p->sh.nn * sizeof(T)))) return rc;
+  if (p->full && (rc = dalloc_bytes(p, &p->flux, p->sh.nb * D * p->sh.nn * sizeof(T)))) return rc;
   if ((rc = upload_tables<T>(p))) return rc;
   if ((rc = dalloc(p, &p->spacing, p->sh.nb))) return rc;
   if ((rc = h2d(p, p->spacing, d->spacing, p->sh.nb * sizeof(double)))) return rc;

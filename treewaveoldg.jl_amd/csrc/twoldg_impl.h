@@ -270,7 +270,8 @@ template <class T, int D, int P1>
 __global__ void k_gradient_cart(Shape sh, const Tables<T>* __restrict__ tbp,
                                 const double* __restrict__ spacing, const T* __restrict__ y,
                                 const T* __restrict__ ug, const T* __restrict__ coef,
-                                const uint8_t* __restrict__ flat, T* __restrict__ cart) {
+                                const uint8_t* __restrict__ flat, T* __restrict__ cart,
+                                T* __restrict__ flux) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= sh.nb * sh.nn) return;
   const int64_t b = g / sh.nn, l = g % sh.nn;
@@ -295,17 +296,33 @@ __global__ void k_gradient_cart(Shape sh, const Tables<T>* __restrict__ tbp,
   }
   const int nv = 1 + D * D;
   const T* c = coef + b * nv * sh.nn;
+  // the Cartesian gradient V, and each dimension's flux F_d = S[d] . V at the
+  // node, which the stage's wide derivative along d reads (evolution.jl:136-220)
+  T V[kMaxD];
   if (flat[b]) {
     const T J = c[0];
 #pragma unroll
-    for (int i = 0; i < D; ++i) cart[(b * D + i) * sh.nn + l] = c[(1 + i * D + i) * sh.nn] * Du[i] / J;
+    for (int i = 0; i < D; ++i) {
+      V[i] = c[(1 + i * D + i) * sh.nn] * Du[i] / J;
+      cart[(b * D + i) * sh.nn + l] = V[i];
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d) flux[(b * D + d) * sh.nn + l] = c[(1 + d * D + d) * sh.nn] * V[d];
   } else {
 #pragma unroll
     for (int i = 0; i < D; ++i) {
       T s = c[(1 + 0 * D + i) * sh.nn + l] * Du[0];
 #pragma unroll
       for (int e = 1; e < D; ++e) s += c[(1 + e * D + i) * sh.nn + l] * Du[e];
-      cart[(b * D + i) * sh.nn + l] = s / c[l];
+      V[i] = s / c[l];
+      cart[(b * D + i) * sh.nn + l] = V[i];
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      T s = c[(1 + d * D + 0) * sh.nn + l] * V[0];
+#pragma unroll
+      for (int j = 1; j < D; ++j) s += c[(1 + d * D + j) * sh.nn + l] * V[j];
+      flux[(b * D + d) * sh.nn + l] = s;
     }
   }
 }
@@ -398,7 +415,7 @@ __global__ void k_stage_flat(Shape sh, const Tables<T>* __restrict__ tbp,
 // a ghost being the adjacent face node's (evolution.jl:169-220).
 template <class T, int D, int P1, int MODE>
 __global__ void k_stage_cart(Shape sh, const Tables<T>* __restrict__ tbp,
-                             const double* __restrict__ spacing, const T* __restrict__ cart,
+                             const double* __restrict__ spacing, const T* __restrict__ flux,
                              const T* __restrict__ cg, const T* __restrict__ coef,
                              const uint8_t* __restrict__ flat, StageIO<T> io) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -419,12 +436,12 @@ __global__ void k_stage_cart(Shape sh, const Tables<T>* __restrict__ tbp,
     const int64_t tl = tlin<D>(I, d, N);
     const T Sdd = c[(1 + d * D + d) * sh.nn];
     auto read = [&](int o) -> T {
-      const int oc = o < 0 ? 0 : (o >= N ? N - 1 : o);
-      const int64_t lf = base + (int64_t)oc * stride;  // frame node (clamped)
+      // inside the block the flux the gradient pass formed at the node; across
+      // a face the ghost's flux with the face node's frame
+      if (o >= 0 && o < N) return flux[(b * D + d) * sh.nn + base + o * stride];
+      const int64_t lf = base + (int64_t)(o < 0 ? 0 : N - 1) * stride;  // the face node
       auto V = [&](int j) -> T {
-        if (o < 0) return cg[((((b * D + d) * 2 + 0) * D) + j) * sh.Nt + tl];
-        if (o >= N) return cg[((((b * D + d) * 2 + 1) * D) + j) * sh.Nt + tl];
-        return cart[(b * D + j) * sh.nn + base + o * stride];
+        return cg[((((b * D + d) * 2 + (o < 0 ? 0 : 1)) * D) + j) * sh.Nt + tl];
       };
       if (isflat) return Sdd * V(d);
       T s = c[(1 + d * D + 0) * sh.nn + lf] * V(0);
@@ -963,6 +980,7 @@ struct twoldg_problem {
   void* ug = nullptr;       // u ghosts (rows)
   void* gg = nullptr;       // gradient ghosts (rows, or rows x D on a full metric)
   void* grads = nullptr;    // (nb, D, nn)
+  void* flux = nullptr;     // full metric: (nb, D, nn) F_d = S[d] . V at every node
   void* tables = nullptr;   // Tables<T>
   void* coef = nullptr;     // (nb, 1+D*D, nn) in T
   void* dcoef = nullptr;    // diagonal metric (nb, 1+D, nn): sqrt(g), A_d
@@ -1348,7 +1366,8 @@ struct Launch {
     } else {
       hipLaunchKernelGGL((k_gradient_cart<T, D, P1>), dim3(grid_for(n, 256)), dim3(256), 0,
                          p->stream, p->sh, (const Tables<T>*)p->tables, p->spacing, y,
-                         (const T*)p->ug, (const T*)p->coef, p->flat, (T*)p->grads);
+                         (const T*)p->ug, (const T*)p->coef, p->flat, (T*)p->grads,
+                         (T*)p->flux);
       ghosts<2>(p, (const T*)p->cval_u, (const T*)p->grads, cdt, (T*)p->gg);
       if (p->nrad)
         hipLaunchKernelGGL((k_radiative_grad<T, D>), dim3(grid_for(p->nrad, 256)), dim3(256), 0,
@@ -1377,7 +1396,7 @@ struct Launch {
     else
       hipLaunchKernelGGL((k_stage_cart<T, D, P1, MODE>), dim3(grid_for(n, 256)), dim3(256), 0,
                          p->stream, p->sh, (const Tables<T>*)p->tables, p->spacing,
-                         (const T*)p->grads, (const T*)p->gg, (const T*)p->coef, p->flat, io);
+                         (const T*)p->flux, (const T*)p->gg, (const T*)p->coef, p->flat, io);
   }
   static void truncate(twoldg_problem* p, T* y) {
     if (!p->truncate) return;
