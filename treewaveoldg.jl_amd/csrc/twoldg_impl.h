@@ -549,22 +549,40 @@ struct FactoredTabs {
   const T* aginv;
 };
 
+// EPW elements of one block per workgroup, each on its own (P+1)^3 threads
+// rounded up to whole waves, so that the many small elements do not each cost a
+// workgroup dispatch; all elements of a workgroup take their block's branch.
+template <int P1>
+constexpr int trunc_elem_threads() { return (P1 * P1 * P1 + 63) / 64 * 64; }
+template <int P1>
+constexpr int trunc_elem_epw() { return trunc_elem_threads<P1>() <= 64 ? 4 : 2; }
+
 template <class T, int P1>
-__global__ void __launch_bounds__(((P1 * P1 * P1) + 63) / 64 * 64)
+__global__ void __launch_bounds__(trunc_elem_threads<P1>() * trunc_elem_epw<P1>())
 k_truncate_elem(Shape sh, int E, const Tables<T>* __restrict__ tbp,
                 const int32_t* __restrict__ tslot, const T* __restrict__ tmodes,
                 const T* __restrict__ tweights, const T* __restrict__ tginv,
                 FactoredTabs<T> F, T* __restrict__ y) {
   constexpr int P = P1 - 1;
   constexpr int n = P1 * P1 * P1, m = P * P * P;
-  __shared__ T sWA[n];
-  __shared__ T sc[P1 * P * P];  // m <= (P+1) P^2: the element's, or the planes' coefficients
-  __shared__ T sz[P1 * P * P];
-  __shared__ T sG[m * m];  // G^-1, column-major (symmetric)
-  const int64_t ne = (int64_t)E * E * E;
-  const int64_t b = blockIdx.x / ne, e = blockIdx.x % ne;
+  constexpr int NTE = trunc_elem_threads<P1>(), EPW = trunc_elem_epw<P1>();
+  __shared__ T sWA_[EPW][n];
+  __shared__ T sc_[EPW][P1 * P * P];  // m <= (P+1) P^2: the element's, or the planes' coefficients
+  __shared__ T sz_[EPW][P1 * P * P];
+  __shared__ T sG_[EPW][m * m];  // G^-1, column-major (symmetric)
+  // the Legendre basis Psi (the element's, or a plane's) in LDS where it fits,
+  // shared by the workgroup's elements
+  constexpr bool psi_lds = (size_t)n * m * sizeof(T) <= 16384;
+  __shared__ T sPsi[psi_lds ? n * m : 1];
+  const int q0 = threadIdx.x / NTE;  // this thread's element of the workgroup
+  T* sWA = sWA_[q0];
+  T* sc = sc_[q0];
+  T* sz = sz_[q0];
+  T* sG = sG_[q0];
+  const int64_t ne = (int64_t)E * E * E, per = ne / EPW;
+  const int64_t b = blockIdx.x / per, e = (blockIdx.x % per) * EPW + q0;
   const int s = tslot ? tslot[b] : -1;
-  const int k = threadIdx.x;
+  const int k = threadIdx.x % NTE;
   const int N = sh.N;
   const int J0 = (int)(e % E), J1 = (int)((e / E) % E), J2 = (int)(e / ((int64_t)E * E));
   T* a = y + b * 2 * sh.nn + sh.nn;
@@ -587,7 +605,10 @@ k_truncate_elem(Shape sh, int E, const Tables<T>* __restrict__ tbp,
     const T* rd = F.rduals + ((int64_t)fs * E + J[kr]) * P1;
     const T* w = F.aweights + ((int64_t)A * E * E + ea) * n2;
     const T* G = F.aginv + ((int64_t)A * E * E + ea) * m2 * m2;
-    for (int i = k; i < m2 * m2; i += blockDim.x) sG[i] = G[i];
+    for (int i = k; i < m2 * m2; i += NTE) sG[i] = G[i];
+    if (psi_lds)
+      for (int i = threadIdx.x; i < n2 * m2; i += blockDim.x) sPsi[i] = F.amodes[i];
+    const T* Psi = psi_lds ? sPsi : F.amodes;
     int kk[3] = {0, 0, 0};
     if (k < n) {
       kk[0] = k % P1;
@@ -613,7 +634,7 @@ k_truncate_elem(Shape sh, int E, const Tables<T>* __restrict__ tbp,
       T cq = 0;
       for (int r = 0; r < n2; ++r) {
         const int node = i * st[kr] + (r % P1) * st[t0] + (r / P1) * st[t1];
-        cq += F.amodes[r * m2 + q] * sWA[node];
+        cq += Psi[r * m2 + q] * sWA[node];
       }
       sc[k] = cq;
     }
@@ -627,7 +648,7 @@ k_truncate_elem(Shape sh, int E, const Tables<T>* __restrict__ tbp,
     lds_barrier();
     if (k < n) {
       T z = 0;
-      for (int q = 0; q < m2; ++q) z += F.amodes[pn * m2 + q] * sz[kk[kr] * m2 + q];
+      for (int q = 0; q < m2; ++q) z += Psi[pn * m2 + q] * sz[kk[kr] * m2 + q];
       a[at(k)] = z;
     }
     return;
@@ -656,12 +677,15 @@ k_truncate_elem(Shape sh, int E, const Tables<T>* __restrict__ tbp,
   }
   const T* w = tweights + ((int64_t)s * ne + e) * n;
   const T* G = tginv + ((int64_t)s * ne + e) * m * m;
-  for (int i = k; i < m * m; i += blockDim.x) sG[i] = G[i];
+  for (int i = k; i < m * m; i += NTE) sG[i] = G[i];
+  if (psi_lds)
+    for (int i = threadIdx.x; i < n * m; i += blockDim.x) sPsi[i] = tmodes[i];
+  const T* Psi = psi_lds ? sPsi : tmodes;
   if (k < n) sWA[k] = w[k] * a[at(k)];
   __syncthreads();
   if (k < m) {
     T cq = 0;
-    for (int i = 0; i < n; ++i) cq += tmodes[i * m + k] * sWA[i];
+    for (int i = 0; i < n; ++i) cq += Psi[i * m + k] * sWA[i];
     sc[k] = cq;
   }
   lds_barrier();
@@ -673,7 +697,7 @@ k_truncate_elem(Shape sh, int E, const Tables<T>* __restrict__ tbp,
   lds_barrier();
   if (k < n) {
     T z = 0;
-    for (int q = 0; q < m; ++q) z += tmodes[k * m + q] * sz[q];
+    for (int q = 0; q < m; ++q) z += Psi[k * m + q] * sz[q];
     a[at(k)] = z;
   }
 }
@@ -1404,8 +1428,9 @@ struct Launch {
     const int64_t n = p->sh.nb * ne;
     if constexpr (D == 3 && P1 <= 5) {
       if (p->has_tables) {  // every block: per-element tables or the tensor projector
-        hipLaunchKernelGGL((k_truncate_elem<T, P1>), dim3((unsigned)n),
-                           dim3((P1 * P1 * P1 + 63) / 64 * 64), 0, p->stream, p->sh, p->E,
+        hipLaunchKernelGGL((k_truncate_elem<T, P1>), dim3((unsigned)(n / trunc_elem_epw<P1>())),
+                           dim3(trunc_elem_threads<P1>() * trunc_elem_epw<P1>()), 0, p->stream,
+                           p->sh, p->E,
                            (const Tables<T>*)p->tables, p->tslot, (const T*)p->tmodes,
                            (const T*)p->tweights, (const T*)p->tginv, factored(p), y);
         return;
