@@ -167,8 +167,8 @@ void destroy(twoldg_problem* p) {
                   p->grads, p->tables, p->coef, (void*)p->flat, (void*)p->tslot, p->tmodes,
                   p->tweights, p->tginv, (void*)p->spacing, (void*)p->rowptr, (void*)p->cb,
                   (void*)p->cl, p->cval_u, p->cval_g, (void*)p->dir_rows, (void*)p->dir_x,
-                  (void*)p->part, (void*)p->red, (void*)p->clock, p->dcoef, p->lmode, p->ldual, (void*)p->fslot, (void*)p->fradial,
-                  p->frmodes, p->frduals, (void*)p->faslot, p->famodes, p->faweights, p->faginv,
+                  (void*)p->part, (void*)p->red, (void*)p->clock, p->dcoef, p->lmode, p->ldual,
+                  (void*)p->fslot, (void*)p->fradial, (void*)p->fblk, (void*)p->oblk, p->frmodes, p->frduals, (void*)p->faslot, p->famodes, p->faweights, p->faginv,
                   (void*)p->rad_rows, (void*)p->rpart, (void*)p->svn, (void*)p->svlo,
                   (void*)p->svs, (void*)p->svacc, p->flux, (void*)p->nbr6, p->uf[0], p->uf[1], p->pf[0],
                   p->pf[1], (void*)p->frows, (void*)p->fsrc, p->fwu,
@@ -370,6 +370,28 @@ int create_buffers(twoldg_problem* p, const twoldg_desc* d, const Csr& csr) {
     std::vector<double> ginv;
     if ((rc = gram_inverses(d->fachol, nA * E * E, m2, ginv))) return rc;
     if ((rc = upload_as<T>(p, &p->faginv, ginv.data(), nA * E * E * m2 * m2))) return rc;
+
+    // the truncation's two block lists: the factored blocks with their tables,
+    // and the others
+    std::vector<FBlock> fb;
+    std::vector<int32_t> ob;
+    for (int64_t b = 0; b < p->sh.nb; ++b) {
+      const int32_t f = d->fslot[b];
+      if (f >= 0)
+        fb.push_back({(int32_t)b, f, d->fradial[f], d->faslot[f]});
+      else
+        ob.push_back((int32_t)b);
+    }
+    p->nfblk = (int64_t)fb.size();
+    p->noblk = (int64_t)ob.size();
+    if (p->nfblk) {
+      if ((rc = dalloc(p, &p->fblk, p->nfblk))) return rc;
+      if ((rc = h2d(p, p->fblk, fb.data(), p->nfblk * sizeof(FBlock)))) return rc;
+    }
+    if (p->noblk) {
+      if ((rc = dalloc(p, &p->oblk, p->noblk))) return rc;
+      if ((rc = h2d(p, p->oblk, ob.data(), p->noblk * 4))) return rc;
+    }
   }
   if (!csr.rad_rows.empty()) {
     p->nrad = (int64_t)csr.rad_rows.size();

@@ -537,8 +537,15 @@ __device__ __forceinline__ void lds_barrier() {
 // -1; per f the radial dimension, the radial pairs (f, e_k, i) and the angular
 // table A; the plane basis Psi ((P+1)^2, P^2); per (A, element across k) the
 // plane's weights and the inverse of its Gram matrix Psi^T W Psi.
+// One factored block of a truncation launch: the block, its table index, its
+// radial dimension and angular table, in one load.
+struct FBlock {
+  int32_t b, f, k, A;
+};
 template <class T>
 struct FactoredTabs {
+  const FBlock* blk;  // the factored blocks (a launch of the factored kind)
+  const int32_t* oblk;  // the other blocks (the other kind), or null: every block
   const int32_t* slot;
   const int32_t* radial;
   const T* rmodes;
@@ -557,7 +564,10 @@ constexpr int trunc_elem_threads() { return (P1 * P1 * P1 + 63) / 64 * 64; }
 template <int P1>
 constexpr int trunc_elem_epw() { return trunc_elem_threads<P1>() <= 64 ? 4 : 2; }
 
-template <class T, int P1>
+// KIND 1 covers the factored blocks (their list F.blk), KIND 0 the others (the
+// list F.oblk, or every block); the two are launched apart, so that the
+// factored kind's workgroups hold only the small angular tables in LDS.
+template <class T, int P1, int KIND>
 __global__ void __launch_bounds__(trunc_elem_threads<P1>() * trunc_elem_epw<P1>())
 k_truncate_elem(Shape sh, int E, const Tables<T>* __restrict__ tbp,
                 const int32_t* __restrict__ tslot, const T* __restrict__ tmodes,
@@ -566,22 +576,28 @@ k_truncate_elem(Shape sh, int E, const Tables<T>* __restrict__ tbp,
   constexpr int P = P1 - 1;
   constexpr int n = P1 * P1 * P1, m = P * P * P;
   constexpr int NTE = trunc_elem_threads<P1>(), EPW = trunc_elem_epw<P1>();
+  constexpr int GS = KIND == 1 ? (P * P) * (P * P) : m * m;
+  constexpr int PS = KIND == 1 ? (P1 * P1) * (P * P) : n * m;
   __shared__ T sWA_[EPW][n];
   __shared__ T sc_[EPW][P1 * P * P];  // m <= (P+1) P^2: the element's, or the planes' coefficients
   __shared__ T sz_[EPW][P1 * P * P];
-  __shared__ T sG_[EPW][m * m];  // G^-1, column-major (symmetric)
+  __shared__ T sG_[EPW][GS];  // G^-1, column-major (symmetric)
   // the Legendre basis Psi (the element's, or a plane's) in LDS where it fits,
   // shared by the workgroup's elements
-  constexpr bool psi_lds = (size_t)n * m * sizeof(T) <= 16384;
-  __shared__ T sPsi[psi_lds ? n * m : 1];
+  constexpr bool psi_lds = (size_t)PS * sizeof(T) <= 16384;
+  __shared__ T sPsi[psi_lds ? PS : 1];
   const int q0 = threadIdx.x / NTE;  // this thread's element of the workgroup
   T* sWA = sWA_[q0];
   T* sc = sc_[q0];
   T* sz = sz_[q0];
   T* sG = sG_[q0];
   const int64_t ne = (int64_t)E * E * E, per = ne / EPW;
-  const int64_t b = blockIdx.x / per, e = (blockIdx.x % per) * EPW + q0;
-  const int s = tslot ? tslot[b] : -1;
+  FBlock fb{0, -1, 0, 0};
+  if (KIND == 1) fb = F.blk[blockIdx.x / per];
+  const int64_t b = KIND == 1 ? (int64_t)fb.b : (F.oblk ? (int64_t)F.oblk[blockIdx.x / per]
+                                                        : (int64_t)(blockIdx.x / per));
+  const int64_t e = (blockIdx.x % per) * EPW + q0;
+  const int s = (KIND == 0 && tslot) ? tslot[b] : -1;
   const int k = threadIdx.x % NTE;
   const int N = sh.N;
   const int J0 = (int)(e % E), J1 = (int)((e / E) % E), J2 = (int)(e / ((int64_t)E * E));
@@ -591,12 +607,11 @@ k_truncate_elem(Shape sh, int E, const Tables<T>* __restrict__ tbp,
     const int k0 = q % P1, k1 = (q / P1) % P1, k2 = q / (P1 * P1);
     return base + k0 + (int64_t)N * k1 + (int64_t)N * N * k2;
   };
-  const int fs = F.slot ? F.slot[b] : -1;
-  if (fs >= 0) {
+  if constexpr (KIND == 1) {
     // evolution.jl:306-380: along k, a <- a - mode (dual . a) on every line;
     // then on each plane across k, a <- Psi G^-1 Psi^T W a
     constexpr int n2 = P1 * P1, m2 = P * P;
-    const int kr = F.radial[fs], A = F.aslot[fs];
+    const int fs = fb.f, kr = fb.k, A = fb.A;
     const int t0 = kr == 0 ? 1 : 0, t1 = kr == 2 ? 1 : 2;
     const int J[3] = {J0, J1, J2};
     const int64_t ea = J[t0] + (int64_t)E * J[t1];
@@ -652,7 +667,7 @@ k_truncate_elem(Shape sh, int E, const Tables<T>* __restrict__ tbp,
       a[at(k)] = z;
     }
     return;
-  }
+  } else {
   if (s < 0) {  // the tensor projector (evolution.jl:238-264), x, y, then z
     const Tables<T>& tb = *tbp;
     if (k < n) sWA[k] = a[at(k)];
@@ -699,6 +714,7 @@ k_truncate_elem(Shape sh, int E, const Tables<T>* __restrict__ tbp,
     T z = 0;
     for (int q = 0; q < m; ++q) z += Psi[k * m + q] * sz[q];
     a[at(k)] = z;
+  }
   }
 }
 
@@ -1019,6 +1035,9 @@ struct twoldg_problem {
   // inverses of their Gram matrices
   int32_t* fslot = nullptr;
   int32_t* fradial = nullptr;
+  FBlock* fblk = nullptr;   // the factored blocks, and the others, for the truncation
+  int32_t* oblk = nullptr;
+  int64_t nfblk = 0, noblk = 0;
   void* frmodes = nullptr;
   void* frduals = nullptr;
   int32_t* faslot = nullptr;
@@ -1400,7 +1419,8 @@ struct Launch {
     }
   }
   static FactoredTabs<T> factored(twoldg_problem* p) {
-    return {p->fslot,           p->fradial,          (const T*)p->frmodes,
+    return {p->fblk,            p->nfblk ? p->oblk : nullptr,
+            p->fslot,           p->fradial,          (const T*)p->frmodes,
             (const T*)p->frduals, p->faslot,         (const T*)p->famodes,
             (const T*)p->faweights, (const T*)p->faginv};
   }
@@ -1428,11 +1448,19 @@ struct Launch {
     const int64_t n = p->sh.nb * ne;
     if constexpr (D == 3 && P1 <= 5) {
       if (p->has_tables) {  // every block: per-element tables or the tensor projector
-        hipLaunchKernelGGL((k_truncate_elem<T, P1>), dim3((unsigned)(n / trunc_elem_epw<P1>())),
-                           dim3(trunc_elem_threads<P1>() * trunc_elem_epw<P1>()), 0, p->stream,
-                           p->sh, p->E,
-                           (const Tables<T>*)p->tables, p->tslot, (const T*)p->tmodes,
-                           (const T*)p->tweights, (const T*)p->tginv, factored(p), y);
+        constexpr int epw = trunc_elem_epw<P1>(), thr = trunc_elem_threads<P1>() * epw;
+        const FactoredTabs<T> F = factored(p);
+        if (p->nfblk)  // the factored blocks, then the others
+          hipLaunchKernelGGL((k_truncate_elem<T, P1, 1>), dim3((unsigned)(p->nfblk * ne / epw)),
+                             dim3(thr), 0, p->stream, p->sh, p->E, (const Tables<T>*)p->tables,
+                             p->tslot, (const T*)p->tmodes, (const T*)p->tweights,
+                             (const T*)p->tginv, F, y);
+        const int64_t no = p->nfblk ? p->noblk : p->sh.nb;
+        if (no)
+          hipLaunchKernelGGL((k_truncate_elem<T, P1, 0>), dim3((unsigned)(no * ne / epw)),
+                             dim3(thr), 0, p->stream, p->sh, p->E, (const Tables<T>*)p->tables,
+                             p->tslot, (const T*)p->tmodes, (const T*)p->tweights,
+                             (const T*)p->tginv, F, y);
         return;
       }
     }
