@@ -429,6 +429,44 @@ def test_batch_matches_oracle_per_problem():
         batch.close()
 
 
+def test_batch_every_block_shape_and_face_kind_matches_oracle():
+    """One batch over the block shapes a batch takes (E = 2 and 4, orders 1 to 7),
+    periodic and reflecting boxes, and a single-root box whose blocks are their
+    own neighbours across both faces: every problem equals the oracle's RK4 run."""
+    from twoldg.drivers import gaussian_pulse
+    from twoldg.batch import initial_states
+    cases = [(1, 4, 2, False), (2, 4, 2, True), (3, 4, 1, False), (7, 2, 2, False),
+             (7, 2, 1, True), (1, 2, 3, True), (4, 4, 1, True), (5, 2, 3, False)]
+    rng = np.random.default_rng(11)
+    probs = []
+    for P, E, roots, refl in cases:
+        forest = T.wave_forest(2, P, E, roots=roots, reflecting=refl)
+        c = rng.uniform(0.3, 0.7, size=2)
+        w = float(rng.uniform(0.08, 0.2))
+        dt = 0.5 * forest.minimum_spacing() / (P + 1)
+        probs.append((forest, gaussian_pulse(c, w), dt, dict(P=P, E=E, roots=roots, refl=refl,
+                                                             c=c, w=w)))
+    u0s = initial_states(probs)
+    n = 4
+    want = []
+    for i, p in enumerate(probs):
+        m = p[3]
+        ref = O.Problem.brick(2, m["P"], m["E"], roots=m["roots"], reflecting=m["refl"])
+        y0 = ref.truncate(ref.fill_exact(3, 0.0, (m["c"][0], m["c"][1], m["w"])))
+        assert np.max(np.abs(y0 - u0s[i])) <= 1e-15
+        want.append(ref.rk4(y0, 0.0, p[2], n))
+    for prec, tol in (("f64", 1e-12), ("f32c", 2e-5)):
+        batch = T.Batch([p[:3] for p in probs], precision=prec)
+        for i, u0 in enumerate(u0s):
+            batch.upload(i, u0)
+        batch.rk4(n)
+        for i, y in enumerate(want):
+            got = batch.download(i)
+            err = np.linalg.norm(got - y) / np.linalg.norm(y)
+            assert err <= tol, (prec, cases[i], err)
+        batch.close()
+
+
 def test_golden_fixtures():
     """The HIP path against the committed fixtures (tests/golden): RHS of a
     seeded state in 2D and on the ball (Dirichlet at t = 0.25), the ball's

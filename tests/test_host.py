@@ -193,19 +193,29 @@ def _batch_desc(forest, precision=_lib.F64, keep=None):
 
 def test_batch_create_refusals():
     """twoldg_batch_create checks every description before it reads any (no GPU
-    call happens on a refusal): a missing one, an unknown precision, no blocks."""
+    call happens on a refusal): a missing one, an unknown precision, no blocks, a
+    neighbour that does not face back."""
     keep = []
     f = T.wave_forest(2, 2, 2, roots=2)
     good = _batch_desc(f, keep=keep)
     bad_prec = _batch_desc(f, precision=7, keep=keep)
     no_blocks = _batch_desc(f, keep=keep)
     no_blocks.nblocks = 0
+    # block 0's x-high neighbour made block 2, whose x-low neighbour is block 3:
+    # block 0's face could not reach block 2's ghost slots
+    one_way = _batch_desc(f, keep=keep)
+    faces = f.faces()
+    assert faces[0 * 4 + 1].nb[0] == 1 and faces[2 * 4 + 0].nb[0] == 3
+    faces[0 * 4 + 1].nb[0] = 2
+    one_way.faces = ctypes.cast(faces, ctypes.c_void_p)
+    keep.append(faces)
     dt = np.array([0.01, 0.01])
     out = ctypes.c_void_p()
     L = _lib.lib()
     for second, msg in [(None, "problem 1: no description"),
                         (bad_prec, "problem 1: unknown precision 7"),
-                        (no_blocks, "problem 1: a problem needs at least one block")]:
+                        (no_blocks, "problem 1: a problem needs at least one block"),
+                        (one_way, "problem 1: block 0 face 1: its neighbour does not face back")]:
         arr = (ctypes.c_void_p * 2)()
         arr[0] = ctypes.cast(ctypes.pointer(good), ctypes.c_void_p)
         arr[1] = ctypes.cast(ctypes.pointer(second), ctypes.c_void_p) if second else None

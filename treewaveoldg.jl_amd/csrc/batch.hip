@@ -1,6 +1,7 @@
 // batch.hip -- many independent 2D problems advanced together: one launch per
-// RK4 stage over a device-resident work list, a hipGraph per step, and a
-// blocks-per-workgroup tile per element order chosen by timing.
+// RK4 stage over every block (a run table in the kernel arguments maps each
+// workgroup to its tile), a hipGraph per step, and a blocks-per-workgroup tile
+// per element order chosen by timing.
 #include "twoldg_impl.h"
 #include "fused2.h"
 
@@ -16,13 +17,14 @@ struct twoldg_batch {
   double tune_ms[10] = {0};                 // first autotune pass: the order's launch alone
   double tune_full_ms[10] = {0};            // second pass: the full launch with the chosen tile
   // device
-  int64_t *off = nullptr, *foff = nullptr, *nfo = nullptr;
+  int64_t *off = nullptr, *foff = nullptr;
+  int32_t* nfo = nullptr;
   void* nrh = nullptr;
   int32_t *P1s = nullptr, *Ns = nullptr;
   double *spacing = nullptr, *dt = nullptr;
   void* tables = nullptr;
-  BatchItem* items = nullptr;
-  int64_t nitems = 0;
+  std::vector<BatchRun> runs;  // the launch's runs (tiles[P1] blocks per workgroup)
+  int64_t nwg = 0;             // its workgroups
   void *yn = nullptr, *ys = nullptr, *acc = nullptr, *lo = nullptr;
   void *uf[2] = {nullptr, nullptr}, *pf[2] = {nullptr, nullptr};
   hipStream_t stream = nullptr;
@@ -79,45 +81,44 @@ int batch_tables(twoldg_batch* B) {
   return 0;
 }
 
-// Blocks per workgroup that fit the threads' nodes and the LDS of the kernel
-// (elt: bytes per value).
-int max_tile(int N, size_t elt) {
-  const int NT = N * N, per = batch_per_block(N);
-  const int nodes = elt == 4 ? batch_threads<float>() * batch_k<float>()
-                             : batch_threads<double>() * batch_k<double>();
-  const int lds = elt == 4 ? batch_lds<float>() : batch_lds<double>();
-  return std::min(nodes / NT, lds / per);
+// Blocks per workgroup at most (elt: bytes per value).
+int max_tile(int N, int E, size_t elt) {
+  return batch_max_tile(N, E, elt == 4 ? batch_threads<float>() : batch_threads<double>());
 }
 
-// The work list with tile[P1] blocks per entry; only orders in `only` (0: all).
-std::vector<BatchItem> make_items(const twoldg_batch* B, const std::vector<int32_t>& Nh,
-                                  const int* tile, int only) {
-  const std::vector<int64_t>& offh = B->off_host;
-  const std::vector<int64_t>& foffh = B->foff_host;
-  std::vector<BatchItem> items;
-  int64_t b = 0;
+// The runs with tile[P1] blocks per workgroup; only orders in `only` (0: all).
+// Returns the number of workgroups.
+int64_t make_runs(const twoldg_batch* B, const std::vector<int32_t>& Nh, const int* tile, int only,
+                  std::vector<BatchRun>& runs) {
+  runs.clear();
+  int64_t b = 0, wg = 0;
   while (b < B->nb) {
     const int P1 = B->P1s_host[b], N = Nh[b];
     int64_t e = b;
     while (e < B->nb && B->P1s_host[e] == P1 && Nh[e] == N) ++e;
     if (only == 0 || only == P1) {
-      const int t = std::max(1, tile[P1]);
-      for (int64_t f = b; f < e; f += t)
-        items.push_back(BatchItem{P1, N, (int32_t)std::min<int64_t>(t, e - f), 0, f, offh[f],
-                                  foffh[f]});
+      // an order's tile, within what this run's block size allows
+      const int t = std::max(1, std::min(tile[P1], max_tile(N, N / P1, B->elt)));
+      runs.push_back(BatchRun{P1, N, t, 0, wg, b, e, B->off_host[b], B->foff_host[b]});
+      wg += (e - b + t - 1) / t;
     }
     b = e;
   }
-  return items;
+  return wg;
 }
 
 template <class T, int MODE>
-void launch_stage(twoldg_batch* B, const BatchItem* items, int64_t nitems, const T* ys, T* yo,
-                  double coef, int in) {
+void launch_stage(twoldg_batch* B, const std::vector<BatchRun>& runs, int64_t nwg, const T* ys,
+                  T* yo, double coef, int in) {
   BatchArgs<T> A{};
-  A.items = items;
-  A.off = B->off;
-  A.foff = B->foff;
+  for (int r = 0; r < kBatchMaxRuns; ++r) {
+    if (r < (int)runs.size()) {
+      A.runs[r] = runs[r];
+    } else {
+      A.runs[r] = runs[0];
+      A.runs[r].wg0 = INT64_MAX;
+    }
+  }
   A.nfo = B->nfo;
   A.nrh = (const T*)B->nrh;
   A.spacing = B->spacing;
@@ -134,7 +135,7 @@ void launch_stage(twoldg_batch* B, const BatchItem* items, int64_t nitems, const
   A.pf_out = (T*)B->pf[1 - in];
   A.coef = coef;
   A.truncate = B->truncate ? 1 : 0;
-  hipLaunchKernelGGL((k_batch2<T, MODE>), dim3((unsigned)nitems), dim3(batch_threads<T>()), 0,
+  hipLaunchKernelGGL((k_batch2<T, MODE>), dim3((unsigned)nwg), dim3(batch_threads<T>()), 0,
                      B->stream, A);
 }
 
@@ -142,13 +143,13 @@ template <class T>
 void enqueue_step(twoldg_batch* B) {
   T* yn = (T*)B->yn;
   T* ys = (T*)B->ys;
-  launch_stage<T, 1>(B, B->items, B->nitems, yn, ys, 0.5, 0);
-  launch_stage<T, 2>(B, B->items, B->nitems, ys, ys, 0.5, 1);
-  launch_stage<T, 2>(B, B->items, B->nitems, ys, ys, 1.0, 0);
+  launch_stage<T, 1>(B, B->runs, B->nwg, yn, ys, 0.5, 0);
+  launch_stage<T, 2>(B, B->runs, B->nwg, ys, ys, 0.5, 1);
+  launch_stage<T, 2>(B, B->runs, B->nwg, ys, ys, 1.0, 0);
   if (B->precision == TWOLDG_F32C)
-    launch_stage<T, 4>(B, B->items, B->nitems, ys, yn, 1.0 / 6.0, 1);
+    launch_stage<T, 4>(B, B->runs, B->nwg, ys, yn, 1.0 / 6.0, 1);
   else
-    launch_stage<T, 3>(B, B->items, B->nitems, ys, yn, 1.0 / 6.0, 1);
+    launch_stage<T, 3>(B, B->runs, B->nwg, ys, yn, 1.0 / 6.0, 1);
 }
 
 // Time the middle stage for each order and candidate tile; keep the fastest.
@@ -161,31 +162,28 @@ int autotune(twoldg_batch* B, const std::vector<int32_t>& Nh) {
   for (int64_t b = 0; b < B->nb; ++b)
     if (std::find(orders.begin(), orders.end(), B->P1s_host[b]) == orders.end())
       orders.push_back(B->P1s_host[b]);
-  BatchItem* scratch = nullptr;
-  int rc = balloc(B, (void**)&scratch, sizeof(BatchItem) * (size_t)B->nb);
-  if (rc) return rc;
   constexpr int kReps = 10;
+  std::vector<BatchRun> runs;
   for (int P1 : orders) {
     int N = 0;
     for (int64_t b = 0; b < B->nb; ++b)
       if (B->P1s_host[b] == P1) N = std::max<int>(N, Nh[b]);
-    const int tmax = max_tile(N, sizeof(T));
+    const int tmax = max_tile(N, N / P1, sizeof(T));
     float best = 1e30f;
     int besttile = 1;
     for (int t = 1; t <= tmax; t = (t == tmax) ? tmax + 1 : std::min(2 * t, tmax)) {
       int tile[10];
       for (int i = 0; i < 10; ++i) tile[i] = t;
-      std::vector<BatchItem> it = make_items(B, Nh, tile, P1);
-      if ((rc = bh2d(B, scratch, it.data(), it.size() * sizeof(BatchItem)))) return rc;
+      const int64_t nwg = make_runs(B, Nh, tile, P1, runs);
       // stage 2 writes the accumulator and the stage buffer only
-      launch_stage<T, 2>(B, scratch, (int64_t)it.size(), (const T*)B->yn, (T*)B->ys, 0.5, 0);
+      launch_stage<T, 2>(B, runs, nwg, (const T*)B->yn, (T*)B->ys, 0.5, 0);
       // the faster of two timings of kReps launches each: one order's launch
       // is short, so a few launches leave the choice to timer noise
       float ms = 1e30f;
       for (int round = 0; round < 2; ++round) {
         HIP_TRY(hipEventRecord(B->ev0, B->stream));
         for (int r = 0; r < kReps; ++r)
-          launch_stage<T, 2>(B, scratch, (int64_t)it.size(), (const T*)B->yn, (T*)B->ys, 0.5, 0);
+          launch_stage<T, 2>(B, runs, nwg, (const T*)B->yn, (T*)B->ys, 0.5, 0);
         HIP_TRY(hipEventRecord(B->ev1, B->stream));
         HIP_TRY(hipEventSynchronize(B->ev1));
         float m = 0;
@@ -204,21 +202,20 @@ int autotune(twoldg_batch* B, const std::vector<int32_t>& Nh) {
     int N = 0;
     for (int64_t b = 0; b < B->nb; ++b)
       if (B->P1s_host[b] == P1) N = std::max<int>(N, Nh[b]);
-    const int tmax = max_tile(N, sizeof(T));
+    const int tmax = max_tile(N, N / P1, sizeof(T));
     float best = 1e30f;
     int besttile = B->tiles[P1];
     for (int t = 1; t <= tmax; t = (t == tmax) ? tmax + 1 : std::min(2 * t, tmax)) {
       int tile[10];
       for (int i = 0; i < 10; ++i) tile[i] = B->tiles[i];
       tile[P1] = t;
-      std::vector<BatchItem> it = make_items(B, Nh, tile, 0);
-      if ((rc = bh2d(B, scratch, it.data(), it.size() * sizeof(BatchItem)))) return rc;
-      launch_stage<T, 2>(B, scratch, (int64_t)it.size(), (const T*)B->yn, (T*)B->ys, 0.5, 0);
+      const int64_t nwg = make_runs(B, Nh, tile, 0, runs);
+      launch_stage<T, 2>(B, runs, nwg, (const T*)B->yn, (T*)B->ys, 0.5, 0);
       float ms = 1e30f;
       for (int round = 0; round < 2; ++round) {
         HIP_TRY(hipEventRecord(B->ev0, B->stream));
         for (int r = 0; r < kReps; ++r)
-          launch_stage<T, 2>(B, scratch, (int64_t)it.size(), (const T*)B->yn, (T*)B->ys, 0.5, 0);
+          launch_stage<T, 2>(B, runs, nwg, (const T*)B->yn, (T*)B->ys, 0.5, 0);
         HIP_TRY(hipEventRecord(B->ev1, B->stream));
         HIP_TRY(hipEventSynchronize(B->ev1));
         float m = 0;
@@ -233,18 +230,28 @@ int autotune(twoldg_batch* B, const std::vector<int32_t>& Nh) {
     B->tiles[P1] = besttile;
     B->tune_full_ms[P1] = best / kReps;
   }
-  pool_free(scratch);
   return 0;
 }
 
 template <class T>
 int batch_build(twoldg_batch* B, const twoldg_desc* const* d, int64_t n, const double* dt) {
-  // order problems by element order (then as given): each order's blocks are
-  // one run of the block list, so one tile applies to the run
+  // order problems by element order and block size (then as given): the blocks
+  // of each (order, size) are one run of the block list, so one tile applies
+  // to the run
   std::vector<int64_t> order(n);
   for (int64_t i = 0; i < n; ++i) order[i] = i;
-  std::stable_sort(order.begin(), order.end(),
-                   [&](int64_t a, int64_t b) { return d[a]->P < d[b]->P; });
+  std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+    return d[a]->P != d[b]->P ? d[a]->P < d[b]->P : d[a]->E < d[b]->E;
+  });
+  {
+    int nruns = 0;
+    for (int64_t oi = 0; oi < n; ++oi)
+      if (oi == 0 || d[order[oi]]->P != d[order[oi - 1]]->P || d[order[oi]]->E != d[order[oi - 1]]->E)
+        ++nruns;
+    if (nruns > kBatchMaxRuns)
+      return fail(TWOLDG_EINVAL, "a batch takes at most " + std::to_string(kBatchMaxRuns) +
+                                     " distinct (P, E) pairs");
+  }
   B->nprob = n;
   B->prob_off.assign(n, 0);
   B->prob_len.assign(n, 0);
@@ -271,9 +278,11 @@ int batch_build(twoldg_batch* B, const twoldg_desc* const* d, int64_t n, const d
     }
     nb += p->nblocks;
   }
-  // per block face: the face-buffer index of the neighbour's facing face and
-  // its reciprocal spacing (-1 and 0 for a mirror)
-  std::vector<int64_t> nfoh;
+  if (fsum > INT32_MAX)
+    return fail(TWOLDG_EINVAL, "a batch's ghost slots exceed 2^31 values");
+  // per block face: the ghost slots of the neighbour's facing face (where the
+  // face goes) and the neighbour's reciprocal spacing (-1 and 0 for a mirror)
+  std::vector<int32_t> nfoh;
   std::vector<T> nrhh;
   for (int64_t oi = 0; oi < n; ++oi) {
     const int64_t i = order[oi];
@@ -284,7 +293,7 @@ int batch_build(twoldg_batch* B, const twoldg_desc* const* d, int64_t n, const d
         const twoldg_face& F = p->faces[b * 4 + f];
         if (F.kind == TWOLDG_FACE_SAME) {
           const int64_t nb_ = first[i] + F.nb[0];
-          nfoh.push_back(foffh[nb_] + (int64_t)(f ^ 1) * N);
+          nfoh.push_back((int32_t)(foffh[nb_] + (int64_t)(f ^ 1) * N));
           nrhh.push_back((T)(1.0 / p->spacing[F.nb[0]]));
         } else {
           nfoh.push_back(-1);
@@ -299,14 +308,14 @@ int batch_build(twoldg_batch* B, const twoldg_desc* const* d, int64_t n, const d
   B->off_host = offh;
   B->foff_host = foffh;
   if ((rc = balloc(B, (void**)&B->off, nb * 8)) || (rc = balloc(B, (void**)&B->foff, nb * 8)) ||
-      (rc = balloc(B, (void**)&B->nfo, nb * 32)) || (rc = balloc(B, &B->nrh, nb * 4 * sizeof(T))) ||
+      (rc = balloc(B, (void**)&B->nfo, nb * 16)) || (rc = balloc(B, &B->nrh, nb * 4 * sizeof(T))) ||
       (rc = balloc(B, (void**)&B->P1s, nb * 4)) ||
       (rc = balloc(B, (void**)&B->Ns, nb * 4)) || (rc = balloc(B, (void**)&B->spacing, nb * 8)) ||
       (rc = balloc(B, (void**)&B->dt, nb * 8)))
     return rc;
   if ((rc = bh2d(B, B->off, offh.data(), nb * 8))) return rc;
   if ((rc = bh2d(B, B->foff, foffh.data(), nb * 8))) return rc;
-  if ((rc = bh2d(B, B->nfo, nfoh.data(), nb * 32))) return rc;
+  if ((rc = bh2d(B, B->nfo, nfoh.data(), nb * 16))) return rc;
   if ((rc = bh2d(B, B->nrh, nrhh.data(), nb * 4 * sizeof(T)))) return rc;
   if ((rc = bh2d(B, B->P1s, B->P1s_host.data(), nb * 4))) return rc;
   if ((rc = bh2d(B, B->Ns, Nh.data(), nb * 4))) return rc;
@@ -324,10 +333,7 @@ int batch_build(twoldg_batch* B, const twoldg_desc* const* d, int64_t n, const d
       HIP_TRY(hipMemsetAsync(p, 0, (p == B->uf[0] || p == B->uf[1] || p == B->pf[0] || p == B->pf[1] ? fsum : total) * sizeof(T), B->stream));
   // tiles per order, then the work list, then the step graph
   if ((rc = autotune<T>(B, Nh))) return rc;
-  std::vector<BatchItem> it = make_items(B, Nh, B->tiles, 0);
-  B->nitems = (int64_t)it.size();
-  if ((rc = balloc(B, (void**)&B->items, it.size() * sizeof(BatchItem)))) return rc;
-  if ((rc = bh2d(B, B->items, it.data(), it.size() * sizeof(BatchItem)))) return rc;
+  B->nwg = make_runs(B, Nh, B->tiles, 0, B->runs);
   HIP_TRY(hipStreamSynchronize(B->stream));
   HIP_TRY(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
   enqueue_step<T>(B);
@@ -342,7 +348,7 @@ void batch_destroy(twoldg_batch* B) {
   if (B->exec) (void)hipGraphExecDestroy(B->exec);
   if (B->graph) (void)hipGraphDestroy(B->graph);
   for (void* p : {(void*)B->off, (void*)B->foff, (void*)B->nfo, B->nrh, (void*)B->P1s, (void*)B->Ns,
-                  (void*)B->spacing, (void*)B->dt, B->tables, (void*)B->items, B->yn, B->ys,
+                  (void*)B->spacing, (void*)B->dt, B->tables, B->yn, B->ys,
                   B->acc, B->lo, B->uf[0], B->uf[1], B->pf[0], B->pf[1]})
     if (p) pool_free(p);
   if (B->ev0) (void)hipEventDestroy(B->ev0);
@@ -354,7 +360,7 @@ void batch_destroy(twoldg_batch* B) {
 template <class T>
 int batch_steps(twoldg_batch* B, int64_t nsteps) {
   hipLaunchKernelGGL((k_batch_faces<T>), dim3((unsigned)B->nb), dim3(64), 0, B->stream, B->nb,
-                     B->P1s, B->Ns, B->off, B->foff, (const Tables<T>*)B->tables,
+                     B->P1s, B->Ns, B->off, B->nfo, (const Tables<T>*)B->tables,
                      (const T*)B->yn, (T*)B->uf[0], (T*)B->pf[0]);
   for (int64_t s = 0; s < nsteps; ++s) HIP_TRY(hipGraphLaunch(B->exec, B->stream));
   HIP_TRY(hipGetLastError());
@@ -382,12 +388,12 @@ int twoldg_batch_create(const twoldg_desc* const* descs, int64_t n, const double
       return fail(TWOLDG_EINVAL, who + "a batch takes flat two-dimensional problems");
     if (d->precision != TWOLDG_F64 && d->precision != TWOLDG_F32C)
       return fail(TWOLDG_EINVAL, who + "unknown precision " + std::to_string(d->precision));
-    if (d->P < 1 || d->P > 7 || d->E < 2 || d->E % 2)
-      return fail(TWOLDG_EINVAL, who + "P must be 1..7 and E even");
+    if (d->P < 1 || d->P > 7 || !batch_e_ok(d->E))
+      return fail(TWOLDG_EINVAL, who + "P must be 1..7 and E 2 or 4");
     if (d->nblocks < 1 || !d->spacing || !d->faces)
       return fail(TWOLDG_EINVAL, who + "a problem needs at least one block, its spacings and faces");
     const int N = d->E * (d->P + 1);
-    if (max_tile(N, d->precision == TWOLDG_F32C ? 4 : 8) < 1)
+    if (max_tile(N, d->E, d->precision == TWOLDG_F32C ? 4 : 8) < 1)
       return fail(TWOLDG_EINVAL, who + "blocks of " + std::to_string(N) +
                                      "^2 nodes exceed a workgroup");
     if (d->precision != descs[0]->precision || d->truncate != descs[0]->truncate)
@@ -401,6 +407,15 @@ int twoldg_batch_create(const twoldg_desc* const* descs, int64_t n, const double
         return fail(TWOLDG_EINVAL, who + "a batch takes uniform problems (same-level or mirror faces)");
       if (k == TWOLDG_FACE_SAME && (d->faces[f].nb[0] < 0 || d->faces[f].nb[0] >= d->nblocks))
         return fail(TWOLDG_EINVAL, who + "face neighbour out of range");
+    }
+    // a block's faces go into its neighbours' ghost slots, so the neighbour
+    // across face 2d + s must face back across 2d + 1 - s
+    for (int64_t f = 0; f < d->nblocks * 4; ++f) {
+      if (d->faces[f].kind != TWOLDG_FACE_SAME) continue;
+      const twoldg_face& back = d->faces[d->faces[f].nb[0] * 4 + ((f % 4) ^ 1)];
+      if (back.kind != TWOLDG_FACE_SAME || back.nb[0] != f / 4)
+        return fail(TWOLDG_EINVAL, who + "block " + std::to_string(f / 4) + " face " +
+                                       std::to_string(f % 4) + ": its neighbour does not face back");
     }
   }
   const int prec = descs[0]->precision;
@@ -488,7 +503,7 @@ int twoldg_batch_info(const twoldg_batch* B, int64_t* nodes, int64_t* workgroups
                       double* tune_ms) {
   if (!B) return fail(TWOLDG_EINVAL, "no batch");
   if (nodes) *nodes = B->total / 2;
-  if (workgroups) *workgroups = B->nitems;
+  if (workgroups) *workgroups = B->nwg;
   for (int P = 1; P <= 8; ++P) {
     if (tiles) tiles[P - 1] = B->tiles[P + 1];
     if (tune_ms) {

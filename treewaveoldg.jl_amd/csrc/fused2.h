@@ -1,38 +1,35 @@
 // fused2.h -- one-launch RK4 stage over a batch of independent 2D problems of
 // different element orders.
 //
-// The blocks of every problem are laid out one after another; a device-resident
-// work list gives each workgroup one entry (P+1, first block, block count), so
-// one launch advances every block of every problem by one stage.  Inside a
-// workgroup each thread owns one or more nodes (all their global operands are
-// loaded before the first barrier); a block lives in LDS with its x and y
-// ghosts: u, then the two gradients, then the Laplacian, the RK4 combination
-// (stepping.jl:72-78) and v's truncation (evolution.jl:238-264, x then y).
-// Ghosts come from face buffers written by the previous stage (the face plane
-// of u and the block-local part of the face node's gradient), as in the 3D
-// fused stage.  Each problem has its own spacing and time step.
-
+// The blocks of every problem are laid out one after another, grouped in runs
+// of equal order and block size; each run is cut into tiles of a fixed number
+// of blocks, one tile per workgroup, and the workgroup finds its tile from the
+// run table in the kernel arguments (no load), so one launch advances every
+// block of every problem by one stage.  Inside a workgroup each thread owns one
+// or more nodes (all their global operands are loaded before the first
+// barrier); a block lives in LDS with its x and y ghosts: u, then the two
+// gradients, then the Laplacian, the RK4 combination (stepping.jl:72-78) and
+// v's truncation (evolution.jl:238-264, x then y).
+//
+// Ghosts are pushed: at the end of a stage a block writes each of its faces
+// (the face plane of the new u and the block-local part of the face node's
+// gradient) into the ghost slots of the neighbour across that face, in the
+// other of two ghost buffers; the next stage reads its own slots, at an offset
+// fixed by its block index.  So every global load of a stage depends only on
+// the workgroup's index: one round of loads, not a chain.  Each problem has its
+// own spacing and time step.
 #pragma once
 
-// Launch shape of the batch stage per precision, from timing the 512-problem
-// batch (orders 2..6): FP32 one node per thread, 512 threads, compiled for 8
-// waves per SIMD; FP64 two nodes per thread, 256 threads, the compiler's
-// register budget.  K nodes per thread put K times the loads in flight
-// between two barriers; more waves per SIMD do the same across workgroups.
+// Launch shape of the batch stage: threads per workgroup and waves per SIMD
+// (0: the compiler's register budget) per precision.
 #ifndef TWOLDG_BATCH_THREADS_F32
-#define TWOLDG_BATCH_THREADS_F32 512
+#define TWOLDG_BATCH_THREADS_F32 256
 #endif
 #ifndef TWOLDG_BATCH_THREADS_F64
 #define TWOLDG_BATCH_THREADS_F64 256
 #endif
-#ifndef TWOLDG_BATCH_K_F32
-#define TWOLDG_BATCH_K_F32 1
-#endif
-#ifndef TWOLDG_BATCH_K_F64
-#define TWOLDG_BATCH_K_F64 2
-#endif
 #ifndef TWOLDG_BATCH_WAVES_F32
-#define TWOLDG_BATCH_WAVES_F32 8
+#define TWOLDG_BATCH_WAVES_F32 0
 #endif
 #ifndef TWOLDG_BATCH_WAVES_F64
 #define TWOLDG_BATCH_WAVES_F64 0
@@ -42,47 +39,54 @@ constexpr int batch_threads() {
   return sizeof(T) == 4 ? TWOLDG_BATCH_THREADS_F32 : TWOLDG_BATCH_THREADS_F64;
 }
 template <class T>
-constexpr int batch_k() {
-  return sizeof(T) == 4 ? TWOLDG_BATCH_K_F32 : TWOLDG_BATCH_K_F64;
-}
-// waves per SIMD (0: the compiler's choice)
-template <class T>
 constexpr int batch_waves() {
   return sizeof(T) == 4 ? TWOLDG_BATCH_WAVES_F32 : TWOLDG_BATCH_WAVES_F64;
 }
-// LDS values of T per workgroup: room for the tiles of up to TH K nodes at
-// N >= 6 (23 values per 4 nodes at most, at N = 6).
+// The element counts per block side a batch takes.
+constexpr bool batch_e_ok(int E) { return E == 2 || E == 4; }
+// Blocks of N^2 nodes (E elements per side) per workgroup at most: a thread
+// per element line along x and one along y, E N of each per block.
+inline __host__ __device__ int batch_max_tile(int N, int E, int threads) {
+  return threads / (E * N);
+}
+// LDS values of T per workgroup: two planes per block (u, then the next u;
+// the x-Laplacian sums, then v for its truncation), at most TH / (E N) blocks
+// of 2 N^2 values: 2 TH P1 values, P1 <= 8.
 template <class T>
 constexpr int batch_lds() {
-  return (batch_threads<T>() * batch_k<T>() * 23 / 4 + 63) / 64 * 64;
-}
-// LDS values per block: U (the u plane with its ghost ring, later the next u),
-// GX and GY (the gradients with their ghosts along their axis; GX later the
-// x-truncated v), VA (the stage's v).
-inline __host__ __device__ int batch_per_block(int N) {
-  const int NP = N + 2;
-  return NP * NP + 2 * N * NP + N * N;
+  return batch_threads<T>() * 16;
 }
 
+// One workgroup's tile: consecutive blocks of one order and block size.
 struct BatchItem {
-  int32_t P1;      // element nodes of this entry's blocks
+  int32_t P1;      // element nodes of the tile's blocks
   int32_t N;       // nodes per block side, E (P+1)
-  int32_t count;   // blocks in this entry (consecutive, stored one after another)
-  int32_t pad;
+  int32_t count;   // blocks in the tile
   int64_t first;   // first block
   int64_t off0;    // state offset of the first block (2 N^2 values per block)
-  int64_t foff0;   // face-buffer offset of the first block (4 N values per block)
+  int64_t foff0;   // ghost-slot offset of the first block (4 N values per block)
 };
+
+// A run of consecutive blocks of one order and block size, cut into tiles of
+// `tile` blocks; its tiles are the workgroups wg0, wg0 + 1, ...  Unused entries
+// of the table have wg0 beyond every workgroup.
+struct BatchRun {
+  int32_t P1, N, tile, pad;
+  int64_t wg0;     // the run's first workgroup
+  int64_t first;   // first block
+  int64_t end;     // one past the last block
+  int64_t off0;    // state offset of the first block
+  int64_t foff0;   // ghost-slot offset of the first block
+};
+constexpr int kBatchMaxRuns = 16;
 
 template <class T>
 struct BatchArgs {
-  const BatchItem* items;
-  const int64_t* off;      // (nb) state offset of block b (2 N^2 values per block)
-  const int64_t* foff;     // (nb) face-buffer offset (4 N values per block)
-  // (nb, 4) across face 2d + s: the face-buffer index of the neighbour's facing
-  // face (its tangential node 0), or -1 for a mirror; and the neighbour's
-  // reciprocal spacing -- so an edge node's ghost is one load away
-  const int64_t* nfo;
+  BatchRun runs[kBatchMaxRuns];
+  // (nb, 4) across face 2d + s: the ghost slots (tangential node 0) of the
+  // neighbour's facing face, where this block's face goes, or -1 for a mirror;
+  // and the neighbour's reciprocal spacing, 0 for a mirror
+  const int32_t* nfo;
   const T* nrh;
   const double* spacing;   // (nb)
   const double* dt;        // (nb) the block's problem's time step
@@ -92,266 +96,269 @@ struct BatchArgs {
   T* acc;
   T* yo;
   T* lo;
-  const T* uf_in;
-  const T* pf_in;
-  T* uf_out;
+  const T* uf_in;          // this stage's ghosts: per block 4 N slots, face 2d + s
+  const T* pf_in;          //   at 2d + s N (u on the face plane, gradient part)
+  T* uf_out;               // the next stage's ghosts (the neighbours' slots)
   T* pf_out;
   double coef;             // stage coefficient in units of dt: 1/2, 1 or 1/6
   int truncate;
 };
 
-// The general body: N = E (P+1) nodes per block side, E even.  Thread t owns
-// the entry's nodes t, t + TH, ... (K of them, TH threads), node s being node
-// s % N^2 of the entry's block s / N^2.
-template <class T, int P1, int MODE>
-__device__ void batch_blocks(const BatchArgs<T>& A, const BatchItem& it, T* lds, T* ltab) {
-  constexpr int K = batch_k<T>();
-  constexpr int TH = batch_threads<T>();
-  constexpr int PW = P1 + 2;
+// The derivative along one element line of a block (x or y): the gradient at
+// the element's P1 nodes and at the two nodes beyond it (the neighbour
+// elements' face nodes, or the block face's ghost gradient), then the
+// Laplacian sums lap[j] (evolution.jl:72-110 along one dimension).  rd(c) reads
+// u at node c - 1 along the line (c = 1..N); ulo/uhi give the ghost u beyond
+// the block's faces, glo/ghi the ghost gradient there from the own face value
+// and gradient.
+template <class T, int P1, int E, class RD, class UL, class UH, class GL, class GH>
+__device__ __forceinline__ void batch_line_lap(int el, RD rd, UL ulo, UH uhi, GL glo, GH ghi,
+                                               T rh, T* lap) {
+  constexpr FusedTab<T, P1> tb = fused_tab<T, P1>();
+  constexpr int PW = P1 + 2, P = P1 - 1;
+  T uw[PW];
+  uw[0] = el == 0 ? ulo() : rd(el * P1);
+#pragma unroll
+  for (int k = 1; k < PW - 1; ++k) uw[k] = rd(el * P1 + k);
+  uw[PW - 1] = el == E - 1 ? uhi() : rd(el * P1 + PW - 1);
+  T g[PW];
+#pragma unroll
+  for (int j = 0; j < P1; ++j) {
+    T s = 0;
+#pragma unroll
+    for (int k = 0; k < PW; ++k) {
+      if ((k == 0 && j != 0) || (k == PW - 1 && j != P)) continue;
+      s += tb.W[j][k] * uw[k];
+    }
+    g[j + 1] = s * rh;
+  }
+  if (el > 0) {  // row P of the element below: its nodes, then our first node
+    T s = 0;
+#pragma unroll
+    for (int k = 1; k < PW - 1; ++k) s += tb.W[P][k] * rd((el - 1) * P1 + k);
+    s += tb.W[P][PW - 1] * uw[1];
+    g[0] = s * rh;
+  } else {
+    g[0] = glo(uw[1], g[1]);
+  }
+  if (el < E - 1) {  // row 0 of the element above: our last node, then its nodes
+    T s = tb.W[0][0] * uw[P1];
+#pragma unroll
+    for (int k = 1; k < PW - 1; ++k) s += tb.W[0][k] * rd((el + 1) * P1 + k);
+    g[PW - 1] = s * rh;
+  } else {
+    g[PW - 1] = ghi(uw[P1], g[P1]);
+  }
+#pragma unroll
+  for (int j = 0; j < P1; ++j) {
+    T s = 0;
+#pragma unroll
+    for (int k = 0; k < PW; ++k) {
+      if ((k == 0 && j != 0) || (k == PW - 1 && j != P)) continue;
+      s += tb.W[j][k] * g[k];
+    }
+    lap[j] = s;
+  }
+}
+
+// One tile: blocks of N = E P1 nodes per side.  Thread t of the workgroup
+// serves block t / (E N) of the tile in two roles: the y line (column xy,
+// element ey), whose P1 nodes it loads, combines and stores, and the x line
+// (element ex, row yx), whose x-derivative and x-truncation it forms in LDS.
+// Every global load is issued in phase 1; phase 3 stores as it goes (nothing
+// is loaded after it, so no wait covers a store).
+template <class T, int P1, int E, int MODE>
+__device__ void batch_lines(const BatchArgs<T>& A, const BatchItem& it, T* lds) {
+  constexpr int N = P1 * E, NT = N * N, LB = E * N;
   constexpr int P = P1 - 1;
   constexpr FusedTab<T, P1> tb = fused_tab<T, P1>();
-  const int N = it.N, NT = N * N, NP = N + 2;
-  const int per = batch_per_block(N);
-  const int nslots = it.count * NT;
+  static_assert(batch_threads<T>() / LB * 2 * NT <= batch_lds<T>(), "tile exceeds the LDS");
   const int tid = threadIdx.x;
-  // the rows of W and the top mode of the entry's order, for the per-node rows
-  {
-    const Tables<T>& gt = A.tables[P1 - 2];
-    if (tid < P1 * PW) ltab[tid] = gt.W[tid / PW][tid % PW];
-    if (tid < P1) ltab[P1 * PW + tid] = gt.mode[tid];
-  }
-  // the entry's blocks are consecutive in the state and the face buffers
-  const int64_t off0 = it.off0, foff0 = it.foff0;
+  const int b0 = tid / LB, r = tid - b0 * LB;
+  const bool act = b0 < it.count;
+  const int xy = r % N, ey = r / N;  // y-line role
+  const int ex = r % E, yx = r / E;  // x-line role
+  T* U = lds + b0 * 2 * NT;  // u, then the next u
+  T* L = U + NT;             // the x-Laplacian sums, then v for its x-truncation
+  // the tile's arrays from uniform bases, per-thread 32-bit offsets
+  const T* ys = A.ys + it.off0;
+  const T* yn = A.yn + it.off0;
+  T* acc = A.acc + it.off0;
+  T* yo = A.yo + it.off0;
+  T* lo = A.lo ? A.lo + it.off0 : nullptr;
+  const T* ufi = A.uf_in + it.foff0;
+  const T* pfi = A.pf_in + it.foff0;
+  const int32_t* nfo = A.nfo + it.first * 4;
+  const T* nrh = A.nrh + it.first * 4;
+  const unsigned ob = (unsigned)b0 * 2 * NT;  // the block's state
+  const unsigned gb = (unsigned)b0 * 4 * N;   // its ghost slots
+  auto node = [&](int j) { return (ey * P1 + j) * N + xy; };  // y line's node j in the plane
   const T Wlo = tb.W[0][0], Whi = tb.W[P][P1 + 1];
 
-  int bl[K], x[K], y[K];
-  bool act[K];
-  T uz[K], ku[K], yu[K], yv[K], au[K], av[K], lu[K], lv[K];
-  T pgx[K], rgx[K], pgy[K], rgy[K], rh[K], ak[K];
-  int64_t nbx[K], nby[K];  // the facing face of the neighbour across the node's x (y)
-                           // face, -1 none or a mirror
-
-  // (1) every global operand of the thread's nodes, then u and its ghosts into
-  // LDS: no wait for a load in a later phase also waits for the stage's
-  // stores (the vector memory counter covers both)
+  // (1) every global operand: the y line's u, v, the step start, the
+  // accumulator (and compensation words), the block's spacing and time step,
+  // and the ghosts of the faces the two lines end on
+  T rh = 0, ak = 0;
+  T uy[P1], ku[P1], yu[P1], yv[P1], au[P1], av[P1], lu[P1], lv[P1];
+  T gyu = 0, gyp = 0, gyr = 0, gxu = 0, gxp = 0, gxr = 0;
+  int32_t nby = -1, nbx = -1;  // where the y (x) line's face goes, -1 a mirror
 #pragma unroll
-  for (int k = 0; k < K; ++k) {
-    const int s = tid + k * TH;
-    act[k] = s < nslots;
-    const int b0 = act[k] ? s / NT : 0;
-    const int node = act[k] ? s - b0 * NT : 0;
-    bl[k] = b0;
-    x[k] = node % N;
-    y[k] = node / N;
-    uz[k] = ku[k] = yu[k] = yv[k] = au[k] = av[k] = lu[k] = lv[k] = T(0);
-    pgx[k] = rgx[k] = pgy[k] = rgy[k] = T(0);
-    rh[k] = ak[k] = T(0);
-    nbx[k] = nby[k] = -1;
-    if (act[k]) {
-      const int64_t b = it.first + b0;
-      const int64_t iu = off0 + (int64_t)b0 * 2 * NT + node, iv = iu + NT;
-      uz[k] = A.ys[iu];
-      ku[k] = A.ys[iv];
+  for (int j = 0; j < P1; ++j) uy[j] = ku[j] = yu[j] = yv[j] = au[j] = av[j] = lu[j] = lv[j] = T(0);
+  if (act) {
+    rh = T(1) / (T)A.spacing[it.first + b0];
+    ak = (T)(A.coef * A.dt[it.first + b0]);
+#pragma unroll
+    for (int j = 0; j < P1; ++j) {
+      const unsigned o = ob + node(j);
+      uy[j] = ys[o];
+      ku[j] = ys[o + NT];
       if (MODE != 1) {  // the compensated form: the high words
-        yu[k] = A.yn[iu];
-        yv[k] = A.yn[iv];
+        yu[j] = yn[o];
+        yv[j] = yn[o + NT];
       }
       if (MODE == 4) {
-        lu[k] = A.lo[iu];
-        lv[k] = A.lo[iv];
+        lu[j] = lo[o];
+        lv[j] = lo[o + NT];
       }
       if (MODE >= 2) {
-        au[k] = A.acc[iu];
-        av[k] = A.acc[iv];
-      }
-      rh[k] = T(1) / (T)A.spacing[b];
-      ak[k] = (T)(A.coef * A.dt[b]);
-      const int xk = x[k], yk = y[k];
-      if (xk == 0 || xk == N - 1) {
-        const int64_t f = b * 4 + (xk == 0 ? 0 : 1);
-        nbx[k] = A.nfo[f];
-        rgx[k] = A.nrh[f];
-      }
-      if (yk == 0 || yk == N - 1) {
-        const int64_t f = b * 4 + (yk == 0 ? 2 : 3);
-        nby[k] = A.nfo[f];
-        rgy[k] = A.nrh[f];
+        au[j] = acc[o];
+        av[j] = acc[o + NT];
       }
     }
-  }
-  T gux[K], guy[K];  // the ghost u across an edge node's x (y) face
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    gux[k] = guy[k] = uz[k];  // a mirror's ghost u is the own face value
-    if (!act[k]) continue;
-    if (nbx[k] >= 0) {
-      pgx[k] = A.pf_in[nbx[k] + y[k]];
-      gux[k] = A.uf_in[nbx[k] + y[k]];
+    // an end face's ghosts sit in the block's own slots (read whatever the
+    // face, so that no load waits for another; a mirror's are not used)
+    if (ey == 0 || ey == E - 1) {
+      const int f = ey == 0 ? 2 : 3;
+      nby = nfo[b0 * 4 + f];
+      gyr = nrh[b0 * 4 + f];
+      gyu = ufi[gb + f * N + xy];
+      gyp = pfi[gb + f * N + xy];
     }
-    if (nby[k] >= 0) {
-      pgy[k] = A.pf_in[nby[k] + x[k]];
-      guy[k] = A.uf_in[nby[k] + x[k]];
+    if (ex == 0 || ex == E - 1) {
+      const int f = ex == 0 ? 0 : 1;
+      nbx = nfo[b0 * 4 + f];
+      gxr = nrh[b0 * 4 + f];
+      gxu = ufi[gb + f * N + yx];
+      gxp = pfi[gb + f * N + yx];
     }
-  }
 #pragma unroll
-  for (int k = 0; k < K; ++k) {
-    if (!act[k]) continue;
-    const int xk = x[k], yk = y[k];
-    T* U = lds + bl[k] * per;
-    U[(yk + 1) * NP + xk + 1] = uz[k];
-    if (xk == 0) U[(yk + 1) * NP] = gux[k];
-    if (xk == N - 1) U[(yk + 1) * NP + N + 1] = gux[k];
-    if (yk == 0) U[xk + 1] = guy[k];
-    if (yk == N - 1) U[(N + 1) * NP + xk + 1] = guy[k];
+    for (int j = 0; j < P1; ++j) U[node(j)] = uy[j];
   }
   lds_barrier();
-  // (2) gradients and their ghosts
+  // (2) x line: the x-Laplacian sums to L; y line: its own, kept
+  T ly[P1];
+  if (act) {
+    {
+      const T* Ur = U + yx * N - 1;
+      T lx[P1];
+      batch_line_lap<T, P1, E>(
+          ex, [&](int c) { return Ur[c]; },
+          [&]() { return gxr != T(0) ? gxu : Ur[1]; },  // a mirror's ghost u is the face value
+          [&]() { return gxr != T(0) ? gxu : Ur[N]; },
+          [&](T uf, T gf) { return gxr != T(0) ? (gxp + Whi * uf) * gxr : -gf; },
+          [&](T uf, T gf) { return gxr != T(0) ? (gxp + Wlo * uf) * gxr : -gf; }, rh, lx);
 #pragma unroll
-  for (int k = 0; k < K; ++k) {
-    if (!act[k]) continue;
-    const int xk = x[k], yk = y[k];
-    const T* U = lds + bl[k] * per;
-    T* GX = lds + bl[k] * per + NP * NP;
-    T* GY = GX + N * NP;
-    const int ix = xk % P1, iy = yk % P1;
-    const int sx = xk - ix - 1, sy = yk - iy - 1;
-    T gx = 0, gy = 0;
-#pragma unroll
-    for (int q = 0; q < PW; ++q) {
-      gx += ltab[ix * PW + q] * U[(yk + 1) * NP + (sx + q + 1)];
-      gy += ltab[iy * PW + q] * U[(sy + q + 1) * NP + xk + 1];
+      for (int j = 0; j < P1; ++j) L[yx * N + ex * P1 + j] = lx[j];
     }
-    gx *= rh[k];
-    gy *= rh[k];
-    GX[yk * NP + xk + 1] = gx;
-    GY[(yk + 1) * N + xk] = gy;
-    // a mirror's ghost gradient is minus the own face gradient
-    if (xk == 0) GX[yk * NP] = nbx[k] >= 0 ? (pgx[k] + Whi * uz[k]) * rgx[k] : -gx;
-    if (xk == N - 1) GX[yk * NP + N + 1] = nbx[k] >= 0 ? (pgx[k] + Wlo * uz[k]) * rgx[k] : -gx;
-    if (yk == 0) GY[xk] = nby[k] >= 0 ? (pgy[k] + Whi * uz[k]) * rgy[k] : -gy;
-    if (yk == N - 1) GY[(N + 1) * N + xk] = nby[k] >= 0 ? (pgy[k] + Wlo * uz[k]) * rgy[k] : -gy;
+    const T* Uc = U - N + xy;
+    batch_line_lap<T, P1, E>(
+        ey, [&](int c) { return Uc[c * N]; },
+        [&]() { return gyr != T(0) ? gyu : Uc[N]; },
+        [&]() { return gyr != T(0) ? gyu : Uc[N * N]; },
+        [&](T uf, T gf) { return gyr != T(0) ? (gyp + Whi * uf) * gyr : -gf; },
+        [&](T uf, T gf) { return gyr != T(0) ? (gyp + Wlo * uf) * gyr : -gf; }, rh, ly);
   }
   lds_barrier();
-  // (3) the Laplacian and the stage combination; the next u into the u plane's
-  // place (U is dead), the stage's v into VA
-  T unext[K], w[K];
+  // (3) y line: the Laplacian, the stage combination (stepping.jl:72-78) and its
+  // stores; the next u into U (this thread's nodes only), v truncated along y
+  // into L, the y faces of the next u into the neighbours' ghost slots
+  if (act) {
+    T w[P1];
+    T plo = 0, phi = 0, ulo = 0, uhi = 0;
 #pragma unroll
-  for (int k = 0; k < K; ++k) {
-    unext[k] = w[k] = T(0);
-    if (!act[k]) continue;
-    const int xk = x[k], yk = y[k];
-    const T* GX = lds + bl[k] * per + NP * NP;
-    const T* GY = GX + N * NP;
-    T* VA = lds + bl[k] * per + NP * NP + 2 * N * NP;
-    const int ix = xk % P1, iy = yk % P1;
-    const int sx = xk - ix - 1, sy = yk - iy - 1;
-    T lx = 0, ly = 0;
-#pragma unroll
-    for (int q = 0; q < PW; ++q) {
-      lx += ltab[ix * PW + q] * GX[yk * NP + (sx + q + 1)];
-      ly += ltab[iy * PW + q] * GY[(sy + q + 1) * N + xk];
+    for (int j = 0; j < P1; ++j) {
+      const int l = node(j);
+      const unsigned o = ob + l;
+      const T kv = ((T(0) + L[l]) + ly[j]) * rh;
+      T unext;
+      if (MODE == 1) {  // stage 1: the stage input is the step start
+        acc[o] = ku[j];
+        acc[o + NT] = kv;
+        unext = uy[j] + ak * ku[j];
+        w[j] = ku[j] + ak * kv;
+      } else if (MODE == 2) {
+        acc[o] = au[j] + T(2) * ku[j];
+        acc[o + NT] = av[j] + T(2) * kv;
+        unext = yu[j] + ak * ku[j];
+        w[j] = yv[j] + ak * kv;
+      } else if (MODE == 3) {
+        unext = yu[j] + ak * (au[j] + ku[j]);
+        w[j] = yv[j] + ak * (av[j] + kv);
+      } else {  // compensated: u by Kahan now, v's increment truncated first
+        const T du = ak * (au[j] + ku[j]);
+        const T yk = du - lu[j], tu = yu[j] + yk;
+        lo[o] = (tu - yu[j]) - yk;
+        unext = tu;
+        w[j] = ak * (av[j] + kv);
+      }
+      yo[o] = unext;
+      U[l] = unext;
+      if (j == 0) ulo = unext;
+      if (j == P) uhi = unext;
+      plo += tb.W[0][j + 1] * unext;
+      phi += tb.W[P][j + 1] * unext;
     }
-    const T kv = ((T(0) + lx) + ly) * rh[k];
-    const int64_t iu = off0 + (int64_t)bl[k] * 2 * NT + yk * N + xk, iv = iu + NT;
-    const T a = ak[k];
-    if (MODE == 1) {  // stage 1: the stage input is the step start
-      A.acc[iu] = ku[k];
-      A.acc[iv] = kv;
-      unext[k] = uz[k] + a * ku[k];
-      w[k] = ku[k] + a * kv;
-    } else if (MODE == 2) {
-      A.acc[iu] = au[k] + T(2) * ku[k];
-      A.acc[iv] = av[k] + T(2) * kv;
-      unext[k] = yu[k] + a * ku[k];
-      w[k] = yv[k] + a * kv;
-    } else if (MODE == 3) {
-      unext[k] = yu[k] + a * (au[k] + ku[k]);
-      w[k] = yv[k] + a * (av[k] + kv);
-    } else {  // compensated: u by Kahan now, v's increment truncated first
-      const T du = a * (au[k] + ku[k]);
-      const T yk2 = du - lu[k], tu = yu[k] + yk2;
-      A.lo[iu] = (tu - yu[k]) - yk2;
-      unext[k] = tu;
-      w[k] = a * (av[k] + kv);
+    if (nby >= 0) {
+      A.uf_out[nby + xy] = ey == 0 ? ulo : uhi;
+      A.pf_out[nby + xy] = ey == 0 ? plo : phi;
     }
-    A.yo[iu] = unext[k];
-    VA[yk * N + xk] = w[k];
-  }
-  // the next u goes where U was: every thread is past its reads of U (barrier
-  // 2), and nobody reads UN before barrier 3
-#pragma unroll
-  for (int k = 0; k < K; ++k)
-    if (act[k]) lds[bl[k] * per + y[k] * N + x[k]] = unext[k];
-  lds_barrier();
-  // (4) faces of the next u; truncation along x into VB (where GX was)
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    if (!act[k]) continue;
-    const int xk = x[k], yk = y[k];
-    const T* UN = lds + bl[k] * per;
-    const T* VA = lds + bl[k] * per + NP * NP + 2 * N * NP;
-    T* VB = lds + bl[k] * per + NP * NP;
-    const int64_t fb = foff0 + (int64_t)bl[k] * 4 * N;
-    if (xk == 0) {
-      T s = 0;
-#pragma unroll
-      for (int j = 0; j < P1; ++j) s += tb.W[0][j + 1] * UN[yk * N + j];
-      A.uf_out[fb + 0 * N + yk] = unext[k];
-      A.pf_out[fb + 0 * N + yk] = s;
-    }
-    if (xk == N - 1) {
-      T s = 0;
-#pragma unroll
-      for (int j = 0; j < P1; ++j) s += tb.W[P][j + 1] * UN[yk * N + N - P1 + j];
-      A.uf_out[fb + 1 * N + yk] = unext[k];
-      A.pf_out[fb + 1 * N + yk] = s;
-    }
-    if (yk == 0) {
-      T s = 0;
-#pragma unroll
-      for (int j = 0; j < P1; ++j) s += tb.W[0][j + 1] * UN[j * N + xk];
-      A.uf_out[fb + 2 * N + xk] = unext[k];
-      A.pf_out[fb + 2 * N + xk] = s;
-    }
-    if (yk == N - 1) {
-      T s = 0;
-#pragma unroll
-      for (int j = 0; j < P1; ++j) s += tb.W[P][j + 1] * UN[(N - P1 + j) * N + xk];
-      A.uf_out[fb + 3 * N + xk] = unext[k];
-      A.pf_out[fb + 3 * N + xk] = s;
-    }
-    const int ix = xk % P1;
     if (A.truncate) {
       T c = 0;
 #pragma unroll
-      for (int j = 0; j < P1; ++j) c += tb.dual[j] * VA[yk * N + (xk - ix) + j];
-      VB[yk * N + xk] = w[k] - ltab[P1 * PW + ix] * c;
-    } else {
-      VB[yk * N + xk] = w[k];
+      for (int j = 0; j < P1; ++j) c += tb.dual[j] * w[j];
+#pragma unroll
+      for (int j = 0; j < P1; ++j) w[j] -= tb.mode[j] * c;
+    }
+#pragma unroll
+    for (int j = 0; j < P1; ++j) L[node(j)] = w[j];  // L at these nodes was read above, by us
+  }
+  lds_barrier();
+  // (4) x line: v's truncation along x; the x faces of the next u
+  if (act) {
+    const int l = yx * N + ex * P1;
+    if (A.truncate) {
+      T w[P1];
+      T c = 0;
+#pragma unroll
+      for (int j = 0; j < P1; ++j) {
+        w[j] = L[l + j];
+        c += tb.dual[j] * w[j];
+      }
+#pragma unroll
+      for (int j = 0; j < P1; ++j) L[l + j] = w[j] - tb.mode[j] * c;
+    }
+    if (nbx >= 0) {
+      T s = 0;
+#pragma unroll
+      for (int j = 0; j < P1; ++j) s += (ex == 0 ? tb.W[0][j + 1] : tb.W[P][j + 1]) * U[l + j];
+      A.uf_out[nbx + yx] = U[l + (ex == 0 ? 0 : P)];
+      A.pf_out[nbx + yx] = s;
     }
   }
   lds_barrier();
-  // (5) truncation along y; v out
+  // (5) y line: v out
+  if (act) {
 #pragma unroll
-  for (int k = 0; k < K; ++k) {
-    if (!act[k]) continue;
-    const int xk = x[k], yk = y[k];
-    const T* VB = lds + bl[k] * per + NP * NP;
-    const int iy = yk % P1;
-    T vv = VB[yk * N + xk];
-    if (A.truncate) {
-      T c = 0;
-#pragma unroll
-      for (int j = 0; j < P1; ++j) c += tb.dual[j] * VB[((yk - iy) + j) * N + xk];
-      vv -= ltab[P1 * PW + iy] * c;
-    }
-    const int64_t iv = off0 + (int64_t)bl[k] * 2 * NT + NT + yk * N + xk;
-    if (MODE == 4) {
-      const T yk2 = vv - lv[k], tv = yv[k] + yk2;
-      A.lo[iv] = (tv - yv[k]) - yk2;
-      A.yo[iv] = tv;
-    } else {
-      A.yo[iv] = vv;
+    for (int j = 0; j < P1; ++j) {
+      const unsigned o = ob + node(j) + NT;
+      const T vv = L[node(j)];
+      if (MODE == 4) {
+        const T yk = vv - lv[j], tv = yv[j] + yk;
+        lo[o] = (tv - yv[j]) - yk;
+        yo[o] = tv;
+      } else {
+        yo[o] = vv;
+      }
     }
   }
 }
@@ -359,26 +366,48 @@ __device__ void batch_blocks(const BatchArgs<T>& A, const BatchItem& it, T* lds,
 template <class T, int MODE>
 __global__ void __launch_bounds__(batch_threads<T>(), batch_waves<T>()) k_batch2(BatchArgs<T> A) {
   __shared__ T lds[batch_lds<T>()];
-  __shared__ T ltab[8 * 10 + 8];  // W rows and the top mode of the entry's order
-  const BatchItem it = A.items[blockIdx.x];
+  // the workgroup's run: the last that starts at or before it
+  BatchRun R = A.runs[0];
+#pragma unroll
+  for (int r = 1; r < kBatchMaxRuns; ++r)
+    if ((int64_t)blockIdx.x >= A.runs[r].wg0) R = A.runs[r];
+  const int64_t j = (int64_t)blockIdx.x - R.wg0;
+  BatchItem it;
+  it.P1 = R.P1;
+  it.N = R.N;
+  it.first = R.first + j * R.tile;
+  it.count = (int32_t)min((int64_t)R.tile, R.end - it.first);
+  it.off0 = R.off0 + j * R.tile * 2 * (int64_t)R.N * R.N;
+  it.foff0 = R.foff0 + j * R.tile * 4 * (int64_t)R.N;
+  const int E = R.N / R.P1;
+#define TW_BATCH_CASE(P1_)                                                 \
+  case P1_:                                                                \
+    if (E == 2)                                                            \
+      batch_lines<T, P1_, 2, MODE>(A, it, lds);                            \
+    else                                                                   \
+      batch_lines<T, P1_, 4, MODE>(A, it, lds);                            \
+    break;
   switch (it.P1) {
-    case 2: batch_blocks<T, 2, MODE>(A, it, lds, ltab); break;
-    case 3: batch_blocks<T, 3, MODE>(A, it, lds, ltab); break;
-    case 4: batch_blocks<T, 4, MODE>(A, it, lds, ltab); break;
-    case 5: batch_blocks<T, 5, MODE>(A, it, lds, ltab); break;
-    case 6: batch_blocks<T, 6, MODE>(A, it, lds, ltab); break;
-    case 7: batch_blocks<T, 7, MODE>(A, it, lds, ltab); break;
-    case 8: batch_blocks<T, 8, MODE>(A, it, lds, ltab); break;
+    TW_BATCH_CASE(2)
+    TW_BATCH_CASE(3)
+    TW_BATCH_CASE(4)
+    TW_BATCH_CASE(5)
+    TW_BATCH_CASE(6)
+    TW_BATCH_CASE(7)
+    TW_BATCH_CASE(8)
     default: break;
   }
+#undef TW_BATCH_CASE
 }
 
-// The face buffers of a state: per block face, the face plane of u and the
-// block-local part of the face gradient.  One thread per (block, face, node).
+// The ghosts of a state, pushed as the stage does: per block face, the face
+// plane of u and the block-local part of the face gradient, into the ghost
+// slots of the neighbour across the face (nfo; none across a mirror).  One
+// thread per (block, face, node).
 template <class T>
 __global__ void k_batch_faces(int64_t nb, const int32_t* __restrict__ P1s,
                               const int32_t* __restrict__ Ns, const int64_t* __restrict__ off,
-                              const int64_t* __restrict__ foff,
+                              const int32_t* __restrict__ nfo,
                               const Tables<T>* __restrict__ tables, const T* __restrict__ y,
                               T* __restrict__ uf, T* __restrict__ pf) {
   const int64_t b = blockIdx.x;
@@ -387,6 +416,8 @@ __global__ void k_batch_faces(int64_t nb, const int32_t* __restrict__ P1s,
   const Tables<T>& tb = tables[P1 - 2];
   for (int q = threadIdx.x; q < 4 * N; q += blockDim.x) {
     const int f = q / N, t = q % N, d = f / 2, s = f % 2;
+    const int32_t dst = nfo[b * 4 + f];
+    if (dst < 0) continue;
     const int first = s ? N - P1 : 0, row = s ? P : 0;
     T acc = 0;
     for (int j = 0; j < P1; ++j) {
@@ -396,7 +427,7 @@ __global__ void k_batch_faces(int64_t nb, const int32_t* __restrict__ P1s,
     }
     const int i = s ? N - 1 : 0;
     const int64_t l = d == 0 ? (int64_t)t * N + i : (int64_t)i * N + t;
-    uf[foff[b] + q] = y[off[b] + l];
-    pf[foff[b] + q] = acc;
+    uf[dst + t] = y[off[b] + l];
+    pf[dst + t] = acc;
   }
 }
