@@ -641,80 +641,109 @@ k_fused3(FusedArgs<T, TV> A) {
   }
 }
 
-// The coarse-fine rows of a problem: groups of rows with `count` exchange
-// entries each, stored entry-major (entry j of the group's row i at
-// entry0 + j * nrows + i), the groups' rows one after another.
-struct FaceGroups {
-  int ngroups;
-  int count[4];
-  int64_t row0[4], entry0[4], nrows[4];
+// The coarse-fine faces, one pair per job: a fine block's face F and the
+// quarter of its coarser neighbour's face C that F covers (the quarter's coarse
+// elements are the parents of F's elements: E is even).  Face planes share the
+// tangential axes t0 < t1, node t = i0 + N i1.
+struct CFJob {
+  int32_t bf, ff;  // fine block, its face 2d + s
+  int32_t bc, fc;  // coarse block, its face 2d + 1 - s
+  int32_t q;       // C's first quarter node: o0 N/2 + N o1 N/2, (o0, o1) F's half along t0, t1
+  int32_t pad;
 };
 
-// Row i of the concatenated groups: its group and entry base and stride.
-__device__ __forceinline__ bool face_group_row(const FaceGroups& G, int64_t i, int64_t& e,
-                                               int64_t& stride, int& count) {
-#pragma unroll
-  for (int g = 0; g < 4; ++g) {
-    if (g < G.ngroups && i >= G.row0[g] && i < G.row0[g] + G.nrows[g]) {
-      e = G.entry0[g] + (i - G.row0[g]);
-      stride = G.nrows[g];
-      count = G.count[g];
-      return true;
-    }
-  }
-  return false;
-}
-
-// The complete ghosts of the coarse-fine block faces, from the face buffers the
-// previous stage wrote.  rows[i] is an exchange row (block, face, tangential
-// node) of such a face; its entries read face-buffer values src (source block,
-// its face, tangential node) with weights wu (u) and wg (gradient): the
-// tangential interpolation from a coarser neighbour, or the restriction from
-// the finer ones, injection along the normal.
-//
-// First the ghost u of every row, and the row's own complete face gradient:
-// the block-local part plus the ghost term with that ghost u, divided by the
-// block's spacing (evolution.jl:72-110 at a face node) -- every source of a
-// coarse-fine row lies on a coarse-fine face, so these are the gradients the
-// second launch gathers.
-template <class T, int P1>
-__global__ void k_face_ghost_u(int64_t ntot, FaceGroups G, int64_t N2,
-                               const int32_t* __restrict__ rows, const int32_t* __restrict__ src,
-                               const T* __restrict__ wu, const T* __restrict__ uf,
-                               const T* __restrict__ pf, const double* __restrict__ spacing,
-                               T* __restrict__ gu, T* __restrict__ fg) {
+// The complete ghosts of both sides of a coarse-fine face pair, from the face
+// buffers the previous stage wrote (u on the face plane, pf the block-local part
+// of the face gradient); injection along the normal, tensor products of the
+// element's 1D operators along the two tangential axes:
+//   F's ghost u = I (C's u on the quarter), C's ghost u = R (F's u), with I the
+//   coarse element's polynomial at its children's nodes and R = M_c^-1 I^T M_f;
+//   a face's complete gradient = (pf + W_ghost ghost u) / h (evolution.jl:72-110
+//   at a face node), and the ghost gradients the same I and R of the other
+//   side's complete gradients.
+// One workgroup per pair; every value stays in LDS between the passes.
+template <class T, int P1, int E>
+__global__ void __launch_bounds__(256) k_cf_ghosts(const CFJob* __restrict__ jobs,
+                                                   const T* __restrict__ uf,
+                                                   const T* __restrict__ pf,
+                                                   const double* __restrict__ spacing,
+                                                   T* __restrict__ gu, T* __restrict__ gg) {
+  constexpr int N = P1 * E, N2 = N * N, H = N / 2, H2 = H * H, P = P1 - 1;
   constexpr FusedTab<T, P1> tb = fused_tab<T, P1>();
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= ntot) return;
-  int64_t e, stride;
-  int count;
-  if (!face_group_row(G, i, e, stride, count)) return;
-  T acc = 0;
-#pragma unroll 8
-  for (int j = 0; j < count; ++j)
-    acc += ld_table(wu + e + j * stride) * uf[ld_table(src + e + j * stride)];
-  const int64_t r = ld_table(rows + i);
-  gu[r] = acc;
-  const int64_t face = r / N2;  // block * 6 + face
-  const T wc = (face % 2) ? tb.W[P1 - 1][P1 + 1] : tb.W[0][0];
-  fg[r] = (pf[r] + wc * acc) * (T(1) / (T)ld_table(spacing + face / 6));
-}
-
-// Then the ghost gradient of every row: the weighted sources' face gradients.
-template <class T>
-__global__ void k_face_ghost_g(int64_t ntot, FaceGroups G, const int32_t* __restrict__ rows,
-                               const int32_t* __restrict__ src, const T* __restrict__ wg,
-                               const T* __restrict__ fg, T* __restrict__ gg) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= ntot) return;
-  int64_t e, stride;
-  int count;
-  if (!face_group_row(G, i, e, stride, count)) return;
-  T acc = 0;
-#pragma unroll 8
-  for (int j = 0; j < count; ++j)
-    acc += ld_table(wg + e + j * stride) * fg[ld_table(src + e + j * stride)];
-  gg[ld_table(rows + i)] = acc;
+  static_assert(E % 2 == 0, "a coarse element's children pair up inside a block");
+  __shared__ T sI[2 * P1 * P1];  // I[h P1 + a][c]: child h's node a from coarse node c
+  __shared__ T sR[P1 * 2 * P1];  // R[c][h P1 + a]
+  __shared__ T sUF[N2], sPF[N2], sUC[H2], sPC[H2];
+  __shared__ T sA[N * H], sB[H * N];  // first-axis passes: I along t0 (N x H), R along t0 (H x N)
+  __shared__ T sGF[N2], sGC[H2];      // complete face gradients
+  const int tid = threadIdx.x;
+  const CFJob J = jobs[blockIdx.x];
+  const twoldg::ElementTables& et = twoldg::kElementTables[P1 - 2];
+  for (int k = tid; k < 2 * P1 * P1; k += 256) {
+    sI[k] = (T)et.interp[k / P1][k % P1];
+    sR[k] = (T)et.restrict_[k / (2 * P1)][k % (2 * P1)];
+  }
+  const int64_t of = ((int64_t)J.bf * 6 + J.ff) * N2;
+  const int64_t oc = ((int64_t)J.bc * 6 + J.fc) * N2 + J.q;
+  for (int k = tid; k < N2; k += 256) {
+    sUF[k] = uf[of + k];
+    sPF[k] = pf[of + k];
+  }
+  for (int k = tid; k < H2; k += 256) {
+    const int64_t c = oc + (k % H) + (int64_t)N * (k / H);
+    sUC[k] = uf[c];
+    sPC[k] = pf[c];
+  }
+  const T rhf = T(1) / (T)spacing[J.bf], rhc = T(1) / (T)spacing[J.bc];
+  // the ghost term of each side's own face row (row 0 on a low face, P on a high one)
+  const T wf = (J.ff % 2) ? tb.W[P][P1 + 1] : tb.W[0][0];
+  const T wc = (J.fc % 2) ? tb.W[P][P1 + 1] : tb.W[0][0];
+  __syncthreads();
+  // I along one axis: fine node i of element fe = i / P1 from coarse element fe / 2
+  auto interp = [&](const T* src, int i, int stride) {
+    const int fe = i / P1, row = (fe % 2) * P1 + i % P1;
+    const T* s = src + (fe / 2) * P1 * stride;
+    T acc = 0;
+#pragma unroll
+    for (int c = 0; c < P1; ++c) acc += sI[row * P1 + c] * s[c * stride];
+    return acc;
+  };
+  // R along one axis: coarse node c of element ce = c / P1 from fine elements 2 ce, 2 ce + 1
+  auto restrict1 = [&](const T* src, int c, int stride) {
+    const int ce = c / P1, row = c % P1;
+    const T* s = src + 2 * ce * P1 * stride;
+    T acc = 0;
+#pragma unroll
+    for (int f = 0; f < 2 * P1; ++f) acc += sR[row * 2 * P1 + f] * s[f * stride];
+    return acc;
+  };
+  // two passes of the same operators: on u (then the complete gradients), on those
+#pragma unroll 1
+  for (int pass = 0; pass < 2; ++pass) {
+    const T* uc = pass ? sGC : sUC;  // C's quarter, H x H
+    const T* ufn = pass ? sGF : sUF; // F's face, N x N
+    for (int k = tid; k < N * H; k += 256) {
+      const int i = k % N, cj = k / N;  // sA[cj][i]: I along t0 of C's row cj
+      sA[k] = interp(uc + cj * H, i, 1);
+      const int ci = k % H, j = k / H;  // sB[j][ci]: R along t0 of F's row j
+      sB[k] = restrict1(ufn + j * N, ci, 1);
+    }
+    __syncthreads();
+    T* out = pass ? gg : gu;
+    for (int k = tid; k < N2; k += 256) {
+      const int i = k % N, j = k / N;
+      const T g = interp(sA + i, j, N);  // along t1: column i of sA
+      out[of + k] = g;
+      if (pass == 0) sGF[k] = (sPF[k] + wf * g) * rhf;
+    }
+    for (int k = tid; k < H2; k += 256) {
+      const int ci = k % H, cj = k / H;
+      const T g = restrict1(sB + ci, cj, H);  // along t1: column ci of sB
+      out[oc + ci + (int64_t)N * cj] = g;
+      if (pass == 0) sGC[k] = (sPC[k] + wc * g) * rhc;
+    }
+    __syncthreads();
+  }
 }
 
 // The face buffers of a state (before the first fused stage): per block face,

@@ -171,8 +171,7 @@ void destroy(twoldg_problem* p) {
                   (void*)p->fslot, (void*)p->fradial, (void*)p->fblk, (void*)p->oblk, p->frmodes, p->frduals, (void*)p->faslot, p->famodes, p->faweights, p->faginv,
                   (void*)p->rad_rows, (void*)p->rpart, (void*)p->svn, (void*)p->svlo,
                   (void*)p->svs, (void*)p->svacc, p->flux, (void*)p->nbr6, p->uf[0], p->uf[1], p->pf[0],
-                  p->pf[1], (void*)p->frows, (void*)p->fsrc, p->fwu,
-                  p->fwg, p->fgu, p->fgg, p->ffg})
+                  p->pf[1], (void*)p->cfjobs, p->fgu, p->fgg})
     dfree(b);
   if (p->step_exec) (void)hipGraphExecDestroy(p->step_exec);
   if (p->step_graph) (void)hipGraphDestroy(p->step_graph);
@@ -427,72 +426,42 @@ int create_buffers(twoldg_problem* p, const twoldg_desc* d, const Csr& csr) {
       eligible = false;
   }
   if (eligible && nonconforming && !p->no_fused) {
-    // the coarse-fine faces' exchange rows, re-indexed into the face buffers
-    // (sources (block, face, t)) and grouped by entry count (P+1)^2 from a
-    // coarser neighbour, (2P+2)^2 from the finer ones; each group is stored
-    // entry-major (ELL), so that a wave's rows read consecutive entries
-    const int N = p->N;
+    // the coarse-fine faces as (fine face, coarse quarter) pairs: every fine
+    // face's coarser neighbour must face back through the quarter it names
+    const int N = p->N, H = N / 2;
     const int64_t N2 = (int64_t)N * N;
-    std::map<int, std::vector<int32_t>> group;  // entry count -> rows
+    std::vector<CFJob> jobs;
     for (int64_t b = 0; b < p->sh.nb; ++b)
       for (int f = 0; f < 6; ++f) {
-        const int kind = d->faces[b * 6 + f].kind;
-        if (kind != TWOLDG_FACE_COARSER && kind != TWOLDG_FACE_FINER) continue;
-        for (int64_t t = 0; t < N2; ++t) {
-          const int64_t r = (b * 6 + f) * N2 + t;
-          if (r > INT32_MAX) return fail(TWOLDG_EINVAL, "face buffers too large");
-          group[csr.rowptr[r + 1] - csr.rowptr[r]].push_back((int32_t)r);
+        const twoldg_face& F = d->faces[b * 6 + f];
+        if (F.kind == TWOLDG_FACE_FINER) {
+          for (int k = 0; k < 4; ++k) {
+            const int64_t n = F.nb[k];
+            if (n < 0 || n >= p->sh.nb || d->faces[n * 6 + (f ^ 1)].kind != TWOLDG_FACE_COARSER ||
+                d->faces[n * 6 + (f ^ 1)].nb[0] != b)
+              return fail(TWOLDG_EINVAL, "block " + std::to_string(b) + " face " +
+                                             std::to_string(f) + ": a finer neighbour does not face back");
+          }
+          continue;
         }
+        if (F.kind != TWOLDG_FACE_COARSER) continue;
+        const int dd = f / 2, t0 = dd == 0 ? 1 : 0, t1 = dd == 2 ? 1 : 2;
+        const int64_t c = F.nb[0];
+        if (c < 0 || c >= p->sh.nb || d->faces[c * 6 + (f ^ 1)].kind != TWOLDG_FACE_FINER ||
+            F.off[t0] < 0 || F.off[t0] > 1 || F.off[t1] < 0 || F.off[t1] > 1 ||
+            d->faces[c * 6 + (f ^ 1)].nb[F.off[t0] + 2 * F.off[t1]] != b)
+          return fail(TWOLDG_EINVAL, "block " + std::to_string(b) + " face " + std::to_string(f) +
+                                         ": its coarser neighbour does not face back");
+        if ((int64_t)p->sh.nb * 6 * N2 > INT32_MAX)
+          return fail(TWOLDG_EINVAL, "face buffers too large");
+        jobs.push_back(CFJob{(int32_t)b, f, (int32_t)c, f ^ 1, F.off[t0] * H + N * (F.off[t1] * H), 0});
       }
-    std::vector<int32_t> rows, src;
-    std::vector<double> wu, wg;
-    for (const auto& g : group) {
-      const int cnt = g.first;
-      const std::vector<int32_t>& gr = g.second;
-      const int64_t R = (int64_t)gr.size();
-      if (p->fgroups.ngroups == 4) return fail(TWOLDG_EINVAL, "more than four coarse-fine row kinds");
-      const int gi = p->fgroups.ngroups++;
-      p->fgroups.count[gi] = cnt;
-      p->fgroups.row0[gi] = (int64_t)rows.size();
-      p->fgroups.entry0[gi] = (int64_t)src.size();
-      p->fgroups.nrows[gi] = R;
-      rows.insert(rows.end(), gr.begin(), gr.end());
-      const size_t e0 = src.size();
-      src.resize(e0 + (size_t)cnt * R);
-      wu.resize(src.size());
-      wg.resize(src.size());
-      for (int64_t i = 0; i < R; ++i) {
-        const int64_t r = gr[i];
-        const int f = (int)((r / N2) % 6), dd = f / 2, side = f % 2, sf = 2 * dd + (1 - side);
-        const int t0 = dd == 0 ? 1 : 0, t1 = dd == 2 ? 1 : 2;
-        for (int j = 0; j < cnt; ++j) {
-          const int32_t k = csr.rowptr[r] + j;
-          const int64_t l = csr.cl[k];
-          const int I[3] = {(int)(l % N), (int)((l / N) % N), (int)(l / N2)};
-          if (I[dd] != (side ? 0 : N - 1))
-            return fail(TWOLDG_EINVAL, "an exchange source is not on the expected face");
-          const int64_t v = ((int64_t)csr.cb[k] * 6 + sf) * N2 + I[t0] + (int64_t)N * I[t1];
-          if (v > INT32_MAX) return fail(TWOLDG_EINVAL, "face buffers too large");
-          const size_t at = e0 + (size_t)j * R + i;
-          src[at] = (int32_t)v;
-          wu[at] = csr.vu[k];
-          wg[at] = csr.vg[k];
-        }
-      }
-    }
-    if ((rc = dalloc(p, &p->frows, rows.size()))) return rc;
-    if ((rc = dalloc(p, &p->fsrc, src.size()))) return rc;
-    if ((rc = h2d(p, p->frows, rows.data(), rows.size() * 4))) return rc;
-    if ((rc = h2d(p, p->fsrc, src.data(), src.size() * 4))) return rc;
-    if ((rc = upload_as<T>(p, &p->fwu, wu.data(), wu.size()))) return rc;
-    // coarse-fine rows weigh u and the gradient alike: then both ghost launches
-    // read one array (fwg stays null)
-    if (wg != wu && (rc = upload_as<T>(p, &p->fwg, wg.data(), wg.size()))) return rc;
+    if ((rc = dalloc_bytes(p, (void**)&p->cfjobs, jobs.size() * sizeof(CFJob)))) return rc;
+    if ((rc = h2d(p, p->cfjobs, jobs.data(), jobs.size() * sizeof(CFJob)))) return rc;
     const size_t nf = (size_t)p->sh.nb * 6 * N2;
     if ((rc = dalloc_bytes(p, &p->fgu, nf * sizeof(T)))) return rc;
     if ((rc = dalloc_bytes(p, &p->fgg, nf * sizeof(T)))) return rc;
-    if ((rc = dalloc_bytes(p, &p->ffg, nf * sizeof(T)))) return rc;
-    p->nfrows = (int64_t)rows.size();
+    p->ncf = (int64_t)jobs.size();
     p->ghosted = true;
   }
   if (eligible && !p->no_fused) {

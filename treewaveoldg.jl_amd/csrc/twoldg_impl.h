@@ -1078,18 +1078,13 @@ struct twoldg_problem {
   int32_t* nbr6 = nullptr;  // (nb, 6)
   void* uf[2] = {nullptr, nullptr};
   void* pf[2] = {nullptr, nullptr};
-  // ghosted fused stages (coarse-fine faces): the exchange rows re-indexed into
-  // the face buffers, and the complete face ghosts (nb, 6, N^2)
+  // ghosted fused stages (coarse-fine faces): the (fine face, coarse quarter)
+  // pairs, and the complete face ghosts (nb, 6, N^2) formed from them
   bool ghosted = false;
-  FaceGroups fgroups{};        // the groups of coarse-fine rows (entry counts)
-  int64_t nfrows = 0;
-  int32_t* frows = nullptr;    // exchange rows of the coarse-fine faces, by group
-  int32_t* fsrc = nullptr;     // face-buffer index of each entry (entry-major)
-  void* fwu = nullptr;         // weights (u), in T
-  void* fwg = nullptr;         // weights (gradient), in T
+  int64_t ncf = 0;
+  CFJob* cfjobs = nullptr;
   void* fgu = nullptr;
   void* fgg = nullptr;
-  void* ffg = nullptr;         // the coarse-fine rows' own face gradients
   // split form: v of the step start as an FP32 (hi, lo) pair, v of the stage
   // state and of the accumulator in FP32, (nb, N^3) each
   float* svn = nullptr;
@@ -1572,19 +1567,13 @@ int fused_run(twoldg_problem* p, double dt, int64_t nsteps) {
     }
   };
   // the complete ghosts of the coarse-fine faces from the stage input's face
-  // buffers (ghosted form)
+  // buffers (ghosted form): one launch, a workgroup per face pair
   auto ghosts = [&](int in) {
     if constexpr (GHOST) {
-      const int64_t nfr = p->nfrows;
-      if (nfr == 0) return;
-      hipLaunchKernelGGL((k_face_ghost_u<T, P1>), dim3(grid_for(nfr, 256)), dim3(256), 0,
-                         p->stream, nfr, p->fgroups, (int64_t)N * N, p->frows, p->fsrc,
-                         (const T*)p->fwu, (const T*)p->uf[in], (const T*)p->pf[in], p->spacing,
-                         (T*)p->fgu, (T*)p->ffg);
-      hipLaunchKernelGGL((k_face_ghost_g<T>), dim3(grid_for(nfr, 256)), dim3(256), 0, p->stream,
-                         nfr, p->fgroups, p->frows, p->fsrc,
-                         (const T*)(p->fwg ? p->fwg : p->fwu),
-                         (const T*)p->ffg, (T*)p->fgg);
+      if (p->ncf == 0) return;
+      hipLaunchKernelGGL((k_cf_ghosts<T, P1, E>), dim3((unsigned)p->ncf), dim3(256), 0, p->stream,
+                         p->cfjobs, (const T*)p->uf[in], (const T*)p->pf[in], p->spacing,
+                         (T*)p->fgu, (T*)p->fgg);
     }
   };
   auto io = [&](int in) {
