@@ -213,17 +213,25 @@ def nested_cubes(large=False, steps=500):
         timed = steps - warm
         alg = BYTES_PER_NODE_UPDATE[prec] * nodes * timed
         final[prec] = integ.u
+        # the error against the exact cos standing mode at the end (reflect_exact)
+        exact = T.fill_at_nodes(T.reflect_exact(3, 1.0, 2, steps * dt), prob)
+        prob.upload(exact, T._lib.SLOT_AUX)
+        from twoldg.drivers import _errors
+        l2, linf = _errors(prob, (1, 2))
         out[prec] = dict(value=nodes * timed / wall, ms_per_step=wall * 1e3 / timed,
                          alg_gbs_whole_step=alg / wall / 1e9, energy_drift=(e1 - e0) / e0,
-                         steps=steps)
+                         steps=steps, l2_vs_exact=l2, linf_vs_exact=linf)
         log(f"[bench] nested cubes{' (large)' if large else ''} {prec}: {forest.nblocks} blocks, "
             f"{steps} steps, {nodes * timed / wall:.3e} node-updates/s, "
-            f"{alg / wall / 1e9:.0f} GB/s algorithmic, energy drift {(e1 - e0) / e0:.2e}")
+            f"{alg / wall / 1e9:.0f} GB/s algorithmic, energy drift {(e1 - e0) / e0:.2e}, "
+            f"L2 vs exact {l2:.4e}")
         prob.close()
     rel = float(np.linalg.norm(final["f32c"] - final["f64"]) / np.linalg.norm(final["f64"]))
     out["f32c_vs_f64_rel_l2"] = rel
+    out["f32c_over_f64_l2_vs_exact"] = out["f32c"]["l2_vs_exact"] / out["f64"]["l2_vs_exact"]
     log(f"[bench] nested cubes{' (large)' if large else ''}: FP32c vs FP64 after {steps} steps, "
-        f"relative L2 {rel:.2e}")
+        f"relative L2 {rel:.2e}; L2 vs exact FP32c / FP64 "
+        f"{out['f32c_over_f64_l2_vs_exact']:.3f}")
     out["nodes"] = nodes
     out["blocks"] = forest.nblocks
     return out
@@ -237,6 +245,7 @@ def batch2d(steps=20):
     probs = gaussian_pulses()
     u0s = initial_states(probs)
     out = {}
+    final = {}
     for prec in ("f32c", "f64"):
         t0 = time.perf_counter()
         batch = T.Batch([p[:3] for p in probs], precision=prec)
@@ -246,6 +255,7 @@ def batch2d(steps=20):
         batch.rk4(2)
         ms = batch.rk4(steps)
         info = batch.info()
+        final[prec] = [batch.download(i) for i in range(len(probs))]
         nodes = info["nodes"]
         alg = BYTES_PER_NODE_UPDATE[prec] * nodes * steps
         out[prec] = dict(value=nodes * steps / (ms * 1e-3), ms_per_step=ms / steps,
@@ -255,6 +265,16 @@ def batch2d(steps=20):
             f"{nodes * steps / (ms * 1e-3):.3e} node-updates/s ({ms / steps:.3f} ms/step, "
             f"{alg / (ms * 1e-3) / 1e9:.0f} GB/s), tiles {info['tiles']}")
         batch.close()
+    # FP32c against FP64 after the same 2 + steps steps: over the whole batch and
+    # the worst problem (relative L2)
+    a32 = np.concatenate(final["f32c"])
+    a64 = np.concatenate(final["f64"])
+    out["f32c_vs_f64_rel_l2"] = float(np.linalg.norm(a32 - a64) / np.linalg.norm(a64))
+    out["f32c_vs_f64_rel_l2_worst_problem"] = max(
+        float(np.linalg.norm(x - y) / np.linalg.norm(y)) for x, y in zip(final["f32c"], final["f64"]))
+    log(f"[bench] 2D batch: FP32c vs FP64 after {steps + 2} steps, relative L2 "
+        f"{out['f32c_vs_f64_rel_l2']:.2e} (worst problem "
+        f"{out['f32c_vs_f64_rel_l2_worst_problem']:.2e})")
     out["nodes"] = nodes
     out["problems"] = len(probs)
     out["seed"] = 20261203
@@ -282,7 +302,48 @@ def ball(steps=20):
             f"{nodes * steps / wall:.3e} node-updates/s")
         prob.close()
     out["f32c_over_f64_l2"] = out["f32c"]["l2"] / out["f64"]["l2"]
+    out["glue_energy_balance"] = glue_energy_balance(T)
     return out
+
+
+def glue_energy_balance(T, P=3, E=2, roots=(4, 4), seed=5):
+    """The energy flux across the seven-patch ball's glues (sevenpatch_tests.jl:
+    182-226) on the benchmarked ball, FP64 and FP32c: zero Dirichlet data and a
+    seeded random state that is zero in the outer shells' outermost elements, so
+    that nothing crosses the sphere; the rate of the discrete energy along the
+    right-hand side, (E(y + e f) - E(y - e f)) / 2e (exact for the quadratic
+    energy), is then the glues' net flux, reported against the Cauchy-Schwarz
+    bound 2 sqrt(E(y) E(f)) of its terms."""
+    forest, geo = T.ball_forest(P, E, roots=roots)
+    N = forest.N
+    faces = forest.faces()
+    res = {}
+    for prec in ("f64", "f32c"):
+        prob = T.WaveProblem(forest, T.Element(P), precision=prec, geometry=geo,
+                             boundary=T.ZeroData())
+        y = np.random.default_rng(seed).standard_normal(prob.state_len).reshape(
+            forest.nblocks, 2, N, N, N)
+        for b in range(forest.nblocks):
+            for f in range(6):
+                if faces[b * 6 + f].kind != T._lib.FACE_DIRICHLET:
+                    continue
+                d, side = divmod(f, 2)
+                idx = [slice(None)] * 3
+                idx[2 - d] = slice(N - P - 1, N) if side else slice(0, P + 1)
+                y[b][(slice(None),) + tuple(idx)] = 0.0
+        y = T.truncate_state(y.reshape(-1), prob)
+        f = np.empty_like(y)
+        T.wave_rhs(f, y, prob, 0.0)
+        ey, ef = T.wave_energy(prob, y, t=0.0), T.wave_energy(prob, f, t=0.0)
+        eps = math.sqrt(ey / ef)
+        rate = (T.wave_energy(prob, y + eps * f, t=0.0) -
+                T.wave_energy(prob, y - eps * f, t=0.0)) / (2 * eps)
+        bound = 2 * math.sqrt(ey * ef)
+        res[prec] = dict(rate=rate, bound=bound, rel=abs(rate) / bound)
+        log(f"[bench] seven-patch ball glues {prec}: energy rate {rate:.3e} against the "
+            f"bound {bound:.3e} of its terms ({abs(rate) / bound:.1e})")
+        prob.close()
+    return res
 
 
 def annulus(s=8):
