@@ -143,8 +143,10 @@ int twoldg_device_count(int* out);
 /* --- problem (WaveProblem, src/evolution.jl:1043-1108) ------------------- */
 int twoldg_problem_create(const twoldg_desc* desc, twoldg_problem** out);
 int twoldg_problem_destroy(twoldg_problem* p);
-/* state vector length, device bytes held, and whether RK4 runs the fused
-   one-launch stages (1) or the generic exchange + gradient + stage sequence (0) */
+/* state vector length, device bytes held, and which stages RK4 runs: the fused
+   one-launch 3D stages (1), the batch stage on a two-dimensional uniform flat
+   problem (2: one launch per stage), or the generic exchange + gradient + stage
+   sequence (0) */
 int twoldg_problem_info(const twoldg_problem* p, int64_t* state_len, int64_t* device_bytes,
                         int32_t* fused);
 

@@ -114,6 +114,10 @@ def test_wave_errors_2d_p4_matches_oracle():
 ], ids=_id)
 def test_rk4_matches_oracle(case):
     prob, ref = _pair(**case)
+    # a 2D uniform problem steps by the batch stage, a refined one by the generic
+    # stages, the 3D refined forest by the ghosted fused stage
+    want = {2: "generic" if case.get("regions") else "batch2d", 3: "fused3d"}[case["D"]]
+    assert prob.info()["stages"] == want
     refl = case.get("reflecting", False)
     f = T.reflect_exact if refl else T.wave_exact
     u0 = T.initial_state(prob, f(case["D"], 1.0, 2 if refl else 1, 0.0))
@@ -513,9 +517,9 @@ def test_generic_step_graph_replays_match_eager_steps(which):
         wave = T.PlaneWave((1.1, -0.7, 0.5), 0.3)
         mk = lambda: T.WaveProblem(forest, T.Element(3), geometry=geo, boundary=wave)
         f0 = T.sevenpatch_exact(wave, 0.0)
-    else:
+    else:  # the generic stages (a 2D uniform problem otherwise runs the batch stage)
         forest = T.wave_forest(2, 4, 2, roots=4)
-        mk = lambda: T.WaveProblem(forest, T.Element(4))
+        mk = lambda: T.WaveProblem(forest, T.Element(4), generic=True)
         f0 = T.wave_exact(2, 1.0, 1, 0.0)
     a, b = mk(), mk()
     assert not a.info()["fused"]

@@ -32,6 +32,10 @@ struct twoldg_batch {
   hipGraphExec_t exec = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int64_t device_bytes = 0;
+  // a problem's own batch of one: the state, stage, accumulator and
+  // compensation buffers and the stream are the problem's
+  bool borrowed = false;
+  double cur_dt = 0;  // the time step in the device table (borrowed form)
 };
 
 namespace {
@@ -322,17 +326,29 @@ int batch_build(twoldg_batch* B, const twoldg_desc* const* d, int64_t n, const d
   if ((rc = bh2d(B, B->spacing, hh.data(), nb * 8))) return rc;
   if ((rc = bh2d(B, B->dt, dth.data(), nb * 8))) return rc;
   if ((rc = batch_tables<T>(B))) return rc;
-  for (void** p : {&B->yn, &B->ys, &B->acc})
-    if ((rc = balloc(B, p, total * sizeof(T)))) return rc;
-  if (B->precision == TWOLDG_F32C && (rc = balloc(B, &B->lo, total * sizeof(T)))) return rc;
+  if (!B->borrowed) {
+    for (void** p : {&B->yn, &B->ys, &B->acc})
+      if ((rc = balloc(B, p, total * sizeof(T)))) return rc;
+    if (B->precision == TWOLDG_F32C && (rc = balloc(B, &B->lo, total * sizeof(T)))) return rc;
+    for (void* p : {B->yn, B->ys, B->acc, B->lo})
+      if (p) HIP_TRY(hipMemsetAsync(p, 0, total * sizeof(T), B->stream));
+  }
   for (int k = 0; k < 2; ++k)
     if ((rc = balloc(B, &B->uf[k], fsum * sizeof(T))) || (rc = balloc(B, &B->pf[k], fsum * sizeof(T))))
       return rc;
-  for (void* p : {B->yn, B->ys, B->acc, B->lo, B->uf[0], B->uf[1], B->pf[0], B->pf[1]})
-    if (p)
-      HIP_TRY(hipMemsetAsync(p, 0, (p == B->uf[0] || p == B->uf[1] || p == B->pf[0] || p == B->pf[1] ? fsum : total) * sizeof(T), B->stream));
-  // tiles per order, then the work list, then the step graph
-  if ((rc = autotune<T>(B, Nh))) return rc;
+  for (void* p : {B->uf[0], B->uf[1], B->pf[0], B->pf[1]})
+    HIP_TRY(hipMemsetAsync(p, 0, fsum * sizeof(T), B->stream));
+  // tiles per order (timed; a problem's own batch of one takes the largest
+  // tile that still leaves two workgroups per CU, so a small problem spreads
+  // over the chip), then the work list, then the step graph
+  if (B->borrowed) {
+    const int64_t nb1 = B->nb;
+    const int P1 = B->P1s_host[0], N = Nh[0];
+    B->tiles[P1] = (int)std::max<int64_t>(1, std::min<int64_t>(max_tile(N, N / P1, sizeof(T)),
+                                                               nb1 / 512));
+  } else if ((rc = autotune<T>(B, Nh))) {
+    return rc;
+  }
   B->nwg = make_runs(B, Nh, B->tiles, 0, B->runs);
   HIP_TRY(hipStreamSynchronize(B->stream));
   HIP_TRY(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
@@ -348,12 +364,14 @@ void batch_destroy(twoldg_batch* B) {
   if (B->exec) (void)hipGraphExecDestroy(B->exec);
   if (B->graph) (void)hipGraphDestroy(B->graph);
   for (void* p : {(void*)B->off, (void*)B->foff, (void*)B->nfo, B->nrh, (void*)B->P1s, (void*)B->Ns,
-                  (void*)B->spacing, (void*)B->dt, B->tables, B->yn, B->ys,
-                  B->acc, B->lo, B->uf[0], B->uf[1], B->pf[0], B->pf[1]})
+                  (void*)B->spacing, (void*)B->dt, B->tables, B->uf[0], B->uf[1], B->pf[0], B->pf[1]})
     if (p) pool_free(p);
+  if (!B->borrowed)
+    for (void* p : {B->yn, B->ys, B->acc, B->lo})
+      if (p) pool_free(p);
   if (B->ev0) (void)hipEventDestroy(B->ev0);
   if (B->ev1) (void)hipEventDestroy(B->ev1);
-  if (B->stream) (void)hipStreamDestroy(B->stream);
+  if (B->stream && !B->borrowed) (void)hipStreamDestroy(B->stream);
   delete B;
 }
 
@@ -369,6 +387,97 @@ int batch_steps(twoldg_batch* B, int64_t nsteps) {
 
 }  // namespace
 
+namespace twoldg_detail {
+
+// Whether a description is one a batch takes; why not, if `why`.
+bool batch_desc_ok(const twoldg_desc* d, std::string* why) {
+  auto no = [&](const std::string& w) {
+    if (why) *why = w;
+    return false;
+  };
+  if (!d) return no("no description");
+  if (d->abi_version != TWOLDG_ABI_VERSION)
+    return no("ABI version " + std::to_string(d->abi_version) + " does not match the library's " +
+              std::to_string(TWOLDG_ABI_VERSION));
+  if (d->D != 2 || d->coef || d->dcoef) return no("a batch takes flat two-dimensional problems");
+  if (d->precision != TWOLDG_F64 && d->precision != TWOLDG_F32C)
+    return no("unknown precision " + std::to_string(d->precision));
+  if (d->P < 1 || d->P > 7 || !batch_e_ok(d->E)) return no("P must be 1..7 and E 2 or 4");
+  if (d->nblocks < 1 || !d->spacing || !d->faces)
+    return no("a problem needs at least one block, its spacings and faces");
+  const int N = d->E * (d->P + 1);
+  if (max_tile(N, d->E, d->precision == TWOLDG_F32C ? 4 : 8) < 1)
+    return no("blocks of " + std::to_string(N) + "^2 nodes exceed a workgroup");
+  for (int64_t b = 0; b < d->nblocks; ++b)
+    if (!(d->spacing[b] > 0)) return no("block spacing must be positive");
+  for (int64_t f = 0; f < d->nblocks * 4; ++f) {
+    const int k = d->faces[f].kind;
+    if (k != TWOLDG_FACE_SAME && k != TWOLDG_FACE_MIRROR)
+      return no("a batch takes uniform problems (same-level or mirror faces)");
+    if (k == TWOLDG_FACE_SAME && (d->faces[f].nb[0] < 0 || d->faces[f].nb[0] >= d->nblocks))
+      return no("face neighbour out of range");
+  }
+  // a block's faces go into its neighbours' ghost slots, so the neighbour
+  // across face 2d + s must face back across 2d + 1 - s
+  for (int64_t f = 0; f < d->nblocks * 4; ++f) {
+    if (d->faces[f].kind != TWOLDG_FACE_SAME) continue;
+    const twoldg_face& back = d->faces[d->faces[f].nb[0] * 4 + ((f % 4) ^ 1)];
+    if (back.kind != TWOLDG_FACE_SAME || back.nb[0] != f / 4)
+      return no("block " + std::to_string(f / 4) + " face " + std::to_string(f % 4) +
+                ": its neighbour does not face back");
+  }
+  return true;
+}
+
+// A two-dimensional problem's own batch of one: the batch stage (one launch
+// per RK4 stage, a graph per step) on the problem's state (yn, with its
+// compensation lo in FP32C), stage and accumulator buffers, on its stream.
+int batch_attach(const twoldg_desc* d, void* yn, void* ys, void* acc, void* lo,
+                 hipStream_t stream, twoldg_batch** out) {
+  *out = nullptr;
+  if (!batch_desc_ok(d, nullptr)) return fail(TWOLDG_EINVAL, "not a batch problem");
+  auto* B = new twoldg_batch();
+  B->borrowed = true;
+  B->precision = d->precision;
+  B->truncate = d->truncate != 0;
+  B->elt = d->precision == TWOLDG_F64 ? 8 : 4;
+  B->stream = stream;
+  B->yn = yn;
+  B->ys = ys;
+  B->acc = acc;
+  B->lo = lo;
+  (void)hipEventCreate(&B->ev0);
+  (void)hipEventCreate(&B->ev1);
+  const double dt1 = 1.0;  // the table is written before the first steps
+  const twoldg_desc* const ds[1] = {d};
+  int rc = d->precision == TWOLDG_F64 ? batch_build<double>(B, ds, 1, &dt1)
+                                      : batch_build<float>(B, ds, 1, &dt1);
+  if (rc) {
+    batch_destroy(B);
+    return rc;
+  }
+  B->cur_dt = NAN;
+  *out = B;
+  return 0;
+}
+
+void batch_detach(twoldg_batch* B) { batch_destroy(B); }
+
+// nsteps RK4 steps of the attached problem with time step dt.
+int batch_attached_steps(twoldg_batch* B, double dt, int64_t nsteps) {
+  if (nsteps <= 0) return 0;
+  if (!(dt == B->cur_dt)) {
+    std::vector<double> dts((size_t)B->nb, dt);
+    int rc = bh2d(B, B->dt, dts.data(), dts.size() * sizeof(double));
+    if (rc) return rc;
+    B->cur_dt = dt;
+  }
+  return B->precision == TWOLDG_F64 ? batch_steps<double>(B, nsteps) : batch_steps<float>(B, nsteps);
+}
+
+}  // namespace twoldg_detail
+using twoldg_detail::batch_desc_ok;
+
 extern "C" {
 
 int twoldg_batch_create(const twoldg_desc* const* descs, int64_t n, const double* dt,
@@ -379,44 +488,11 @@ int twoldg_batch_create(const twoldg_desc* const* descs, int64_t n, const double
   for (int64_t i = 0; i < n; ++i) {
     const twoldg_desc* d = descs[i];
     const std::string who = "problem " + std::to_string(i) + ": ";
-    if (!d) return fail(TWOLDG_EINVAL, who + "no description");
-    if (d->abi_version != TWOLDG_ABI_VERSION)
-      return fail(TWOLDG_EINVAL, who + "ABI version " + std::to_string(d->abi_version) +
-                                     " does not match the library's " +
-                                     std::to_string(TWOLDG_ABI_VERSION));
-    if (d->D != 2 || d->coef)
-      return fail(TWOLDG_EINVAL, who + "a batch takes flat two-dimensional problems");
-    if (d->precision != TWOLDG_F64 && d->precision != TWOLDG_F32C)
-      return fail(TWOLDG_EINVAL, who + "unknown precision " + std::to_string(d->precision));
-    if (d->P < 1 || d->P > 7 || !batch_e_ok(d->E))
-      return fail(TWOLDG_EINVAL, who + "P must be 1..7 and E 2 or 4");
-    if (d->nblocks < 1 || !d->spacing || !d->faces)
-      return fail(TWOLDG_EINVAL, who + "a problem needs at least one block, its spacings and faces");
-    const int N = d->E * (d->P + 1);
-    if (max_tile(N, d->E, d->precision == TWOLDG_F32C ? 4 : 8) < 1)
-      return fail(TWOLDG_EINVAL, who + "blocks of " + std::to_string(N) +
-                                     "^2 nodes exceed a workgroup");
+    std::string why;
+    if (!batch_desc_ok(d, &why)) return fail(TWOLDG_EINVAL, who + why);
     if (d->precision != descs[0]->precision || d->truncate != descs[0]->truncate)
       return fail(TWOLDG_EINVAL, "every problem of a batch shares its precision and truncation");
     if (!(dt[i] > 0)) return fail(TWOLDG_EINVAL, who + "time steps must be positive");
-    for (int64_t b = 0; b < d->nblocks; ++b)
-      if (!(d->spacing[b] > 0)) return fail(TWOLDG_EINVAL, who + "block spacing must be positive");
-    for (int64_t f = 0; f < d->nblocks * 4; ++f) {
-      const int k = d->faces[f].kind;
-      if (k != TWOLDG_FACE_SAME && k != TWOLDG_FACE_MIRROR)
-        return fail(TWOLDG_EINVAL, who + "a batch takes uniform problems (same-level or mirror faces)");
-      if (k == TWOLDG_FACE_SAME && (d->faces[f].nb[0] < 0 || d->faces[f].nb[0] >= d->nblocks))
-        return fail(TWOLDG_EINVAL, who + "face neighbour out of range");
-    }
-    // a block's faces go into its neighbours' ghost slots, so the neighbour
-    // across face 2d + s must face back across 2d + 1 - s
-    for (int64_t f = 0; f < d->nblocks * 4; ++f) {
-      if (d->faces[f].kind != TWOLDG_FACE_SAME) continue;
-      const twoldg_face& back = d->faces[d->faces[f].nb[0] * 4 + ((f % 4) ^ 1)];
-      if (back.kind != TWOLDG_FACE_SAME || back.nb[0] != f / 4)
-        return fail(TWOLDG_EINVAL, who + "block " + std::to_string(f / 4) + " face " +
-                                       std::to_string(f % 4) + ": its neighbour does not face back");
-    }
   }
   const int prec = descs[0]->precision;
   const int trunc = descs[0]->truncate;

@@ -162,6 +162,7 @@ namespace {
 void destroy(twoldg_problem* p) {
   if (!p) return;
   if (p->stream) (void)hipStreamSynchronize(p->stream);
+  if (p->b2) twoldg_detail::batch_detach(p->b2);
   if (p->pinned) pool_free(p->pinned);
   for (void* b : {p->slot[0], p->slot[1], p->slot[2], p->slot[3], p->lo, p->ys, p->acc, p->ug, p->gg,
                   p->grads, p->tables, p->coef, (void*)p->flat, (void*)p->tslot, p->tmodes,
@@ -627,6 +628,11 @@ int twoldg_problem_create(const twoldg_desc* d, twoldg_problem** out) {
   (void)hipEventCreate(&p->ev0);
   (void)hipEventCreate(&p->ev1);
   rc = p->precision != TWOLDG_F32C ? create_buffers<double>(p, d, csr) : create_buffers<float>(p, d, csr);
+  // two-dimensional uniform flat problems step by the batch stage: one launch
+  // per RK4 stage, a graph per step
+  if (!rc && p->D == 2 && !p->no_fused && twoldg_detail::batch_desc_ok(d, nullptr))
+    rc = twoldg_detail::batch_attach(d, p->slot[TWOLDG_SLOT_STATE], p->ys, p->acc, p->lo,
+                                     p->stream, &p->b2);
   if (!rc && p->fused) {
     // One throwaway fused RK4 step over the stage buffers (the state slot is not
     // touched; the compensation is cleared after it).  The first fused step run
@@ -664,7 +670,7 @@ int twoldg_problem_info(const twoldg_problem* p, int64_t* state_len, int64_t* de
   if (!p) return fail(TWOLDG_EINVAL, "no problem");
   if (state_len) *state_len = p->state_len;
   if (device_bytes) *device_bytes = p->device_bytes;
-  if (fused) *fused = p->fused ? 1 : 0;
+  if (fused) *fused = p->fused ? 1 : (p->b2 ? 2 : 0);
   return 0;
 }
 

@@ -27,6 +27,14 @@ int fail(int code, const std::string& msg);
 // repeat of that step on the same problem).
 hipError_t pool_alloc(void** ptr, size_t bytes, bool host);
 void pool_free(void* ptr);
+// A two-dimensional problem's own batch of one (batch.hip): whether a
+// description is one the batch stage takes, and the batch on the problem's
+// buffers and stream.
+bool batch_desc_ok(const twoldg_desc* d, std::string* why);
+int batch_attach(const twoldg_desc* d, void* yn, void* ys, void* acc, void* lo,
+                 hipStream_t stream, twoldg_batch** out);
+void batch_detach(twoldg_batch* B);
+int batch_attached_steps(twoldg_batch* B, double dt, int64_t nsteps);
 }  // namespace twoldg_detail
 using twoldg_detail::fail;
 using twoldg_detail::pool_alloc;
@@ -1095,6 +1103,8 @@ struct twoldg_problem {
   hipGraph_t step_graph = nullptr;
   hipGraphExec_t step_exec = nullptr;
   double step_dt = 0;
+  // two-dimensional uniform flat problems: RK4 steps by the batch stage
+  twoldg_batch* b2 = nullptr;
   // per-launch timing of the stage kernels (bench): event pairs
   bool timing = false;
   std::vector<hipEvent_t> tev;
@@ -1684,6 +1694,7 @@ inline bool fused_supported(int D, int P1, int E) {
 template <class T, int D, int P1>
 int rk4_steps(twoldg_problem* p, double t0, double dt, int64_t nsteps) {
   if (p->fused) return fused_dispatch<T, D, P1>(p, dt, nsteps);
+  if (p->b2) return twoldg_detail::batch_attached_steps(p->b2, dt, nsteps);
   using L = Launch<T, D, P1>;
   T* yn = (T*)p->slot[TWOLDG_SLOT_STATE];
   T* ys = (T*)p->ys;

@@ -204,7 +204,8 @@ class WaveProblem:
         f = ctypes.c_int32()
         _lib.check(_lib.lib().twoldg_problem_info(self._h, ctypes.byref(n), ctypes.byref(b),
                                                   ctypes.byref(f)))
-        return {"state_len": n.value, "device_bytes": b.value, "fused": bool(f.value)}
+        return {"state_len": n.value, "device_bytes": b.value, "fused": bool(f.value),
+                "stages": {0: "generic", 1: "fused3d", 2: "batch2d"}[f.value]}
 
     # -- positions -------------------------------------------------------------
     def node_coordinates(self):
