@@ -87,6 +87,21 @@ __device__ __forceinline__ void unravel(int64_t l, int N, int I[kMaxD]) {
   if (D >= 3) I[2] = (int)(l / ((int64_t)N * N));
 }
 
+// A node index inside a block (below 2^31): 32-bit divisions.
+template <int D>
+__device__ __forceinline__ void unravel(int l, int N, int I[kMaxD]) {
+  const unsigned ul = (unsigned)l, uN = (unsigned)N;
+  I[0] = (int)(ul % uN);
+  if (D >= 2) I[1] = (int)((ul / uN) % uN);
+  if (D >= 3) I[2] = (int)(ul / (uN * uN));
+}
+
+// Block-node launches: the grid's x is the block, its y with the thread the node
+// within the block, so no thread divides a 64-bit global index.
+inline dim3 block_node_grid(const Shape& sh) {
+  return dim3((unsigned)sh.nb, (unsigned)((sh.nn + 255) / 256));
+}
+
 template <int D>
 __device__ __forceinline__ int64_t ravel(const int I[kMaxD], int N) {
   int64_t l = I[0];
@@ -247,9 +262,9 @@ __global__ void k_gradient(Shape sh, const Tables<T>* __restrict__ tbp,
                            const double* __restrict__ spacing, const T* __restrict__ y,
                            const T* __restrict__ ug, const T* __restrict__ dcoef,
                            T* __restrict__ grads) {
-  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= sh.nb * sh.nn) return;
-  const int64_t b = g / sh.nn, l = g % sh.nn;
+  const int64_t b = blockIdx.x;
+  const int l = (int)(blockIdx.y * blockDim.x + threadIdx.x);
+  if (l >= sh.nn) return;
   const Tables<T>& tb = *tbp;
   const int N = sh.N;
   int I[kMaxD] = {0, 0, 0};
@@ -258,9 +273,9 @@ __global__ void k_gradient(Shape sh, const Tables<T>* __restrict__ tbp,
   const T* u = y + b * 2 * sh.nn;
 #pragma unroll
   for (int d = 0; d < D; ++d) {
-    const int64_t stride = d == 0 ? 1 : (d == 1 ? N : (int64_t)N * N);
-    const int64_t base = l - (int64_t)I[d] * stride;
-    const int64_t tl = tlin<D>(I, d, N);
+    const int stride = d == 0 ? 1 : (d == 1 ? N : N * N);
+    const int base = l - I[d] * stride;
+    const int tl = (int)tlin<D>(I, d, N);
     auto read = [&](int o) -> T {
       if (o < 0) return ug[((b * D + d) * 2 + 0) * sh.Nt + tl];
       if (o >= N) return ug[((b * D + d) * 2 + 1) * sh.Nt + tl];
@@ -280,9 +295,9 @@ __global__ void k_gradient_cart(Shape sh, const Tables<T>* __restrict__ tbp,
                                 const T* __restrict__ ug, const T* __restrict__ coef,
                                 const uint8_t* __restrict__ flat, T* __restrict__ cart,
                                 T* __restrict__ flux) {
-  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= sh.nb * sh.nn) return;
-  const int64_t b = g / sh.nn, l = g % sh.nn;
+  const int64_t b = blockIdx.x;
+  const int l = (int)(blockIdx.y * blockDim.x + threadIdx.x);
+  if (l >= sh.nn) return;
   const Tables<T>& tb = *tbp;
   const int N = sh.N;
   int I[kMaxD] = {0, 0, 0};
@@ -292,9 +307,9 @@ __global__ void k_gradient_cart(Shape sh, const Tables<T>* __restrict__ tbp,
   T Du[kMaxD];
 #pragma unroll
   for (int d = 0; d < D; ++d) {
-    const int64_t stride = d == 0 ? 1 : (d == 1 ? N : (int64_t)N * N);
-    const int64_t base = l - (int64_t)I[d] * stride;
-    const int64_t tl = tlin<D>(I, d, N);
+    const int stride = d == 0 ? 1 : (d == 1 ? N : N * N);
+    const int base = l - I[d] * stride;
+    const int tl = (int)tlin<D>(I, d, N);
     auto read = [&](int o) -> T {
       if (o < 0) return ug[((b * D + d) * 2 + 0) * sh.Nt + tl];
       if (o >= N) return ug[((b * D + d) * 2 + 1) * sh.Nt + tl];
@@ -390,9 +405,9 @@ __global__ void k_stage_flat(Shape sh, const Tables<T>* __restrict__ tbp,
                              const double* __restrict__ spacing, const T* __restrict__ grads,
                              const T* __restrict__ gg, const T* __restrict__ dcoef,
                              StageIO<T> io) {
-  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= sh.nb * sh.nn) return;
-  const int64_t b = g / sh.nn, l = g % sh.nn;
+  const int64_t b = blockIdx.x;
+  const int l = (int)(blockIdx.y * blockDim.x + threadIdx.x);
+  if (l >= sh.nn) return;
   const Tables<T>& tb = *tbp;
   const int N = sh.N;
   int I[kMaxD] = {0, 0, 0};
@@ -400,9 +415,9 @@ __global__ void k_stage_flat(Shape sh, const Tables<T>* __restrict__ tbp,
   T acc = 0;
 #pragma unroll
   for (int d = 0; d < D; ++d) {
-    const int64_t stride = d == 0 ? 1 : (d == 1 ? N : (int64_t)N * N);
-    const int64_t base = l - (int64_t)I[d] * stride;
-    const int64_t tl = tlin<D>(I, d, N);
+    const int stride = d == 0 ? 1 : (d == 1 ? N : N * N);
+    const int base = l - I[d] * stride;
+    const int tl = (int)tlin<D>(I, d, N);
     const T* gd = grads + (b * D + d) * sh.nn;
     auto read = [&](int o) -> T {
       if (o < 0) return gg[((b * D + d) * 2 + 0) * sh.Nt + tl];
@@ -426,9 +441,9 @@ __global__ void k_stage_cart(Shape sh, const Tables<T>* __restrict__ tbp,
                              const double* __restrict__ spacing, const T* __restrict__ flux,
                              const T* __restrict__ cg, const T* __restrict__ coef,
                              const uint8_t* __restrict__ flat, StageIO<T> io) {
-  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= sh.nb * sh.nn) return;
-  const int64_t b = g / sh.nn, l = g % sh.nn;
+  const int64_t b = blockIdx.x;
+  const int l = (int)(blockIdx.y * blockDim.x + threadIdx.x);
+  if (l >= sh.nn) return;
   const Tables<T>& tb = *tbp;
   const int N = sh.N;
   int I[kMaxD] = {0, 0, 0};
@@ -439,15 +454,15 @@ __global__ void k_stage_cart(Shape sh, const Tables<T>* __restrict__ tbp,
   T acc = 0;
 #pragma unroll
   for (int d = 0; d < D; ++d) {
-    const int64_t stride = d == 0 ? 1 : (d == 1 ? N : (int64_t)N * N);
-    const int64_t base = l - (int64_t)I[d] * stride;
-    const int64_t tl = tlin<D>(I, d, N);
+    const int stride = d == 0 ? 1 : (d == 1 ? N : N * N);
+    const int base = l - I[d] * stride;
+    const int tl = (int)tlin<D>(I, d, N);
     const T Sdd = c[(1 + d * D + d) * sh.nn];
     auto read = [&](int o) -> T {
       // inside the block the flux the gradient pass formed at the node; across
       // a face the ghost's flux with the face node's frame
       if (o >= 0 && o < N) return flux[(b * D + d) * sh.nn + base + o * stride];
-      const int64_t lf = base + (int64_t)(o < 0 ? 0 : N - 1) * stride;  // the face node
+      const int lf = base + (o < 0 ? 0 : N - 1) * stride;  // the face node
       auto V = [&](int j) -> T {
         return cg[((((b * D + d) * 2 + (o < 0 ? 0 : 1)) * D) + j) * sh.Nt + tl];
       };
@@ -1407,12 +1422,12 @@ struct Launch {
     exchange_u(p, y, cdt);
     const int64_t n = p->sh.nb * p->sh.nn;
     if (!p->full) {
-      hipLaunchKernelGGL((k_gradient<T, D, P1>), dim3(grid_for(n, 256)), dim3(256), 0, p->stream,
+      hipLaunchKernelGGL((k_gradient<T, D, P1>), block_node_grid(p->sh), dim3(256), 0, p->stream,
                          p->sh, (const Tables<T>*)p->tables, p->spacing, y, (const T*)p->ug,
                          (const T*)p->dcoef, (T*)p->grads);
       ghosts<1>(p, (const T*)p->cval_g, (const T*)p->grads, cdt, (T*)p->gg);
     } else {
-      hipLaunchKernelGGL((k_gradient_cart<T, D, P1>), dim3(grid_for(n, 256)), dim3(256), 0,
+      hipLaunchKernelGGL((k_gradient_cart<T, D, P1>), block_node_grid(p->sh), dim3(256), 0,
                          p->stream, p->sh, (const Tables<T>*)p->tables, p->spacing, y,
                          (const T*)p->ug, (const T*)p->coef, p->flat, (T*)p->grads,
                          (T*)p->flux);
@@ -1439,11 +1454,11 @@ struct Launch {
   template <int MODE>
   static void stage_launch(twoldg_problem* p, const StageIO<T>& io, int64_t n) {
     if (!p->full)
-      hipLaunchKernelGGL((k_stage_flat<T, D, P1, MODE>), dim3(grid_for(n, 256)), dim3(256), 0,
+      hipLaunchKernelGGL((k_stage_flat<T, D, P1, MODE>), block_node_grid(p->sh), dim3(256), 0,
                          p->stream, p->sh, (const Tables<T>*)p->tables, p->spacing,
                          (const T*)p->grads, (const T*)p->gg, (const T*)p->dcoef, io);
     else
-      hipLaunchKernelGGL((k_stage_cart<T, D, P1, MODE>), dim3(grid_for(n, 256)), dim3(256), 0,
+      hipLaunchKernelGGL((k_stage_cart<T, D, P1, MODE>), block_node_grid(p->sh), dim3(256), 0,
                          p->stream, p->sh, (const Tables<T>*)p->tables, p->spacing,
                          (const T*)p->flux, (const T*)p->gg, (const T*)p->coef, p->flat, io);
   }
