@@ -28,15 +28,17 @@ sys.path.insert(0, os.path.join(ROOT, "treewaveoldg.jl_amd"))
 
 PEAK_HBM_GBS = 8000.0
 # algorithmic values moved per node-update by the fused three-register RK4:
-# stage 1 reads y_n (2), writes acc (2) and y_2 (2); stages 2 and 3 read y_s,
-# y_n, acc (6) and write acc, y_next (4); stage 4 reads y_4, y_n, acc (6) and
-# writes y_{n+1} (2): 34 values.  The compensated form adds the low words of
+# stage 1 reads y_n (2), writes the accumulator's v (1) and y_2 (2); stage 2
+# reads y_2, y_n (4) and the accumulator's v (1), writes acc, y_3 (4) (its
+# accumulator u, v_n + 2 k2_u, from y_n's v); stage 3 reads y_3, y_n, acc (6)
+# and writes acc, y_4 (4); stage 4 reads y_4, y_n, acc (6) and writes y_{n+1}
+# (2): 32 values, u's 15 and v's 17.  The compensated form adds the low words of
 # u and v, read and written at stage 4 (4 more values).
-VALUES_PER_NODE_UPDATE = {"f64": 34, "f32c": 38}
+VALUES_PER_NODE_UPDATE = {"f64": 32, "f32c": 36}
 BYTES_PER_VALUE = {"f64": 8, "f32c": 4}
-# The split form (u in FP64, v in FP32) moves the same 34 values, u's 17 in
+# The split form (u in FP64, v in FP32) moves the same 32 values, u's 15 in
 # FP64 and v's 17 in FP32, plus the low word of v read and written at stage 4.
-BYTES_PER_NODE_UPDATE = {"f64": 272, "f32c": 152, "split": 17 * 8 + 17 * 4 + 2 * 4}
+BYTES_PER_NODE_UPDATE = {"f64": 256, "f32c": 144, "split": 15 * 8 + 17 * 4 + 2 * 4}
 
 
 # HBM bytes per launch of the stage kernel on the default workload, measured with

@@ -148,7 +148,7 @@ void enqueue_step(twoldg_batch* B) {
   T* yn = (T*)B->yn;
   T* ys = (T*)B->ys;
   launch_stage<T, 1>(B, B->runs, B->nwg, yn, ys, 0.5, 0);
-  launch_stage<T, 2>(B, B->runs, B->nwg, ys, ys, 0.5, 1);
+  launch_stage<T, 6>(B, B->runs, B->nwg, ys, ys, 0.5, 1);
   launch_stage<T, 2>(B, B->runs, B->nwg, ys, ys, 1.0, 0);
   if (B->precision == TWOLDG_F32C)
     launch_stage<T, 4>(B, B->runs, B->nwg, ys, yn, 1.0 / 6.0, 1);

@@ -220,8 +220,8 @@ __device__ void batch_lines(const BatchArgs<T>& A, const BatchItem& it, T* lds) 
         lu[j] = lo[o];
         lv[j] = lo[o + NT];
       }
-      if (MODE >= 2) {
-        au[j] = acc[o];
+      if (MODE >= 2) {  // stage 2 (MODE 6): the accumulator's u is y_n's v
+        au[j] = MODE == 6 ? yv[j] : acc[o];
         av[j] = acc[o + NT];
       }
     }
@@ -281,12 +281,12 @@ __device__ void batch_lines(const BatchArgs<T>& A, const BatchItem& it, T* lds) 
       const unsigned o = ob + l;
       const T kv = ((T(0) + L[l]) + ly[j]) * rh;
       T unext;
-      if (MODE == 1) {  // stage 1: the stage input is the step start
-        acc[o] = ku[j];
+      if (MODE == 1) {  // stage 1: the stage input is the step start; the
+        // accumulator's u would be k1's u = y_n's v, which stage 2 reads itself
         acc[o + NT] = kv;
         unext = uy[j] + ak * ku[j];
         w[j] = ku[j] + ak * kv;
-      } else if (MODE == 2) {
+      } else if (MODE == 2 || MODE == 6) {
         acc[o] = au[j] + T(2) * ku[j];
         acc[o + NT] = av[j] + T(2) * kv;
         unext = yu[j] + ak * ku[j];

@@ -14,7 +14,7 @@
 //     term).  A neighbour's face gradient is that part plus the ghost term
 //     formed with our own face value.
 // So one stage reads u, v of the stage state, y_n and the accumulator once and
-// writes the accumulator and the next state once (34 values per node and step
+// writes the accumulator and the next state once (32 values per node and step
 // in FP64), plus faces, O(1/N) of that.
 
 #pragma once
@@ -136,10 +136,14 @@ constexpr FusedTab<T, P1> fused_tab() {
 // Face ghosts are formed here from a same-level neighbour's face buffers (nbr
 // >= 0) or from the own face on a mirror (-1); with GHOST, a coarse-fine face
 // (-2) takes them complete from k_face_ghost_u/_g.
-// MODE 1: stage 1 (ys = yn); 2: stages 2 and 3; 3: the final stage; 4: the
+// MODE 1: stage 1 (ys = yn); 6: stage 2; 2: stage 3; 3: the final stage; 4: the
 // final stage of the compensated form (u by a Kahan sum into (yn, lo), v's
 // truncated increment likewise into (vn, vlo)); 5: the final stage of the split
 // form (u plain in FP64, v's increment added to the FP32 pair (vn, vlo) in FP64).
+// The accumulator's u after stage 1 would be k1's u, which is y_n's v: stage 1
+// stores only the accumulator's v, and stage 2 forms acc_u = v_n + 2 k2_u from
+// the y_n it reads anyway (the same values bit for bit: two values fewer per
+// node and step).
 template <class T, int P1, int E, int MODE, bool GHOST, class TV = T>
 __global__ void __launch_bounds__(((P1 * E) * (P1 * E) + 63) / 64 * 64, (fused_waves_per_eu<T, P1, E>()))
 k_fused3(FusedArgs<T, TV> A) {
@@ -470,7 +474,7 @@ k_fused3(FusedArgs<T, TV> A) {
           // compensated form: u's low word; split form: unused
           yv = MODE == 4 ? ld_once(lo + iu) : (MODE == 5 ? T(0) : (T)ld_once(vn + iu));
         }
-        au = MODE >= 2 ? ld_once(acc + iu) : T(0);
+        au = MODE == 6 ? yv : (MODE >= 2 ? ld_once(acc + iu) : T(0));
         av = MODE >= 2 ? (T)ld_once(vacc + iu) : T(0);
       };
       // the stage operands one node ahead where registers allow
@@ -494,7 +498,7 @@ k_fused3(FusedArgs<T, TV> A) {
             LX[l] = ou;
             if (with_v) LZ[l] = ov;
           } else {
-            st_once(pu + iu, ou);
+            if (MODE != 1) st_once(pu + iu, ou);
             if (with_v) st_once(vacc + iu, (TV)ov);
           }
         };
@@ -502,7 +506,7 @@ k_fused3(FusedArgs<T, TV> A) {
           out(acc, true, ku, kv);
           unext = yu + A.a * ku;
           w[j] = yv + A.a * kv;
-        } else if (MODE == 2) {
+        } else if (MODE == 2 || MODE == 6) {
           out(acc, true, au + T(2) * ku, av + T(2) * kv);
           unext = yu + A.a * ku;
           w[j] = yv + A.a * kv;
@@ -612,10 +616,8 @@ k_fused3(FusedArgs<T, TV> A) {
         if (defer) {  // the stage update's outputs of the column
           const int li = j * NT + col;
           st_once(yo + iu, UN[li]);
-          if (MODE == 1 || MODE == 2) {
-            st_once(acc + iu, LX[li]);
-            st_once(vacc + iu, (TV)LZ[li]);
-          }
+          if (MODE == 2 || MODE == 6) st_once(acc + iu, LX[li]);
+          if (MODE == 1 || MODE == 2 || MODE == 6) st_once(vacc + iu, (TV)LZ[li]);
           if (MODE == 4) st_once(lo + iu, LX[li]);
         }
         if (MODE == 4) {

@@ -1623,7 +1623,7 @@ int fused_run(twoldg_problem* p, double dt, int64_t nsteps) {
     io(1);
     tmark(p);
     ghosts(1);
-    hipLaunchKernelGGL((k_fused3<T, P1, E, 2, GHOST, TV>), dim3((unsigned)nb), dim3(threads), 0,
+    hipLaunchKernelGGL((k_fused3<T, P1, E, 6, GHOST, TV>), dim3((unsigned)nb), dim3(threads), 0,
                        p->stream, A);
     tmark(p);
     A.a = (T)dt;
