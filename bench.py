@@ -32,13 +32,15 @@ PEAK_HBM_GBS = 8000.0
 # reads y_2, y_n (4) and the accumulator's v (1), writes acc, y_3 (4) (its
 # accumulator u, v_n + 2 k2_u, from y_n's v); stage 3 reads y_3, y_n, acc (6)
 # and writes acc, y_4 (4); stage 4 reads y_4, y_n, acc (6) and writes y_{n+1}
-# (2): 32 values, u's 15 and v's 17.  The compensated form adds the low words of
-# u and v, read and written at stage 4 (4 more values).
-VALUES_PER_NODE_UPDATE = {"f64": 32, "f32c": 36}
+# (2): 32 values, u's 15 and v's 17.  FP64 stores z = y_n + dt/6 (acc + 2 k3) in
+# place of the accumulator at stage 3, so that stage 4 reads y_4 and z (4) and
+# not y_n: 30 values.  The compensated form keeps y_n at stage 4 and adds the
+# low words of u and v, read and written there (4 more values: 36).
+VALUES_PER_NODE_UPDATE = {"f64": 30, "f32c": 36}
 BYTES_PER_VALUE = {"f64": 8, "f32c": 4}
-# The split form (u in FP64, v in FP32) moves the same 32 values, u's 15 in
-# FP64 and v's 17 in FP32, plus the low word of v read and written at stage 4.
-BYTES_PER_NODE_UPDATE = {"f64": 256, "f32c": 144, "split": 15 * 8 + 17 * 4 + 2 * 4}
+# The split form (u in FP64, v in FP32) moves 32 values, u's 15 in FP64 and
+# v's 17 in FP32, plus the low word of v read and written at stage 4.
+BYTES_PER_NODE_UPDATE = {"f64": 240, "f32c": 144, "split": 15 * 8 + 17 * 4 + 2 * 4}
 
 
 # HBM bytes per launch of the stage kernel on the default workload, measured with

@@ -113,7 +113,7 @@ int64_t make_runs(const twoldg_batch* B, const std::vector<int32_t>& Nh, const i
 
 template <class T, int MODE>
 void launch_stage(twoldg_batch* B, const std::vector<BatchRun>& runs, int64_t nwg, const T* ys,
-                  T* yo, double coef, int in) {
+                  T* yo, double coef, int in, double coef2 = 0) {
   BatchArgs<T> A{};
   for (int r = 0; r < kBatchMaxRuns; ++r) {
     if (r < (int)runs.size()) {
@@ -138,6 +138,7 @@ void launch_stage(twoldg_batch* B, const std::vector<BatchRun>& runs, int64_t nw
   A.uf_out = (T*)B->uf[1 - in];
   A.pf_out = (T*)B->pf[1 - in];
   A.coef = coef;
+  A.coef2 = coef2;
   A.truncate = B->truncate ? 1 : 0;
   hipLaunchKernelGGL((k_batch2<T, MODE>), dim3((unsigned)nwg), dim3(batch_threads<T>()), 0,
                      B->stream, A);
@@ -149,11 +150,13 @@ void enqueue_step(twoldg_batch* B) {
   T* ys = (T*)B->ys;
   launch_stage<T, 1>(B, B->runs, B->nwg, yn, ys, 0.5, 0);
   launch_stage<T, 6>(B, B->runs, B->nwg, ys, ys, 0.5, 1);
-  launch_stage<T, 2>(B, B->runs, B->nwg, ys, ys, 1.0, 0);
-  if (B->precision == TWOLDG_F32C)
+  if (B->precision == TWOLDG_F32C) {
+    launch_stage<T, 2>(B, B->runs, B->nwg, ys, ys, 1.0, 0);
     launch_stage<T, 4>(B, B->runs, B->nwg, ys, yn, 1.0 / 6.0, 1);
-  else
-    launch_stage<T, 3>(B, B->runs, B->nwg, ys, yn, 1.0 / 6.0, 1);
+  } else {  // FP64: stage 3 leaves z = y_n + dt/6 (acc + 2 k3), the last stage adds dt/6 k4
+    launch_stage<T, 7>(B, B->runs, B->nwg, ys, ys, 1.0, 0, 1.0 / 6.0);
+    launch_stage<T, 8>(B, B->runs, B->nwg, ys, yn, 1.0 / 6.0, 1);
+  }
 }
 
 // Time the middle stage for each order and candidate tile; keep the fastest.

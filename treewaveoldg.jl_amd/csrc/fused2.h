@@ -101,6 +101,7 @@ struct BatchArgs {
   T* uf_out;               // the next stage's ghosts (the neighbours' slots)
   T* pf_out;
   double coef;             // stage coefficient in units of dt: 1/2, 1 or 1/6
+  double coef2;            // mode 7: the final combination's, 1/6
   int truncate;
 };
 
@@ -198,7 +199,7 @@ __device__ void batch_lines(const BatchArgs<T>& A, const BatchItem& it, T* lds) 
   // (1) every global operand: the y line's u, v, the step start, the
   // accumulator (and compensation words), the block's spacing and time step,
   // and the ghosts of the faces the two lines end on
-  T rh = 0, ak = 0;
+  T rh = 0, ak = 0, ak2 = 0;
   T uy[P1], ku[P1], yu[P1], yv[P1], au[P1], av[P1], lu[P1], lv[P1];
   T gyu = 0, gyp = 0, gyr = 0, gxu = 0, gxp = 0, gxr = 0;
   int32_t nby = -1, nbx = -1;  // where the y (x) line's face goes, -1 a mirror
@@ -207,12 +208,13 @@ __device__ void batch_lines(const BatchArgs<T>& A, const BatchItem& it, T* lds) 
   if (act) {
     rh = T(1) / (T)A.spacing[it.first + b0];
     ak = (T)(A.coef * A.dt[it.first + b0]);
+    ak2 = (T)(A.coef2 * A.dt[it.first + b0]);
 #pragma unroll
     for (int j = 0; j < P1; ++j) {
       const unsigned o = ob + node(j);
       uy[j] = ys[o];
       ku[j] = ys[o + NT];
-      if (MODE != 1) {  // the compensated form: the high words
+      if (MODE != 1 && MODE != 8) {  // the compensated form: the high words
         yu[j] = yn[o];
         yv[j] = yn[o + NT];
       }
@@ -294,6 +296,14 @@ __device__ void batch_lines(const BatchArgs<T>& A, const BatchItem& it, T* lds) 
       } else if (MODE == 3) {
         unext = yu[j] + ak * (au[j] + ku[j]);
         w[j] = yv[j] + ak * (av[j] + kv);
+      } else if (MODE == 7) {  // stage 3, and z = y_n + dt/6 (acc + 2 k3) for the last
+        acc[o] = yu[j] + ak2 * (au[j] + T(2) * ku[j]);
+        acc[o + NT] = yv[j] + ak2 * (av[j] + T(2) * kv);
+        unext = yu[j] + ak * ku[j];
+        w[j] = yv[j] + ak * kv;
+      } else if (MODE == 8) {  // the final stage from z (no y_n)
+        unext = au[j] + ak * ku[j];
+        w[j] = av[j] + ak * kv;
       } else {  // compensated: u by Kahan now, v's increment truncated first
         const T du = ak * (au[j] + ku[j]);
         const T yk = du - lu[j], tu = yu[j] + yk;

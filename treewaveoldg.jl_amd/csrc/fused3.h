@@ -14,7 +14,7 @@
 //     term).  A neighbour's face gradient is that part plus the ghost term
 //     formed with our own face value.
 // So one stage reads u, v of the stage state, y_n and the accumulator once and
-// writes the accumulator and the next state once (32 values per node and step
+// writes the accumulator and the next state once (30 values per node and step
 // in FP64), plus faces, O(1/N) of that.
 
 #pragma once
@@ -96,15 +96,17 @@ struct FusedArgs {
   TV* vlo;           // compensation of yn's v (final stage, compensated and split forms)
   int64_t vstride;
   const T* uf_in;    // faces of ys: (nb, 6, N^2) u on the face plane
-  const T* pf_in;    //              block-local part of the face gradient
+  const T* pf_in;    //              block-local part of the face gradient (= uf_in + nfb)
   T* uf_out;         // faces of yo
   T* pf_out;
   const int32_t* nbr;       // (nb, 6) same-level neighbour, -1 a mirror, -2 coarse-fine
   const T* gu;              // complete ghosts of the coarse-fine faces (u, gradient),
-  const T* gg;              //   (nb, 6, N^2), written by k_face_ghost_u/_g
+  const T* gg;              //   (nb, 6, N^2) each, gg = gu + nfb
+  int64_t nfb;              // nb 6 N^2: pf_in - uf_in, gg - gu
   const double* spacing;    // (nb)
   const Tables<T>* tables;
   T a;               // stage coefficient (dt/2, dt or dt/6)
+  T a2;              // mode 7: the final combination's dt/6
   int truncate;
 };
 
@@ -143,7 +145,9 @@ constexpr FusedTab<T, P1> fused_tab() {
 // The accumulator's u after stage 1 would be k1's u, which is y_n's v: stage 1
 // stores only the accumulator's v, and stage 2 forms acc_u = v_n + 2 k2_u from
 // the y_n it reads anyway (the same values bit for bit: two values fewer per
-// node and step).
+// node and step).  FP64: 7, stage 3, stores z = y_n + dt/6 (acc + 2 k3) in the
+// accumulator's place, and 8, the final stage, forms y_n+1 = z + dt/6 k4 from
+// it without reading y_n (two values fewer again).
 template <class T, int P1, int E, int MODE, bool GHOST, class TV = T>
 __global__ void __launch_bounds__(((P1 * E) * (P1 * E) + 63) / 64 * 64, (fused_waves_per_eu<T, P1, E>()))
 k_fused3(FusedArgs<T, TV> A) {
@@ -291,9 +295,9 @@ k_fused3(FusedArgs<T, TV> A) {
                   t = r % (P1 * N);  // t = (plane in layer) * N + tangential index
         const int nbq = f == 0 ? nxl : (f == 1 ? nxh : (f == 2 ? nyl : nyh));
         if (nbq >= 0)
-          gv[k] = (q ? A.pf_in : A.uf_in)[((int64_t)nbq * 6 + (f ^ 1)) * N2 + e * P1 * N + t];
+          gv[k] = A.uf_in[q * A.nfb + ((int64_t)nbq * 6 + (f ^ 1)) * N2 + e * P1 * N + t];
         else if (GHOST && nbq == CF)
-          gv[k] = (q ? A.gg : A.gu)[((int64_t)b * 6 + f) * N2 + e * P1 * N + t];
+          gv[k] = A.gu[q * A.nfb + ((int64_t)b * 6 + f) * N2 + e * P1 * N + t];
       }
     }
   };
@@ -469,6 +473,8 @@ k_fused3(FusedArgs<T, TV> A) {
           // the layer's plane in LDS (this thread's node, not yet overwritten)
           yu = U[py * NT + (ey * P1 + j) * N + xy];
           yv = ku;
+        } else if (MODE == 8) {  // the final stage from z: no y_n
+          yu = yv = T(0);
         } else {
           yu = ld_once(un_ + iu);  // compensated form: the high word
           // compensated form: u's low word; split form: unused
@@ -513,6 +519,13 @@ k_fused3(FusedArgs<T, TV> A) {
         } else if (MODE == 3) {
           unext = yu + A.a * (au + ku);
           w[j] = yv + A.a * (av + kv);
+        } else if (MODE == 7) {  // stage 3, and z for the final stage
+          out(acc, true, yu + A.a2 * (au + T(2) * ku), yv + A.a2 * (av + T(2) * kv));
+          unext = yu + A.a * ku;
+          w[j] = yv + A.a * kv;
+        } else if (MODE == 8) {  // the final stage from z (in au, av)
+          unext = au + A.a * ku;
+          w[j] = av + A.a * kv;
         } else if (MODE == 4) {  // compensated: u by Kahan now, v's increment truncated first
           const T du = A.a * (au + ku);
           const T yk = du - yv, tu = yu + yk;
@@ -616,8 +629,8 @@ k_fused3(FusedArgs<T, TV> A) {
         if (defer) {  // the stage update's outputs of the column
           const int li = j * NT + col;
           st_once(yo + iu, UN[li]);
-          if (MODE == 2 || MODE == 6) st_once(acc + iu, LX[li]);
-          if (MODE == 1 || MODE == 2 || MODE == 6) st_once(vacc + iu, (TV)LZ[li]);
+          if (MODE == 2 || MODE == 6 || MODE == 7) st_once(acc + iu, LX[li]);
+          if (MODE == 1 || MODE == 2 || MODE == 6 || MODE == 7) st_once(vacc + iu, (TV)LZ[li]);
           if (MODE == 4) st_once(lo + iu, LX[li]);
         }
         if (MODE == 4) {
@@ -679,7 +692,15 @@ __global__ void __launch_bounds__(256) k_cf_ghosts(const CFJob* __restrict__ job
   __shared__ T sA[N * H], sB[H * N];  // first-axis passes: I along t0 (N x H), R along t0 (H x N)
   __shared__ T sGF[N2], sGC[H2];      // complete face gradients
   const int tid = threadIdx.x;
-  const CFJob J = jobs[blockIdx.x];
+  // the job's fields and the spacings with the device-level loads of the other
+  // per-block tables (ld_table): written at creation into recycled memory
+  const int32_t* jp = (const int32_t*)(jobs + blockIdx.x);
+  CFJob J;
+  J.bf = ld_table(jp + 0);
+  J.ff = ld_table(jp + 1);
+  J.bc = ld_table(jp + 2);
+  J.fc = ld_table(jp + 3);
+  J.q = ld_table(jp + 4);
   const twoldg::ElementTables& et = twoldg::kElementTables[P1 - 2];
   for (int k = tid; k < 2 * P1 * P1; k += 256) {
     sI[k] = (T)et.interp[k / P1][k % P1];
@@ -696,7 +717,7 @@ __global__ void __launch_bounds__(256) k_cf_ghosts(const CFJob* __restrict__ job
     sUC[k] = uf[c];
     sPC[k] = pf[c];
   }
-  const T rhf = T(1) / (T)spacing[J.bf], rhc = T(1) / (T)spacing[J.bc];
+  const T rhf = T(1) / (T)ld_table(spacing + J.bf), rhc = T(1) / (T)ld_table(spacing + J.bc);
   // the ghost term of each side's own face row (row 0 on a low face, P on a high one)
   const T wf = (J.ff % 2) ? tb.W[P][P1 + 1] : tb.W[0][0];
   const T wc = (J.fc % 2) ? tb.W[P][P1 + 1] : tb.W[0][0];

@@ -171,8 +171,8 @@ void destroy(twoldg_problem* p) {
                   (void*)p->part, (void*)p->red, (void*)p->clock, p->dcoef, p->lmode, p->ldual,
                   (void*)p->fslot, (void*)p->fradial, (void*)p->fblk, (void*)p->oblk, p->frmodes, p->frduals, (void*)p->faslot, p->famodes, p->faweights, p->faginv,
                   (void*)p->rad_rows, (void*)p->rpart, (void*)p->svn, (void*)p->svlo,
-                  (void*)p->svs, (void*)p->svacc, p->flux, (void*)p->nbr6, p->uf[0], p->uf[1], p->pf[0],
-                  p->pf[1], (void*)p->cfjobs, p->fgu, p->fgg})
+                  (void*)p->svs, (void*)p->svacc, p->flux, (void*)p->nbr6, p->uf[0], p->uf[1],
+                  (void*)p->cfjobs, p->fgu})
     dfree(b);
   if (p->step_exec) (void)hipGraphExecDestroy(p->step_exec);
   if (p->step_graph) (void)hipGraphDestroy(p->step_graph);
@@ -460,8 +460,10 @@ int create_buffers(twoldg_problem* p, const twoldg_desc* d, const Csr& csr) {
     if ((rc = dalloc_bytes(p, (void**)&p->cfjobs, jobs.size() * sizeof(CFJob)))) return rc;
     if ((rc = h2d(p, p->cfjobs, jobs.data(), jobs.size() * sizeof(CFJob)))) return rc;
     const size_t nf = (size_t)p->sh.nb * 6 * N2;
-    if ((rc = dalloc_bytes(p, &p->fgu, nf * sizeof(T)))) return rc;
-    if ((rc = dalloc_bytes(p, &p->fgg, nf * sizeof(T)))) return rc;
+    // the gradient ghosts follow the u ghosts in one allocation (the stage
+    // kernel picks either by offset: fgg = fgu + nb 6 N^2)
+    if ((rc = dalloc_bytes(p, &p->fgu, 2 * nf * sizeof(T)))) return rc;
+    p->fgg = (T*)p->fgu + nf;
     p->ncf = (int64_t)jobs.size();
     p->ghosted = true;
   }
@@ -477,8 +479,10 @@ int create_buffers(twoldg_problem* p, const twoldg_desc* d, const Csr& csr) {
     if ((rc = h2d(p, p->nbr6, nbr.data(), nbr.size() * 4))) return rc;
     const size_t nf = (size_t)p->sh.nb * 6 * p->sh.Nt;
     for (int k = 0; k < 2; ++k) {
-      if ((rc = dalloc_bytes(p, &p->uf[k], nf * sizeof(T)))) return rc;
-      if ((rc = dalloc_bytes(p, &p->pf[k], nf * sizeof(T)))) return rc;
+      // pf follows uf in one allocation (the stage kernel picks either by
+      // offset: pf = uf + nb 6 N^2)
+      if ((rc = dalloc_bytes(p, &p->uf[k], 2 * nf * sizeof(T)))) return rc;
+      p->pf[k] = (T*)p->uf[k] + nf;
     }
     p->fused = true;
   }

@@ -1574,6 +1574,7 @@ int fused_run(twoldg_problem* p, double dt, int64_t nsteps) {
   A.truncate = p->truncate ? 1 : 0;
   A.gu = (const T*)p->fgu;
   A.gg = (const T*)p->fgg;
+  A.nfb = nb * 6 * (int64_t)N * N;
   A.vstride = split ? nn : 2 * nn;
   // v of a stage: the v halves of the state arrays, or the split form's own
   auto vptrs = [&](int stage) {
@@ -1627,11 +1628,17 @@ int fused_run(twoldg_problem* p, double dt, int64_t nsteps) {
                        p->stream, A);
     tmark(p);
     A.a = (T)dt;
+    A.a2 = (T)(dt / 6);
     io(0);
     tmark(p);
     ghosts(0);
-    hipLaunchKernelGGL((k_fused3<T, P1, E, 2, GHOST, TV>), dim3((unsigned)nb), dim3(threads), 0,
-                       p->stream, A);
+    // FP64: stage 3 leaves z = y_n + dt/6 (acc + 2 k3) for the final stage
+    if (!split && p->precision == TWOLDG_F64)
+      hipLaunchKernelGGL((k_fused3<T, P1, E, 7, GHOST, TV>), dim3((unsigned)nb), dim3(threads), 0,
+                         p->stream, A);
+    else
+      hipLaunchKernelGGL((k_fused3<T, P1, E, 2, GHOST, TV>), dim3((unsigned)nb), dim3(threads), 0,
+                         p->stream, A);
     tmark(p);
     A.a = (T)(dt / 6);
     A.yo = yn;
@@ -1646,7 +1653,7 @@ int fused_run(twoldg_problem* p, double dt, int64_t nsteps) {
       hipLaunchKernelGGL((k_fused3<T, P1, E, 4, GHOST, TV>), dim3((unsigned)nb), dim3(threads), 0,
                          p->stream, A);
     else
-      hipLaunchKernelGGL((k_fused3<T, P1, E, 3, GHOST, TV>), dim3((unsigned)nb), dim3(threads), 0,
+      hipLaunchKernelGGL((k_fused3<T, P1, E, 8, GHOST, TV>), dim3((unsigned)nb), dim3(threads), 0,
                          p->stream, A);
     tmark(p);
   };
