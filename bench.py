@@ -38,9 +38,10 @@ PEAK_HBM_GBS = 8000.0
 # low words of u and v, read and written there (4 more values: 36).
 VALUES_PER_NODE_UPDATE = {"f64": 30, "f32c": 36}
 BYTES_PER_VALUE = {"f64": 8, "f32c": 4}
-# The split form (u in FP64, v in FP32) moves 32 values, u's 15 in FP64 and
-# v's 17 in FP32, plus the low word of v read and written at stage 4.
-BYTES_PER_NODE_UPDATE = {"f64": 240, "f32c": 144, "split": 15 * 8 + 17 * 4 + 2 * 4}
+# The split form (u in FP64, v in FP32) moves u's 14 values in FP64 (u through z
+# as in FP64) and v's 17 in FP32, plus the low word of v read and written at
+# stage 4.
+BYTES_PER_NODE_UPDATE = {"f64": 240, "f32c": 144, "split": 14 * 8 + 17 * 4 + 2 * 4}
 
 
 # HBM bytes per launch of the stage kernel on the default workload, measured with

@@ -147,7 +147,9 @@ constexpr FusedTab<T, P1> fused_tab() {
 // the y_n it reads anyway (the same values bit for bit: two values fewer per
 // node and step).  FP64: 7, stage 3, stores z = y_n + dt/6 (acc + 2 k3) in the
 // accumulator's place, and 8, the final stage, forms y_n+1 = z + dt/6 k4 from
-// it without reading y_n (two values fewer again).
+// it without reading y_n (two values fewer again).  The split form does the same
+// for u alone: 9, stage 3 (z for u, the accumulator for v), and 10, the final
+// stage (u from z, v's increment into its FP32 pair as in 5).
 template <class T, int P1, int E, int MODE, bool GHOST, class TV = T>
 __global__ void __launch_bounds__(((P1 * E) * (P1 * E) + 63) / 64 * 64, (fused_waves_per_eu<T, P1, E>()))
 k_fused3(FusedArgs<T, TV> A) {
@@ -473,7 +475,7 @@ k_fused3(FusedArgs<T, TV> A) {
           // the layer's plane in LDS (this thread's node, not yet overwritten)
           yu = U[py * NT + (ey * P1 + j) * N + xy];
           yv = ku;
-        } else if (MODE == 8) {  // the final stage from z: no y_n
+        } else if (MODE == 8 || MODE == 10) {  // the final stage from z: no y_n (u)
           yu = yv = T(0);
         } else {
           yu = ld_once(un_ + iu);  // compensated form: the high word
@@ -526,6 +528,13 @@ k_fused3(FusedArgs<T, TV> A) {
         } else if (MODE == 8) {  // the final stage from z (in au, av)
           unext = au + A.a * ku;
           w[j] = av + A.a * kv;
+        } else if (MODE == 9) {  // split, stage 3: z for u, the accumulator for v
+          out(acc, true, yu + A.a2 * (au + T(2) * ku), av + T(2) * kv);
+          unext = yu + A.a * ku;
+          w[j] = yv + A.a * kv;
+        } else if (MODE == 10) {  // split, the final stage: u from z, v's increment
+          unext = au + A.a * ku;
+          w[j] = A.a * (av + kv);
         } else if (MODE == 4) {  // compensated: u by Kahan now, v's increment truncated first
           const T du = A.a * (au + ku);
           const T yk = du - yv, tu = yu + yk;
@@ -563,7 +572,7 @@ k_fused3(FusedArgs<T, TV> A) {
     // ---- (4) x lines: v's truncation along x, faces of the next u along x;
     // columns: faces of the next u along z ----
     TV lov[P1], hv[P1];  // the final stage's v pair, ahead of this phase's stores
-    if (act && (MODE == 4 || MODE == 5)) {
+    if (act && (MODE == 4 || MODE == 5 || MODE == 10)) {
 #pragma unroll
       for (int j = 0; j < P1; ++j) {
         const int iv = col + (e * P1 + j) * N2;
@@ -629,15 +638,16 @@ k_fused3(FusedArgs<T, TV> A) {
         if (defer) {  // the stage update's outputs of the column
           const int li = j * NT + col;
           st_once(yo + iu, UN[li]);
-          if (MODE == 2 || MODE == 6 || MODE == 7) st_once(acc + iu, LX[li]);
-          if (MODE == 1 || MODE == 2 || MODE == 6 || MODE == 7) st_once(vacc + iu, (TV)LZ[li]);
+          if (MODE == 2 || MODE == 6 || MODE == 7 || MODE == 9) st_once(acc + iu, LX[li]);
+          if (MODE == 1 || MODE == 2 || MODE == 6 || MODE == 7 || MODE == 9)
+            st_once(vacc + iu, (TV)LZ[li]);
           if (MODE == 4) st_once(lo + iu, LX[li]);
         }
         if (MODE == 4) {
           const T yk = wl[j] - (T)lov[j], tv = (T)hv[j] + yk;
           st_once(vlo + iu, (TV)((tv - (T)hv[j]) - yk));
           st_once(vo + iu, (TV)tv);
-        } else if (MODE == 5) {
+        } else if (MODE == 5 || MODE == 10) {
           // v_n = hi - lo exactly in T; the new pair rounds v_n + increment
           const T vt = ((T)hv[j] - (T)lov[j]) + wl[j];
           const TV hi = (TV)vt;

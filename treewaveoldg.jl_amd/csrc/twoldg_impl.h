@@ -1632,8 +1632,12 @@ int fused_run(twoldg_problem* p, double dt, int64_t nsteps) {
     io(0);
     tmark(p);
     ghosts(0);
-    // FP64: stage 3 leaves z = y_n + dt/6 (acc + 2 k3) for the final stage
-    if (!split && p->precision == TWOLDG_F64)
+    // FP64: stage 3 leaves z = y_n + dt/6 (acc + 2 k3) for the final stage; the
+    // split form does so for u
+    if constexpr (split)
+      hipLaunchKernelGGL((k_fused3<T, P1, E, 9, GHOST, TV>), dim3((unsigned)nb), dim3(threads), 0,
+                         p->stream, A);
+    else if (p->precision == TWOLDG_F64)
       hipLaunchKernelGGL((k_fused3<T, P1, E, 7, GHOST, TV>), dim3((unsigned)nb), dim3(threads), 0,
                          p->stream, A);
     else
@@ -1647,7 +1651,7 @@ int fused_run(twoldg_problem* p, double dt, int64_t nsteps) {
     tmark(p);
     ghosts(1);
     if constexpr (split)
-      hipLaunchKernelGGL((k_fused3<T, P1, E, 5, GHOST, TV>), dim3((unsigned)nb), dim3(threads), 0,
+      hipLaunchKernelGGL((k_fused3<T, P1, E, 10, GHOST, TV>), dim3((unsigned)nb), dim3(threads), 0,
                          p->stream, A);
     else if (p->precision == TWOLDG_F32C)
       hipLaunchKernelGGL((k_fused3<T, P1, E, 4, GHOST, TV>), dim3((unsigned)nb), dim3(threads), 0,
