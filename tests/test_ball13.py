@@ -325,6 +325,28 @@ def test_gpu_thirteen_rhs_truncate_energy_match_oracle(P, rough):
 
 
 @gpu
+def test_gpu_thirteen_e4_radiative_matches_oracle(rough):
+    """The benchmark ball's element layout (P = 3, E = 4: 16^3 blocks, factored
+    truncation on the outer shells) under the radiative sphere with u/R: the
+    right-hand side, the truncation and the energy against the oracle."""
+    P, E, roots = 3, 4, (2, 1, 1)
+    forest, geo = T.thirteenpatch_forest(P, E, roots=roots, Rm=2.0)
+    p = T.WaveProblem(forest, T.Element(P), geometry=geo, boundary=T.RadiativeBoundary())
+    o = O.Problem.ball_ext(P, E, roots=roots, Rm=2.0, R1=3.0, radiative=True, kappa=1.0,
+                           invR=1 / 3)
+    y = rough(p.state_len, 8)
+    yt = T.truncate_state(y.copy(), p)
+    want = o.truncate(y)
+    assert np.abs(yt - want).max() < 1e-13 * np.abs(want).max()
+    du = np.empty_like(yt)
+    T.wave_rhs(du, yt, p, 0.0)
+    want = o.rhs(yt)
+    assert np.abs(du - want).max() < 1e-12 * np.abs(want).max()
+    assert math.isclose(T.wave_energy(p, yt), o.energy(yt), rel_tol=1e-13)
+    p.close()
+
+
+@gpu
 @pytest.mark.parametrize("P", [2, 3])
 def test_gpu_factored_truncation_is_the_per_element_projection(P, rough):
     # thirteenpatch_tests.jl:121-165: the radial pair times the angular

@@ -194,6 +194,7 @@ FUSED = [
     dict(D=3, P=1, E=4, roots=2, reflecting=True),
     dict(D=3, P=8, E=2, roots=2),                       # the largest order, 18^3 blocks
     dict(D=3, P=5, E=4, roots=2, reflecting=True),      # 24^3 blocks, 576 threads
+    dict(D=3, P=3, E=4, roots=2),                       # the flat box benchmark's 16^3 blocks
 ]
 
 
