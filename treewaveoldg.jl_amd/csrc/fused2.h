@@ -293,9 +293,6 @@ __device__ void batch_lines(const BatchArgs<T>& A, const BatchItem& it, T* lds) 
         acc[o + NT] = av[j] + T(2) * kv;
         unext = yu[j] + ak * ku[j];
         w[j] = yv[j] + ak * kv;
-      } else if (MODE == 3) {
-        unext = yu[j] + ak * (au[j] + ku[j]);
-        w[j] = yv[j] + ak * (av[j] + kv);
       } else if (MODE == 7) {  // stage 3, and z = y_n + dt/6 (acc + 2 k3) for the last
         acc[o] = yu[j] + ak2 * (au[j] + T(2) * ku[j]);
         acc[o + NT] = yv[j] + ak2 * (av[j] + T(2) * kv);

@@ -106,7 +106,7 @@ struct FusedArgs {
   const double* spacing;    // (nb)
   const Tables<T>* tables;
   T a;               // stage coefficient (dt/2, dt or dt/6)
-  T a2;              // mode 7: the final combination's dt/6
+  T a2;              // modes 7, 9: the final combination's dt/6
   int truncate;
 };
 
@@ -137,11 +137,10 @@ constexpr FusedTab<T, P1> fused_tab() {
 
 // Face ghosts are formed here from a same-level neighbour's face buffers (nbr
 // >= 0) or from the own face on a mirror (-1); with GHOST, a coarse-fine face
-// (-2) takes them complete from k_face_ghost_u/_g.
-// MODE 1: stage 1 (ys = yn); 6: stage 2; 2: stage 3; 3: the final stage; 4: the
+// (-2) takes them complete from the ghost slots k_cf_ghosts fills.
+// MODE 1: stage 1 (ys = yn); 6: stage 2; 2: stage 3; 4: the
 // final stage of the compensated form (u by a Kahan sum into (yn, lo), v's
-// truncated increment likewise into (vn, vlo)); 5: the final stage of the split
-// form (u plain in FP64, v's increment added to the FP32 pair (vn, vlo) in FP64).
+// truncated increment likewise into (vn, vlo)).
 // The accumulator's u after stage 1 would be k1's u, which is y_n's v: stage 1
 // stores only the accumulator's v, and stage 2 forms acc_u = v_n + 2 k2_u from
 // the y_n it reads anyway (the same values bit for bit: two values fewer per
@@ -149,7 +148,7 @@ constexpr FusedTab<T, P1> fused_tab() {
 // accumulator's place, and 8, the final stage, forms y_n+1 = z + dt/6 k4 from
 // it without reading y_n (two values fewer again).  The split form does the same
 // for u alone: 9, stage 3 (z for u, the accumulator for v), and 10, the final
-// stage (u from z, v's increment into its FP32 pair as in 5).
+// stage (u from z, v's increment added to the FP32 pair (vn, vlo) in FP64).
 template <class T, int P1, int E, int MODE, bool GHOST, class TV = T>
 __global__ void __launch_bounds__(((P1 * E) * (P1 * E) + 63) / 64 * 64, (fused_waves_per_eu<T, P1, E>()))
 k_fused3(FusedArgs<T, TV> A) {
@@ -479,8 +478,7 @@ k_fused3(FusedArgs<T, TV> A) {
           yu = yv = T(0);
         } else {
           yu = ld_once(un_ + iu);  // compensated form: the high word
-          // compensated form: u's low word; split form: unused
-          yv = MODE == 4 ? ld_once(lo + iu) : (MODE == 5 ? T(0) : (T)ld_once(vn + iu));
+          yv = MODE == 4 ? ld_once(lo + iu) : (T)ld_once(vn + iu);  // compensated: u's low word
         }
         au = MODE == 6 ? yv : (MODE >= 2 ? ld_once(acc + iu) : T(0));
         av = MODE >= 2 ? (T)ld_once(vacc + iu) : T(0);
@@ -518,9 +516,6 @@ k_fused3(FusedArgs<T, TV> A) {
           out(acc, true, au + T(2) * ku, av + T(2) * kv);
           unext = yu + A.a * ku;
           w[j] = yv + A.a * kv;
-        } else if (MODE == 3) {
-          unext = yu + A.a * (au + ku);
-          w[j] = yv + A.a * (av + kv);
         } else if (MODE == 7) {  // stage 3, and z for the final stage
           out(acc, true, yu + A.a2 * (au + T(2) * ku), yv + A.a2 * (av + T(2) * kv));
           unext = yu + A.a * ku;
@@ -572,7 +567,7 @@ k_fused3(FusedArgs<T, TV> A) {
     // ---- (4) x lines: v's truncation along x, faces of the next u along x;
     // columns: faces of the next u along z ----
     TV lov[P1], hv[P1];  // the final stage's v pair, ahead of this phase's stores
-    if (act && (MODE == 4 || MODE == 5 || MODE == 10)) {
+    if (act && (MODE == 4 || MODE == 10)) {
 #pragma unroll
       for (int j = 0; j < P1; ++j) {
         const int iv = col + (e * P1 + j) * N2;
@@ -647,7 +642,7 @@ k_fused3(FusedArgs<T, TV> A) {
           const T yk = wl[j] - (T)lov[j], tv = (T)hv[j] + yk;
           st_once(vlo + iu, (TV)((tv - (T)hv[j]) - yk));
           st_once(vo + iu, (TV)tv);
-        } else if (MODE == 5 || MODE == 10) {
+        } else if (MODE == 10) {
           // v_n = hi - lo exactly in T; the new pair rounds v_n + increment
           const T vt = ((T)hv[j] - (T)lov[j]) + wl[j];
           const TV hi = (TV)vt;
