@@ -27,21 +27,22 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "treewaveoldg.jl_amd"))
 
 PEAK_HBM_GBS = 8000.0
-# algorithmic values moved per node-update by the fused three-register RK4:
-# stage 1 reads y_n (2), writes the accumulator's v (1) and y_2 (2); stage 2
-# reads y_2, y_n (4) and the accumulator's v (1), writes acc, y_3 (4) (its
-# accumulator u, v_n + 2 k2_u, from y_n's v); stage 3 reads y_3, y_n, acc (6)
-# and writes acc, y_4 (4); stage 4 reads y_4, y_n, acc (6) and writes y_{n+1}
-# (2): 32 values, u's 15 and v's 17.  FP64 stores z = y_n + dt/6 (acc + 2 k3) in
-# place of the accumulator at stage 3, so that stage 4 reads y_4 and z (4) and
-# not y_n: 30 values.  The compensated form keeps y_n at stage 4 and adds the
-# low words of u and v, read and written there (4 more values: 36).
-VALUES_PER_NODE_UPDATE = {"f64": 30, "f32c": 36}
+# algorithmic values moved per node-update by the fused RK4 stages.  The
+# compensated form keeps three registers: stage 1 reads y_n (2), writes the
+# accumulator's v (1) and y_2 (2); stage 2 reads y_2, y_n (4) and the
+# accumulator's v (1), writes acc, y_3 (4) (its accumulator u, v_n + 2 k2_u, from
+# y_n's v); stage 3 reads y_3, y_n, acc (6) and writes acc, y_4 (4); stage 4
+# reads y_4, y_n, acc and the low words of u and v (8) and writes y_{n+1} and the
+# low words (4): 36 values.  FP64 keeps no accumulator: stage 1 reads y_n (2) and
+# writes y_2 (2); stage 2 reads y_2, y_n (4) and writes y_3 (2); stage 3 reads
+# y_3, y_2, y_n (6) and writes y_4 and z = (y_2 + 2 y_3)/3 + dt/3 k3 (4); stage 4
+# reads y_4 and z (4) and writes y_{n+1} (2): 26 values.
+VALUES_PER_NODE_UPDATE = {"f64": 26, "f32c": 36}
 BYTES_PER_VALUE = {"f64": 8, "f32c": 4}
-# The split form (u in FP64, v in FP32) moves u's 14 values in FP64 (u through z
-# as in FP64) and v's 17 in FP32, plus the low word of v read and written at
-# stage 4.
-BYTES_PER_NODE_UPDATE = {"f64": 240, "f32c": 144, "split": 14 * 8 + 17 * 4 + 2 * 4}
+# The split form (u in FP64, v in FP32) keeps the three registers: u's 14 values
+# in FP64 (stage 3 leaves z = u_n + dt/6 (acc_u + 2 k3_u) for stage 4) and v's 17
+# in FP32, plus the low word of v read and written at stage 4.
+BYTES_PER_NODE_UPDATE = {"f64": 208, "f32c": 144, "split": 14 * 8 + 17 * 4 + 2 * 4}
 
 
 # HBM bytes per launch of the stage kernel on the default workload, measured with

@@ -192,6 +192,15 @@ int twoldg_norms_noalloc(twoldg_problem* p, int a, int b, int vars_mask, int pno
 int twoldg_timestep(const twoldg_problem* p, double cfl, double* out);
 /* wait for the problem's stream */
 int twoldg_synchronize(twoldg_problem* p);
+/* Fill every working array of p (stage states, ghost rows, gradient sets, face
+   buffers and ghost slots, reduction partials; not the slots) with all-ones
+   bytes, a NaN in FP32 and FP64, and wait.  Every entry point writes what it
+   reads of these arrays first, so any call after this one returns what it
+   would have returned without it.  No counterpart among the reference's entry
+   points: its tests poison the working arrays the same way
+   (test/refined_tests.jl:145-161) to show that no kernel reads an entry it has
+   not written. */
+int twoldg_problem_poison(twoldg_problem* p);
 
 /* --- adaptive refinement ----------------------------------------------------- */
 /* The Legendre-mode amplitudes of variable var (0 = u, 1 = v) of a slot, one

@@ -113,7 +113,7 @@ int64_t make_runs(const twoldg_batch* B, const std::vector<int32_t>& Nh, const i
 
 template <class T, int MODE>
 void launch_stage(twoldg_batch* B, const std::vector<BatchRun>& runs, int64_t nwg, const T* ys,
-                  T* yo, double coef, int in, double coef2 = 0) {
+                  T* yo, double coef, int in, double coef2 = 0, T* acc = nullptr) {
   BatchArgs<T> A{};
   for (int r = 0; r < kBatchMaxRuns; ++r) {
     if (r < (int)runs.size()) {
@@ -130,7 +130,7 @@ void launch_stage(twoldg_batch* B, const std::vector<BatchRun>& runs, int64_t nw
   A.tables = (const Tables<T>*)B->tables;
   A.ys = ys;
   A.yn = (const T*)B->yn;
-  A.acc = (T*)B->acc;
+  A.acc = acc ? acc : (T*)B->acc;
   A.yo = yo;
   A.lo = (T*)B->lo;
   A.uf_in = (const T*)B->uf[in];
@@ -148,18 +148,28 @@ template <class T>
 void enqueue_step(twoldg_batch* B) {
   T* yn = (T*)B->yn;
   T* ys = (T*)B->ys;
-  launch_stage<T, 1>(B, B->runs, B->nwg, yn, ys, 0.5, 0);
-  launch_stage<T, 6>(B, B->runs, B->nwg, ys, ys, 0.5, 1);
-  if (B->precision == TWOLDG_F32C) {
+  if constexpr (std::is_same<T, float>::value) {  // FP32c: the accumulator's three registers
+    launch_stage<T, 1>(B, B->runs, B->nwg, yn, ys, 0.5, 0);
+    launch_stage<T, 6>(B, B->runs, B->nwg, ys, ys, 0.5, 1);
     launch_stage<T, 2>(B, B->runs, B->nwg, ys, ys, 1.0, 0);
     launch_stage<T, 4>(B, B->runs, B->nwg, ys, yn, 1.0 / 6.0, 1);
-  } else {  // FP64: stage 3 leaves z = y_n + dt/6 (acc + 2 k3), the last stage adds dt/6 k4
-    launch_stage<T, 7>(B, B->runs, B->nwg, ys, ys, 1.0, 0, 1.0 / 6.0);
-    launch_stage<T, 8>(B, B->runs, B->nwg, ys, yn, 1.0 / 6.0, 1);
+  } else {
+    // FP64, no accumulator: y_2 in ys, y_3 in acc; stage 3 writes y_4 over y_3
+    // and z = (y_2 + 2 y_3)/3 + dt/3 k3 over y_2; the last stage adds dt/6 k4
+    T* y3 = (T*)B->acc;
+    launch_stage<T, 11>(B, B->runs, B->nwg, yn, ys, 0.5, 0);
+    launch_stage<T, 12>(B, B->runs, B->nwg, ys, y3, 0.5, 1);
+    launch_stage<T, 13>(B, B->runs, B->nwg, y3, y3, 1.0, 0, 1.0 / 3.0, ys);
+    launch_stage<T, 8>(B, B->runs, B->nwg, y3, yn, 1.0 / 6.0, 1, 0.0, ys);
   }
 }
 
-// Time the middle stage for each order and candidate tile; keep the fastest.
+// The stage the autotune times: stage 3, which moves the most values (FP32c
+// mode 2, FP64 mode 13).
+template <class T>
+constexpr int kTuneMode = std::is_same<T, float>::value ? 2 : 13;
+
+// Time stage 3 for each order and candidate tile; keep the fastest.
 // First each order alone, then each order again inside the full launch with
 // the other orders at their chosen tiles: two tiles that time alike alone can
 // differ by 5% once every order shares the GPU.
@@ -182,15 +192,15 @@ int autotune(twoldg_batch* B, const std::vector<int32_t>& Nh) {
       int tile[10];
       for (int i = 0; i < 10; ++i) tile[i] = t;
       const int64_t nwg = make_runs(B, Nh, tile, P1, runs);
-      // stage 2 writes the accumulator and the stage buffer only
-      launch_stage<T, 2>(B, runs, nwg, (const T*)B->yn, (T*)B->ys, 0.5, 0);
+      // stage 3 writes the accumulator and the stage buffer only
+      launch_stage<T, kTuneMode<T>>(B, runs, nwg, (const T*)B->yn, (T*)B->ys, 0.5, 0);
       // the faster of two timings of kReps launches each: one order's launch
       // is short, so a few launches leave the choice to timer noise
       float ms = 1e30f;
       for (int round = 0; round < 2; ++round) {
         HIP_TRY(hipEventRecord(B->ev0, B->stream));
         for (int r = 0; r < kReps; ++r)
-          launch_stage<T, 2>(B, runs, nwg, (const T*)B->yn, (T*)B->ys, 0.5, 0);
+          launch_stage<T, kTuneMode<T>>(B, runs, nwg, (const T*)B->yn, (T*)B->ys, 0.5, 0);
         HIP_TRY(hipEventRecord(B->ev1, B->stream));
         HIP_TRY(hipEventSynchronize(B->ev1));
         float m = 0;
@@ -217,12 +227,12 @@ int autotune(twoldg_batch* B, const std::vector<int32_t>& Nh) {
       for (int i = 0; i < 10; ++i) tile[i] = B->tiles[i];
       tile[P1] = t;
       const int64_t nwg = make_runs(B, Nh, tile, 0, runs);
-      launch_stage<T, 2>(B, runs, nwg, (const T*)B->yn, (T*)B->ys, 0.5, 0);
+      launch_stage<T, kTuneMode<T>>(B, runs, nwg, (const T*)B->yn, (T*)B->ys, 0.5, 0);
       float ms = 1e30f;
       for (int round = 0; round < 2; ++round) {
         HIP_TRY(hipEventRecord(B->ev0, B->stream));
         for (int r = 0; r < kReps; ++r)
-          launch_stage<T, 2>(B, runs, nwg, (const T*)B->yn, (T*)B->ys, 0.5, 0);
+          launch_stage<T, kTuneMode<T>>(B, runs, nwg, (const T*)B->yn, (T*)B->ys, 0.5, 0);
         HIP_TRY(hipEventRecord(B->ev1, B->stream));
         HIP_TRY(hipEventSynchronize(B->ev1));
         float m = 0;
@@ -465,6 +475,13 @@ int batch_attach(const twoldg_desc* d, void* yn, void* ys, void* acc, void* lo,
 }
 
 void batch_detach(twoldg_batch* B) { batch_destroy(B); }
+
+void batch_work(const twoldg_batch* B, std::vector<std::pair<void*, size_t>>& out) {
+  for (int k = 0; k < 2; ++k) {
+    out.emplace_back(B->uf[k], (size_t)B->nfaces * B->elt);
+    out.emplace_back(B->pf[k], (size_t)B->nfaces * B->elt);
+  }
+}
 
 // nsteps RK4 steps of the attached problem with time step dt.
 int batch_attached_steps(twoldg_batch* B, double dt, int64_t nsteps) {

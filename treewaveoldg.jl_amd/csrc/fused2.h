@@ -214,7 +214,7 @@ __device__ void batch_lines(const BatchArgs<T>& A, const BatchItem& it, T* lds) 
       const unsigned o = ob + node(j);
       uy[j] = ys[o];
       ku[j] = ys[o + NT];
-      if (MODE != 1 && MODE != 8) {  // the compensated form: the high words
+      if (MODE != 1 && MODE != 8 && MODE != 11) {  // the compensated form: the high words
         yu[j] = yn[o];
         yv[j] = yn[o + NT];
       }
@@ -222,7 +222,8 @@ __device__ void batch_lines(const BatchArgs<T>& A, const BatchItem& it, T* lds) 
         lu[j] = lo[o];
         lv[j] = lo[o + NT];
       }
-      if (MODE >= 2) {  // stage 2 (MODE 6): the accumulator's u is y_n's v
+      // stage 2 (MODE 6): the accumulator's u is y_n's v; MODE 13: y_2
+      if (MODE == 2 || MODE == 4 || MODE == 6 || MODE == 8 || MODE == 13) {
         au[j] = MODE == 6 ? yv[j] : acc[o];
         av[j] = acc[o + NT];
       }
@@ -293,9 +294,17 @@ __device__ void batch_lines(const BatchArgs<T>& A, const BatchItem& it, T* lds) 
         acc[o + NT] = av[j] + T(2) * kv;
         unext = yu[j] + ak * ku[j];
         w[j] = yv[j] + ak * kv;
-      } else if (MODE == 7) {  // stage 3, and z = y_n + dt/6 (acc + 2 k3) for the last
-        acc[o] = yu[j] + ak2 * (au[j] + T(2) * ku[j]);
-        acc[o + NT] = yv[j] + ak2 * (av[j] + T(2) * kv);
+      } else if (MODE == 11) {  // FP64 stage 1: y_2 only
+        unext = uy[j] + ak * ku[j];
+        w[j] = ku[j] + ak * kv;
+      } else if (MODE == 12) {  // FP64 stage 2: y_3 only
+        unext = yu[j] + ak * ku[j];
+        w[j] = yv[j] + ak * kv;
+      } else if (MODE == 13) {
+        // FP64 stage 3: y_4, and z = (y_2 + 2 y_3)/3 + dt/3 k3 over y_2 (in acc)
+        constexpr T third = T(1) / T(3);
+        acc[o] = (au[j] + T(2) * uy[j]) * third + ak2 * ku[j];
+        acc[o + NT] = (av[j] + T(2) * ku[j]) * third + ak2 * kv;
         unext = yu[j] + ak * ku[j];
         w[j] = yv[j] + ak * kv;
       } else if (MODE == 8) {  // the final stage from z (no y_n)

@@ -504,6 +504,26 @@ int create_buffers(twoldg_problem* p, const twoldg_desc* d, const Csr& csr) {
   if ((rc = dalloc(p, &p->part, 3 * p->sh.nb))) return rc;
   if ((rc = dalloc(p, &p->red, 4))) return rc;
   if ((rc = dalloc(p, &p->clock, 2))) return rc;
+  // the working arrays, for twoldg_problem_poison
+  auto work = [&](void* ptr, size_t bytes) {
+    if (ptr) p->work.emplace_back(ptr, bytes);
+  };
+  const size_t sb = (size_t)n * sizeof(T), fb = (size_t)p->sh.nb * 6 * p->sh.Nt * sizeof(T);
+  const size_t gb = (size_t)p->sh.nb * D * p->sh.nn * sizeof(T);
+  const size_t vb = (size_t)p->sh.nb * p->sh.nn * sizeof(float);
+  work(p->ys, sb);
+  work(p->acc, sb);
+  work(p->ug, p->nrows * sizeof(T));
+  work(p->gg, p->nrows * (p->full ? D : 1) * sizeof(T));
+  work(p->grads, gb);
+  work(p->flux, gb);
+  work(p->fgu, 2 * fb);
+  work(p->uf[0], 2 * fb);
+  work(p->uf[1], 2 * fb);
+  for (float* v : {p->svn, p->svlo, p->svs, p->svacc}) work(v, vb);
+  work(p->part, 3 * p->sh.nb * sizeof(double));
+  work(p->red, 4 * sizeof(double));
+  if (p->rpart) work(p->rpart, (p->nrad / p->sh.Nt + 1) * sizeof(double));
   return 0;
 }
 
@@ -812,6 +832,15 @@ int twoldg_timestep(const twoldg_problem* p, double cfl, double* out) {
 
 int twoldg_synchronize(twoldg_problem* p) {
   if (!p) return fail(TWOLDG_EINVAL, "no problem");
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  return 0;
+}
+
+int twoldg_problem_poison(twoldg_problem* p) {
+  if (!p) return fail(TWOLDG_EINVAL, "no problem");
+  std::vector<std::pair<void*, size_t>> w = p->work;
+  if (p->b2) twoldg_detail::batch_work(p->b2, w);
+  for (const auto& a : w) HIP_TRY(hipMemsetAsync(a.first, 0xff, a.second, p->stream));
   HIP_TRY(hipStreamSynchronize(p->stream));
   return 0;
 }

@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/twoldg.h"
@@ -35,6 +36,8 @@ int batch_attach(const twoldg_desc* d, void* yn, void* ys, void* acc, void* lo,
                  hipStream_t stream, twoldg_batch** out);
 void batch_detach(twoldg_batch* B);
 int batch_attached_steps(twoldg_batch* B, double dt, int64_t nsteps);
+// The batch's working arrays (its face slots), as (address, bytes).
+void batch_work(const twoldg_batch* B, std::vector<std::pair<void*, size_t>>& out);
 }  // namespace twoldg_detail
 using twoldg_detail::fail;
 using twoldg_detail::pool_alloc;
@@ -1089,6 +1092,9 @@ struct twoldg_problem {
   double hmin = 0, hspeed = 0;
   bool has_tables = false;
   int64_t device_bytes = 0;
+  // the working arrays (address, bytes): stage states, ghosts, gradients, face
+  // buffers and reduction partials, which every call writes before it reads them
+  std::vector<std::pair<void*, size_t>> work;
   bool uploaded[TWOLDG_NSLOTS] = {false, false, false, false};
   // host <-> device staging of state transfers, allocated on the first
   // transfer and kept: fine-grained host doubles of one state, which the
@@ -1551,6 +1557,7 @@ __global__ void k_join_v(int64_t nb, int64_t nn, const float* __restrict__ hi,
 template <class T, class TV, int P1, int E, bool GHOST>
 int fused_run(twoldg_problem* p, double dt, int64_t nsteps) {
   constexpr bool split = !std::is_same<T, TV>::value;
+  constexpr bool f64 = std::is_same<TV, double>::value;
   constexpr int N = P1 * E;
   constexpr int threads = (N * N + 63) / 64 * 64;
   const int64_t nb = p->sh.nb, nn = p->sh.nn;
@@ -1587,7 +1594,7 @@ int fused_run(twoldg_problem* p, double dt, int64_t nsteps) {
     } else {
       A.vs = (const TV*)(A.ys + nn);
       A.vn = (const TV*)(A.yn + nn);
-      A.vacc = (TV*)(A.acc + nn);
+      A.vacc = A.acc ? (TV*)(A.acc + nn) : nullptr;
       A.vo = (TV*)(A.yo + nn);
       A.vlo = A.lo ? (TV*)(A.lo + nn) : nullptr;
     }
@@ -1608,58 +1615,108 @@ int fused_run(twoldg_problem* p, double dt, int64_t nsteps) {
     A.uf_out = (T*)p->uf[1 - in];
     A.pf_out = (T*)p->pf[1 - in];
   };
-  auto one_step = [&]() {
-    A.ys = yn;  // stage 1 (mode 1) relies on ys = yn
-    A.yo = (T*)p->ys;
-    A.a = (T)(dt / 2);
-    vptrs(1);
-    io(0);
-    tmark(p);
-    ghosts(0);
-    hipLaunchKernelGGL((k_fused3<T, P1, E, 1, GHOST, TV>), dim3((unsigned)nb), dim3(threads), 0,
-                       p->stream, A);
-    tmark(p);
-    A.ys = (T*)p->ys;
-    vptrs(2);
-    io(1);
-    tmark(p);
-    ghosts(1);
-    hipLaunchKernelGGL((k_fused3<T, P1, E, 6, GHOST, TV>), dim3((unsigned)nb), dim3(threads), 0,
-                       p->stream, A);
-    tmark(p);
-    A.a = (T)dt;
-    A.a2 = (T)(dt / 6);
-    io(0);
-    tmark(p);
-    ghosts(0);
-    // FP64: stage 3 leaves z = y_n + dt/6 (acc + 2 k3) for the final stage; the
-    // split form does so for u
-    if constexpr (split)
-      hipLaunchKernelGGL((k_fused3<T, P1, E, 9, GHOST, TV>), dim3((unsigned)nb), dim3(threads), 0,
+  // FP64: the stage states in the two buffers by turns, no accumulator
+  // (modes 11, 12, 13, 8): y_2 in ys; y_3 in acc; y_4 over y_3 and z over y_2
+  auto f64_step = [&]() {
+    if constexpr (f64) {
+      A.ys = yn;  // stage 1 (mode 11) relies on ys = yn
+      A.yo = (T*)p->ys;
+      A.acc = nullptr;
+      A.a = (T)(dt / 2);
+      vptrs(1);
+      io(0);
+      tmark(p);
+      ghosts(0);
+      hipLaunchKernelGGL((k_fused3<T, P1, E, 11, GHOST, TV>), dim3((unsigned)nb), dim3(threads), 0,
                          p->stream, A);
-    else if (p->precision == TWOLDG_F64)
-      hipLaunchKernelGGL((k_fused3<T, P1, E, 7, GHOST, TV>), dim3((unsigned)nb), dim3(threads), 0,
+      tmark(p);
+      A.ys = (T*)p->ys;
+      A.yo = (T*)p->acc;
+      vptrs(2);
+      io(1);
+      tmark(p);
+      ghosts(1);
+      hipLaunchKernelGGL((k_fused3<T, P1, E, 12, GHOST, TV>), dim3((unsigned)nb), dim3(threads), 0,
                          p->stream, A);
-    else
-      hipLaunchKernelGGL((k_fused3<T, P1, E, 2, GHOST, TV>), dim3((unsigned)nb), dim3(threads), 0,
+      tmark(p);
+      A.ys = (T*)p->acc;
+      A.yo = (T*)p->acc;
+      A.acc = (T*)p->ys;
+      A.a = (T)dt;
+      A.a2 = (T)(dt / 3);
+      vptrs(3);
+      io(0);
+      tmark(p);
+      ghosts(0);
+      hipLaunchKernelGGL((k_fused3<T, P1, E, 13, GHOST, TV>), dim3((unsigned)nb), dim3(threads), 0,
                          p->stream, A);
-    tmark(p);
-    A.a = (T)(dt / 6);
-    A.yo = yn;
-    vptrs(4);
-    io(1);
-    tmark(p);
-    ghosts(1);
-    if constexpr (split)
-      hipLaunchKernelGGL((k_fused3<T, P1, E, 10, GHOST, TV>), dim3((unsigned)nb), dim3(threads), 0,
-                         p->stream, A);
-    else if (p->precision == TWOLDG_F32C)
-      hipLaunchKernelGGL((k_fused3<T, P1, E, 4, GHOST, TV>), dim3((unsigned)nb), dim3(threads), 0,
-                         p->stream, A);
-    else
+      tmark(p);
+      A.a = (T)(dt / 6);
+      A.yo = yn;
+      vptrs(4);
+      io(1);
+      tmark(p);
+      ghosts(1);
       hipLaunchKernelGGL((k_fused3<T, P1, E, 8, GHOST, TV>), dim3((unsigned)nb), dim3(threads), 0,
                          p->stream, A);
-    tmark(p);
+      tmark(p);
+    }
+  };
+  // FP32c and split: the accumulator's three-register form (modes 1, 6, then
+  // FP32c 2, 4; split 9, 10)
+  auto acc_step = [&]() {
+    if constexpr (!f64) {
+      A.ys = yn;  // stage 1 (mode 1) relies on ys = yn
+      A.yo = (T*)p->ys;
+      A.a = (T)(dt / 2);
+      vptrs(1);
+      io(0);
+      tmark(p);
+      ghosts(0);
+      hipLaunchKernelGGL((k_fused3<T, P1, E, 1, GHOST, TV>), dim3((unsigned)nb), dim3(threads), 0,
+                         p->stream, A);
+      tmark(p);
+      A.ys = (T*)p->ys;
+      vptrs(2);
+      io(1);
+      tmark(p);
+      ghosts(1);
+      hipLaunchKernelGGL((k_fused3<T, P1, E, 6, GHOST, TV>), dim3((unsigned)nb), dim3(threads), 0,
+                         p->stream, A);
+      tmark(p);
+      A.a = (T)dt;
+      A.a2 = (T)(dt / 6);
+      io(0);
+      tmark(p);
+      ghosts(0);
+      // the split form's stage 3 leaves z = u_n + dt/6 (acc_u + 2 k3_u) for u
+      if constexpr (split)
+        hipLaunchKernelGGL((k_fused3<T, P1, E, 9, GHOST, TV>), dim3((unsigned)nb), dim3(threads), 0,
+                           p->stream, A);
+      else
+        hipLaunchKernelGGL((k_fused3<T, P1, E, 2, GHOST, TV>), dim3((unsigned)nb), dim3(threads), 0,
+                           p->stream, A);
+      tmark(p);
+      A.a = (T)(dt / 6);
+      A.yo = yn;
+      vptrs(4);
+      io(1);
+      tmark(p);
+      ghosts(1);
+      if constexpr (split)
+        hipLaunchKernelGGL((k_fused3<T, P1, E, 10, GHOST, TV>), dim3((unsigned)nb), dim3(threads), 0,
+                           p->stream, A);
+      else
+        hipLaunchKernelGGL((k_fused3<T, P1, E, 4, GHOST, TV>), dim3((unsigned)nb), dim3(threads), 0,
+                           p->stream, A);
+      tmark(p);
+    }
+  };
+  auto one_step = [&]() {
+    if constexpr (f64)
+      f64_step();
+    else
+      acc_step();
   };
   if (p->timing || p->warming || nsteps < 2) {
     for (int64_t s = 0; s < nsteps; ++s) one_step();

@@ -89,6 +89,7 @@ def lib():
                                       ctypes.c_int, _vp, _vp], ctypes.c_int),
             "twoldg_timestep": ([_vp, _dbl, _vp], ctypes.c_int),
             "twoldg_synchronize": ([_vp], ctypes.c_int),
+            "twoldg_problem_poison": ([_vp], ctypes.c_int),
             "twoldg_modes": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp,
                               _dbl, _vp], ctypes.c_int),
             "twoldg_transfer": ([_vp, ctypes.c_int, _vp, ctypes.c_int, _vp], ctypes.c_int),
@@ -131,7 +132,7 @@ def exported_symbols():
             "twoldg_state_upload", "twoldg_state_download", "twoldg_state_copy", "twoldg_rhs",
             "twoldg_truncate", "twoldg_rk4", "twoldg_energy", "twoldg_norms",
             "twoldg_norms_noalloc", "twoldg_timestep",
-            "twoldg_synchronize", "twoldg_modes", "twoldg_transfer", "twoldg_exchange_plan", "twoldg_node_coordinates",
+            "twoldg_synchronize", "twoldg_problem_poison", "twoldg_modes", "twoldg_transfer", "twoldg_exchange_plan", "twoldg_node_coordinates",
             "twoldg_rk4_timed",
             "twoldg_batch_create", "twoldg_batch_destroy", "twoldg_batch_upload",
             "twoldg_batch_download", "twoldg_batch_rk4", "twoldg_batch_info"]

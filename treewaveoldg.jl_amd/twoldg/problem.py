@@ -47,9 +47,11 @@ class WaveProblem:
     `precision`: "f64" (the reference's structure, FP64 throughout), "f32c"
     (FP32 stencil and storage, compensated FP32 hi+lo RK update, FP64
     reductions) or "split" (u and its stencil in FP64, v in FP32 with a
-    compensated step update; the fused 3D stages only).  `geometry`: None (flat, logical coordinates) or a
-    SevenPatchBall (full metric).  `boundary`: Dirichlet data on a full
-    geometry's outer sphere, a PlaneWave.
+    compensated step update; the fused 3D stages only).  `geometry`: None (flat,
+    logical coordinates), a diagonal metric (PolarAnnulus) or a full one
+    (SevenPatchBall, ThirteenPatchBall; a full geometry whose `outer_faces` is
+    False has no outer sphere and takes no boundary).  `boundary`: the data on a
+    full geometry's outer sphere, a PlaneWave (Dirichlet) or a RadiativeBoundary.
     """
 
     def __init__(self, forest, el=None, truncate=True, precision="f64", geometry=None,
@@ -75,7 +77,10 @@ class WaveProblem:
                 "Dirichlet data (`boundary`) are given in Cartesian components and taken by a "
                 "full geometry only: without one the gradient sets hold logical components, "
                 "and the walls are TreeAMR's reflecting faces")
-        if full and boundary is None:
+        # a full geometry without outer faces (geometry_boundary false,
+        # geometry.jl:61-84) takes no boundary data
+        outer = full and getattr(geometry, "outer_faces", True)
+        if outer and boundary is None:
             raise ValueError(
                 "the geometry has outer faces, and no data were given for them: pass "
                 "`boundary = PlaneWave(k, phi)`, the Dirichlet data on the outer sphere, or "
@@ -149,7 +154,7 @@ class WaveProblem:
                 desc.rad_kappa = boundary.kappa
                 desc.rad_invR = boundary.invR(geometry)
                 self.rad_invR = desc.rad_invR
-            else:
+            elif boundary is not None:
                 self.dirichlet_x = geometry.face_positions(forest, el)
                 desc.dirichlet_x = _ptr(self.dirichlet_x)
                 for c in range(3):
@@ -197,6 +202,13 @@ class WaveProblem:
         the problem does not truncate (truncate_state!, evolution.jl:1383-1395)."""
         if self.truncate:
             _lib.check(_lib.lib().twoldg_truncate(self._h, _lib.SLOT_STATE))
+
+    def poison(self):
+        """Fill every working array of the problem (not its state slots) with NaN;
+        every call after this returns what it would have without it (the
+        reference's tests poison `p.ufs.work` and the gradient sets the same way,
+        test/refined_tests.jl:145-161)."""
+        _lib.check(_lib.lib().twoldg_problem_poison(self._h))
 
     def info(self):
         n = ctypes.c_int64()
