@@ -39,10 +39,10 @@ PEAK_HBM_GBS = 8000.0
 # reads y_4 and z (4) and writes y_{n+1} (2): 26 values.
 VALUES_PER_NODE_UPDATE = {"f64": 26, "f32c": 36}
 BYTES_PER_VALUE = {"f64": 8, "f32c": 4}
-# The split form (u in FP64, v in FP32) keeps the three registers: u's 14 values
-# in FP64 (stage 3 leaves z = u_n + dt/6 (acc_u + 2 k3_u) for stage 4) and v's 17
-# in FP32, plus the low word of v read and written at stage 4.
-BYTES_PER_NODE_UPDATE = {"f64": 208, "f32c": 144, "split": 14 * 8 + 17 * 4 + 2 * 4}
+# The split form (u in FP64, v in FP32) moves u's 13 values in FP64, without an
+# accumulator as in FP64, and v's 17 in FP32 with the accumulator, plus the low
+# word of v read and written at stage 4.
+BYTES_PER_NODE_UPDATE = {"f64": 208, "f32c": 144, "split": 13 * 8 + 17 * 4 + 2 * 4}
 
 
 # HBM bytes per launch of the stage kernel on the default workload, measured with

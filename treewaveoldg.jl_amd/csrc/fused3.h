@@ -106,7 +106,7 @@ struct FusedArgs {
   const double* spacing;    // (nb)
   const Tables<T>* tables;
   T a;               // stage coefficient (dt/2, dt or dt/6)
-  T a2;              // mode 9: the final combination's dt/6; 13: dt/3
+  T a2;              // modes 13, 15: dt/3
   int truncate;
 };
 
@@ -138,20 +138,21 @@ constexpr FusedTab<T, P1> fused_tab() {
 // Face ghosts are formed here from a same-level neighbour's face buffers (nbr
 // >= 0) or from the own face on a mirror (-1); with GHOST, a coarse-fine face
 // (-2) takes them complete from the ghost slots k_cf_ghosts fills.
-// MODE selects the stage.  The compensated (FP32c) and split forms keep three
-// registers: 1, stage 1 (ys = yn); 6, stage 2; 2, stage 3; 4, the compensated
-// final stage (u by a Kahan sum into (yn, lo), v's truncated increment likewise
-// into (vn, vlo)).  The accumulator's u after stage 1 would be k1's u, which is
-// y_n's v: stage 1 stores only the accumulator's v, and stage 2 forms
-// acc_u = v_n + 2 k2_u from the y_n it reads anyway.  The split form's stage 3,
-// 9, stores z = u_n + dt/6 (acc_u + 2 k3_u) for u (the accumulator for v), and
-// its final stage, 10, forms u from z without reading y_n and adds v's
-// increment to the FP32 pair (vn, vlo) in FP64.
+// MODE selects the stage.  The compensated form (FP32c) keeps three registers:
+// 1, stage 1 (ys = yn); 6, stage 2; 2, stage 3; 4, the final stage (u by a
+// Kahan sum into (yn, lo), v's truncated increment likewise into (vn, vlo)).
+// The accumulator's u after stage 1 would be k1's u, which is y_n's v: stage 1
+// stores only the accumulator's v, and stage 2 forms acc_u = v_n + 2 k2_u from
+// the y_n it reads anyway.
 // FP64 keeps no accumulator: 11, stage 1, and 12, stage 2, store only the next
 // stage state (y_2, then y_3 beside it); 13, stage 3, forms y_4 and, from y_2
 // and y_3, z = (y_2 + 2 y_3)/3 + dt/3 k3 (= y_n + dt/6 (k1 + 2 k2 + 2 k3), the
 // k's being (y_2 - y_n) 2/dt and (y_3 - y_n) 2/dt) in y_2's place; 8, the final
 // stage, forms y_n+1 = z + dt/6 k4: 26 values per node and step.
+// The split form does so for u and keeps the accumulator for v: 1, stage 1;
+// 14, stage 2 (y_3 beside y_2, v's accumulator); 15, stage 3 (u as in 13, v's
+// accumulator); 10, the final stage (u from z, v's increment added to the FP32
+// pair (vn, vlo) in FP64).
 template <class T, int P1, int E, int MODE, bool GHOST, class TV = T>
 __global__ void __launch_bounds__(((P1 * E) * (P1 * E) + 63) / 64 * 64, (fused_waves_per_eu<T, P1, E>()))
 k_fused3(FusedArgs<T, TV> A) {
@@ -470,8 +471,8 @@ k_fused3(FusedArgs<T, TV> A) {
       T w[P1];
       T plo = 0, phi = 0;  // partial y-face gradients of the next u
       // modes whose accumulator operand is read (modes 8, 10: z; 13: y_2)
-      constexpr bool acc_in = MODE == 2 || MODE == 4 || MODE == 8 || MODE == 9 || MODE == 10 ||
-                              MODE == 13;
+      constexpr bool acc_in = MODE == 2 || MODE == 4 || MODE == 8 || MODE == 10 || MODE == 13 ||
+                              MODE == 15;
       constexpr T third = T(1) / T(3);
       auto ops = [&](int j, T& ku, T& yu, T& yv, T& au, T& av) {
         const int iu = base + j * N;
@@ -488,7 +489,7 @@ k_fused3(FusedArgs<T, TV> A) {
           yv = MODE == 4 ? ld_once(lo + iu) : (T)ld_once(vn + iu);  // compensated: u's low word
         }
         au = MODE == 6 ? yv : (acc_in ? ld_once(acc + iu) : T(0));
-        av = (MODE == 6 || acc_in) ? (T)ld_once(vacc + iu) : T(0);
+        av = (MODE == 6 || MODE == 14 || acc_in) ? (T)ld_once(vacc + iu) : T(0);
       };
       // the stage operands one node ahead where registers allow
       constexpr bool ahead = sizeof(T) == 4 || TWOLDG_FUSED_NB_F64 > 0;
@@ -511,7 +512,7 @@ k_fused3(FusedArgs<T, TV> A) {
             LX[l] = ou;
             if (with_v) LZ[l] = ov;
           } else {
-            if (MODE != 1) st_once(pu + iu, ou);
+            if (MODE != 1 && MODE != 14) st_once(pu + iu, ou);
             if (with_v) st_once(vacc + iu, (TV)ov);
           }
         };
@@ -535,13 +536,19 @@ k_fused3(FusedArgs<T, TV> A) {
               (av + T(2) * ku) * third + A.a2 * kv);
           unext = yu + A.a * ku;
           w[j] = yv + A.a * kv;
+        } else if (MODE == 14) {  // split, stage 2: y_3, the accumulator for v only
+          out(acc, true, T(0), av + T(2) * kv);
+          unext = yu + A.a * ku;
+          w[j] = yv + A.a * kv;
+        } else if (MODE == 15) {
+          // split, stage 3: u as in mode 13 (z over y_2), the accumulator for v
+          const T su = U[l];
+          out(acc, true, (au + T(2) * su) * third + A.a2 * ku, av + T(2) * kv);
+          unext = yu + A.a * ku;
+          w[j] = yv + A.a * kv;
         } else if (MODE == 8) {  // the final stage from z (in au, av)
           unext = au + A.a * ku;
           w[j] = av + A.a * kv;
-        } else if (MODE == 9) {  // split, stage 3: z for u, the accumulator for v
-          out(acc, true, yu + A.a2 * (au + T(2) * ku), av + T(2) * kv);
-          unext = yu + A.a * ku;
-          w[j] = yv + A.a * kv;
         } else if (MODE == 10) {  // split, the final stage: u from z, v's increment
           unext = au + A.a * ku;
           w[j] = A.a * (av + kv);
@@ -648,8 +655,8 @@ k_fused3(FusedArgs<T, TV> A) {
         if (defer) {  // the stage update's outputs of the column
           const int li = j * NT + col;
           st_once(yo + iu, UN[li]);
-          if (MODE == 2 || MODE == 6 || MODE == 9 || MODE == 13) st_once(acc + iu, LX[li]);
-          if (MODE == 1 || MODE == 2 || MODE == 6 || MODE == 9 || MODE == 13)
+          if (MODE == 2 || MODE == 6 || MODE == 13 || MODE == 15) st_once(acc + iu, LX[li]);
+          if (MODE == 1 || MODE == 2 || MODE == 6 || MODE == 13 || MODE == 14 || MODE == 15)
             st_once(vacc + iu, (TV)LZ[li]);
           if (MODE == 4) st_once(lo + iu, LX[li]);
         }

@@ -1557,7 +1557,6 @@ __global__ void k_join_v(int64_t nb, int64_t nn, const float* __restrict__ hi,
 template <class T, class TV, int P1, int E, bool GHOST>
 int fused_run(twoldg_problem* p, double dt, int64_t nsteps) {
   constexpr bool split = !std::is_same<T, TV>::value;
-  constexpr bool f64 = std::is_same<TV, double>::value;
   constexpr int N = P1 * E;
   constexpr int threads = (N * N + 63) / 64 * 64;
   const int64_t nb = p->sh.nb, nn = p->sh.nn;
@@ -1615,11 +1614,12 @@ int fused_run(twoldg_problem* p, double dt, int64_t nsteps) {
     A.uf_out = (T*)p->uf[1 - in];
     A.pf_out = (T*)p->pf[1 - in];
   };
-  // FP64: the stage states in the two buffers by turns, no accumulator
-  // (modes 11, 12, 13, 8): y_2 in ys; y_3 in acc; y_4 over y_3 and z over y_2
-  auto f64_step = [&]() {
-    if constexpr (f64) {
-      A.ys = yn;  // stage 1 (mode 11) relies on ys = yn
+  // FP64 and the split form's u: the stage states in the two buffers by turns,
+  // no accumulator: y_2 in ys; y_3 in acc; y_4 over y_3 and z over y_2 (FP64
+  // modes 11, 12, 13, 8; split 1, 14, 15, 10, whose v keeps its accumulator)
+  auto noacc_step = [&]() {
+    if constexpr (!std::is_same<T, float>::value) {
+      A.ys = yn;  // stage 1 relies on ys = yn
       A.yo = (T*)p->ys;
       A.acc = nullptr;
       A.a = (T)(dt / 2);
@@ -1627,8 +1627,8 @@ int fused_run(twoldg_problem* p, double dt, int64_t nsteps) {
       io(0);
       tmark(p);
       ghosts(0);
-      hipLaunchKernelGGL((k_fused3<T, P1, E, 11, GHOST, TV>), dim3((unsigned)nb), dim3(threads), 0,
-                         p->stream, A);
+      hipLaunchKernelGGL((k_fused3<T, P1, E, split ? 1 : 11, GHOST, TV>), dim3((unsigned)nb),
+                         dim3(threads), 0, p->stream, A);
       tmark(p);
       A.ys = (T*)p->ys;
       A.yo = (T*)p->acc;
@@ -1636,8 +1636,8 @@ int fused_run(twoldg_problem* p, double dt, int64_t nsteps) {
       io(1);
       tmark(p);
       ghosts(1);
-      hipLaunchKernelGGL((k_fused3<T, P1, E, 12, GHOST, TV>), dim3((unsigned)nb), dim3(threads), 0,
-                         p->stream, A);
+      hipLaunchKernelGGL((k_fused3<T, P1, E, split ? 14 : 12, GHOST, TV>), dim3((unsigned)nb),
+                         dim3(threads), 0, p->stream, A);
       tmark(p);
       A.ys = (T*)p->acc;
       A.yo = (T*)p->acc;
@@ -1648,8 +1648,8 @@ int fused_run(twoldg_problem* p, double dt, int64_t nsteps) {
       io(0);
       tmark(p);
       ghosts(0);
-      hipLaunchKernelGGL((k_fused3<T, P1, E, 13, GHOST, TV>), dim3((unsigned)nb), dim3(threads), 0,
-                         p->stream, A);
+      hipLaunchKernelGGL((k_fused3<T, P1, E, split ? 15 : 13, GHOST, TV>), dim3((unsigned)nb),
+                         dim3(threads), 0, p->stream, A);
       tmark(p);
       A.a = (T)(dt / 6);
       A.yo = yn;
@@ -1657,15 +1657,14 @@ int fused_run(twoldg_problem* p, double dt, int64_t nsteps) {
       io(1);
       tmark(p);
       ghosts(1);
-      hipLaunchKernelGGL((k_fused3<T, P1, E, 8, GHOST, TV>), dim3((unsigned)nb), dim3(threads), 0,
-                         p->stream, A);
+      hipLaunchKernelGGL((k_fused3<T, P1, E, split ? 10 : 8, GHOST, TV>), dim3((unsigned)nb),
+                         dim3(threads), 0, p->stream, A);
       tmark(p);
     }
   };
-  // FP32c and split: the accumulator's three-register form (modes 1, 6, then
-  // FP32c 2, 4; split 9, 10)
+  // FP32c: the accumulator's three registers (modes 1, 6, 2, 4)
   auto acc_step = [&]() {
-    if constexpr (!f64) {
+    if constexpr (std::is_same<T, float>::value) {
       A.ys = yn;  // stage 1 (mode 1) relies on ys = yn
       A.yo = (T*)p->ys;
       A.a = (T)(dt / 2);
@@ -1685,17 +1684,11 @@ int fused_run(twoldg_problem* p, double dt, int64_t nsteps) {
                          p->stream, A);
       tmark(p);
       A.a = (T)dt;
-      A.a2 = (T)(dt / 6);
       io(0);
       tmark(p);
       ghosts(0);
-      // the split form's stage 3 leaves z = u_n + dt/6 (acc_u + 2 k3_u) for u
-      if constexpr (split)
-        hipLaunchKernelGGL((k_fused3<T, P1, E, 9, GHOST, TV>), dim3((unsigned)nb), dim3(threads), 0,
-                           p->stream, A);
-      else
-        hipLaunchKernelGGL((k_fused3<T, P1, E, 2, GHOST, TV>), dim3((unsigned)nb), dim3(threads), 0,
-                           p->stream, A);
+      hipLaunchKernelGGL((k_fused3<T, P1, E, 2, GHOST, TV>), dim3((unsigned)nb), dim3(threads), 0,
+                         p->stream, A);
       tmark(p);
       A.a = (T)(dt / 6);
       A.yo = yn;
@@ -1703,20 +1696,16 @@ int fused_run(twoldg_problem* p, double dt, int64_t nsteps) {
       io(1);
       tmark(p);
       ghosts(1);
-      if constexpr (split)
-        hipLaunchKernelGGL((k_fused3<T, P1, E, 10, GHOST, TV>), dim3((unsigned)nb), dim3(threads), 0,
-                           p->stream, A);
-      else
-        hipLaunchKernelGGL((k_fused3<T, P1, E, 4, GHOST, TV>), dim3((unsigned)nb), dim3(threads), 0,
-                           p->stream, A);
+      hipLaunchKernelGGL((k_fused3<T, P1, E, 4, GHOST, TV>), dim3((unsigned)nb), dim3(threads), 0,
+                         p->stream, A);
       tmark(p);
     }
   };
   auto one_step = [&]() {
-    if constexpr (f64)
-      f64_step();
-    else
+    if constexpr (std::is_same<T, float>::value)
       acc_step();
+    else
+      noacc_step();
   };
   if (p->timing || p->warming || nsteps < 2) {
     for (int64_t s = 0; s < nsteps; ++s) one_step();
