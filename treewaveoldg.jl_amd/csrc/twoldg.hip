@@ -172,7 +172,7 @@ void destroy(twoldg_problem* p) {
                   (void*)p->fslot, (void*)p->fradial, (void*)p->fblk, (void*)p->oblk, p->frmodes, p->frduals, (void*)p->faslot, p->famodes, p->faweights, p->faginv,
                   (void*)p->rad_rows, (void*)p->rpart, (void*)p->svn, (void*)p->svlo,
                   (void*)p->svs, (void*)p->svacc, p->flux, (void*)p->nbr6, p->uf[0], p->uf[1],
-                  (void*)p->cfjobs, p->fgu})
+                  (void*)p->cfjobs, p->fgu, p->cartf, (void*)p->cf})
     dfree(b);
   if (p->step_exec) (void)hipGraphExecDestroy(p->step_exec);
   if (p->step_graph) (void)hipGraphDestroy(p->step_graph);
@@ -326,6 +326,19 @@ int create_buffers(twoldg_problem* p, const twoldg_desc* d, const Csr& csr) {
   if (p->full) {
     const size_t nc = (size_t)p->sh.nb * (1 + D * D) * p->sh.nn;
     if ((rc = upload_as<T>(p, &p->coef, d->coef, nc))) return rc;
+    // the Cartesian gradient at the face nodes, and where each exchange entry
+    // finds its source there
+    const int64_t ncf = p->sh.nb * 2 * D * D * p->sh.Nt;
+    if (ncf > INT32_MAX)
+      return fail(TWOLDG_EINVAL, "the face-compact Cartesian gradient of " +
+                                     std::to_string(p->sh.nb) + " blocks exceeds 2^31 entries");
+    if ((rc = dalloc_bytes(p, &p->cartf, (size_t)ncf * sizeof(T)))) return rc;
+    if (nnz) {
+      std::vector<int32_t> cf(nnz);
+      for (size_t k = 0; k < nnz; ++k) cf[k] = (int32_t)csr.cf[k];
+      if ((rc = dalloc(p, &p->cf, nnz))) return rc;
+      if ((rc = h2d(p, p->cf, cf.data(), nnz * 4))) return rc;
+    }
     if ((rc = dalloc(p, &p->flat, (p->sh.nb + 3) / 4 * 4))) return rc;
     std::vector<uint8_t> fl(p->sh.nb, 0);
     if (d->flat)
@@ -517,6 +530,7 @@ int create_buffers(twoldg_problem* p, const twoldg_desc* d, const Csr& csr) {
   work(p->gg, p->nrows * (p->full ? D : 1) * sizeof(T));
   work(p->grads, gb);
   work(p->flux, gb);
+  work(p->cartf, (size_t)p->sh.nb * 2 * D * D * p->sh.Nt * sizeof(T));
   work(p->fgu, 2 * fb);
   work(p->uf[0], 2 * fb);
   work(p->uf[1], 2 * fb);

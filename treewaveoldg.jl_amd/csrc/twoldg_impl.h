@@ -130,13 +130,16 @@ __device__ __forceinline__ int64_t tlin(const int I[kMaxD], int d, int N) {
 // Exchange: ghost rows (b, d, s, t) <- sum_k coef_k * src(block_k, node_k).
 // MODE 0: u from the state (stride 2 nn per block)
 // MODE 1: flat gradient g_d from grads (nb, D, nn), component d of the row
-// MODE 2: Cartesian gradient, all D components, into (rows/Nt, D, Nt)
+// MODE 2: Cartesian gradient, all D components, into (rows/Nt, D, Nt), from the
+//         face-compact gradient (nb, 2D, D, Nt) the gradient pass writes at the
+//         face nodes; cf gives each entry's face and tangential position there
 // ---------------------------------------------------------------------------
 struct ExchangeRows {
   int64_t nrows;
   const int32_t* rowptr;
   const int32_t* cb;
   const int32_t* cl;
+  const int32_t* cf;  // MODE 2: ((block 2D + face) D) Nt + tangential index
 };
 template <class T, int MODE, int D>
 __device__ __forceinline__ void exchange_row(int64_t r, const Shape& sh, const ExchangeRows& X,
@@ -158,11 +161,11 @@ __device__ __forceinline__ void exchange_row(int64_t r, const Shape& sh, const E
     ghost[r] = acc;
   } else {
     const int64_t face = r / sh.Nt, t = r % sh.Nt;
+    const int32_t* __restrict__ cf = X.cf;
 #pragma unroll
     for (int i = 0; i < D; ++i) {
       T acc = 0;
-      for (int k = k0; k < k1; ++k)
-        acc += coef[k] * src[((int64_t)cb[k] * D + i) * sh.nn + cl[k]];
+      for (int k = k0; k < k1; ++k) acc += coef[k] * src[(int64_t)cf[k] + (int64_t)i * sh.Nt];
       ghost[(face * D + i) * sh.Nt + t] = acc;
     }
   }
@@ -292,7 +295,10 @@ __global__ void k_gradient(Shape sh, const Tables<T>* __restrict__ tbp,
 
 // Cartesian gradient (full metric): V_i = sum_e S[e][i] D_e u / sqrt(g), or on a
 // flat block S[i][i] D_i u / J from the block's first node (evolution.jl:144-164).
-template <class T, int D, int P1>
+// FACES (the stages): V at the face nodes into the face-compact cart (nb, 2D, D,
+// Nt), which the exchange reads, and the fluxes; otherwise (the energy) V at every
+// node into cart (nb, D, nn) and no fluxes.
+template <class T, int D, int P1, bool FACES>
 __global__ void k_gradient_cart(Shape sh, const Tables<T>* __restrict__ tbp,
                                 const double* __restrict__ spacing, const T* __restrict__ y,
                                 const T* __restrict__ ug, const T* __restrict__ coef,
@@ -328,12 +334,12 @@ __global__ void k_gradient_cart(Shape sh, const Tables<T>* __restrict__ tbp,
   if (flat[b]) {
     const T J = c[0];
 #pragma unroll
-    for (int i = 0; i < D; ++i) {
-      V[i] = c[(1 + i * D + i) * sh.nn] * Du[i] / J;
-      cart[(b * D + i) * sh.nn + l] = V[i];
-    }
+    for (int i = 0; i < D; ++i) V[i] = c[(1 + i * D + i) * sh.nn] * Du[i] / J;
+    if (FACES) {
 #pragma unroll
-    for (int d = 0; d < D; ++d) flux[(b * D + d) * sh.nn + l] = c[(1 + d * D + d) * sh.nn] * V[d];
+      for (int d = 0; d < D; ++d)
+        flux[(b * D + d) * sh.nn + l] = c[(1 + d * D + d) * sh.nn] * V[d];
+    }
   } else {
 #pragma unroll
     for (int i = 0; i < D; ++i) {
@@ -341,15 +347,32 @@ __global__ void k_gradient_cart(Shape sh, const Tables<T>* __restrict__ tbp,
 #pragma unroll
       for (int e = 1; e < D; ++e) s += c[(1 + e * D + i) * sh.nn + l] * Du[e];
       V[i] = s / c[l];
-      cart[(b * D + i) * sh.nn + l] = V[i];
     }
+    if (FACES) {
 #pragma unroll
-    for (int d = 0; d < D; ++d) {
-      T s = c[(1 + d * D + 0) * sh.nn + l] * V[0];
+      for (int d = 0; d < D; ++d) {
+        T s = c[(1 + d * D + 0) * sh.nn + l] * V[0];
 #pragma unroll
-      for (int j = 1; j < D; ++j) s += c[(1 + d * D + j) * sh.nn + l] * V[j];
-      flux[(b * D + d) * sh.nn + l] = s;
+        for (int j = 1; j < D; ++j) s += c[(1 + d * D + j) * sh.nn + l] * V[j];
+        flux[(b * D + d) * sh.nn + l] = s;
+      }
     }
+  }
+  if (FACES) {
+    // a node on face (d, s) writes V at its tangential position there (a node
+    // on an edge or corner, to each of its faces)
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+#pragma unroll
+      for (int s = 0; s < 2; ++s)
+        if (I[d] == (s ? N - 1 : 0)) {
+          const int64_t base = ((b * 2 * D + 2 * d + s) * D) * sh.Nt + tlin<D>(I, d, N);
+#pragma unroll
+          for (int i = 0; i < D; ++i) cart[base + i * sh.Nt] = V[i];
+        }
+  } else {
+#pragma unroll
+    for (int i = 0; i < D; ++i) cart[(b * D + i) * sh.nn + l] = V[i];
   }
 }
 
@@ -357,10 +380,16 @@ __global__ void k_gradient_cart(Shape sh, const Tables<T>* __restrict__ tbp,
 // Stage kernels: the Laplacian and the RK4 combination in one pass.
 // k = (v_s, lap); by stage:
 //   MODE 0 (plain RHS):  du = k
+// FP32c keeps three registers:
 //   MODE 1 (stage 1):    acc = k;          yo = yn + a k
 //   MODE 2 (stages 2,3): acc = acc + 2 k;  yo = yn + a k
-//   MODE 3 (stage 4):    yn = yn + a (acc + k)          (a = dt/6)
-//   MODE 4 (stage 4, compensated): the same as a Kahan sum into (yn, lo)
+//   MODE 4 (stage 4, compensated): yn = yn + a (acc + k) (a = dt/6) as a Kahan
+//                        sum into (yn, lo)
+// FP64 keeps no accumulator (the k's are (y_2 - y_n) 2/dt and (y_3 - y_n) 2/dt):
+//   MODE 5 (stages 1, 2): yo = yn + a k
+//   MODE 7 (stage 3):    yo = yn + a k (y_4, over y_3 = ys); acc (y_2 in) takes
+//                        z = (y_2 + 2 y_3)/3 + a2 k = y_n + dt/6 (k1 + 2 k2 + 2 k3)
+//   MODE 8 (stage 4):    yo = acc (z) + a k              (a = dt/6)
 // ---------------------------------------------------------------------------
 template <class T>
 struct StageIO {
@@ -370,6 +399,7 @@ struct StageIO {
   T* yo;         // output
   T* lo;         // compensation of yn (MODE 4)
   T a;
+  T a2;          // MODE 7: dt/3
 };
 
 template <class T, int MODE>
@@ -388,9 +418,21 @@ __device__ __forceinline__ void stage_store(const StageIO<T>& io, int64_t iu, in
     io.acc[iv] = io.acc[iv] + T(2) * kv;
     io.yo[iu] = io.yn[iu] + io.a * ku;
     io.yo[iv] = io.yn[iv] + io.a * kv;
-  } else if (MODE == 3) {
-    io.yo[iu] = io.yn[iu] + io.a * (io.acc[iu] + ku);
-    io.yo[iv] = io.yn[iv] + io.a * (io.acc[iv] + kv);
+  } else if (MODE == 5) {
+    io.yo[iu] = io.yn[iu] + io.a * ku;
+    io.yo[iv] = io.yn[iv] + io.a * kv;
+  } else if (MODE == 7) {
+    constexpr T third = T(1) / T(3);
+    const T su = io.ys[iu];  // y_3's u (its v is ku)
+    const T zu = (io.acc[iu] + T(2) * su) * third + io.a2 * ku;
+    const T zv = (io.acc[iv] + T(2) * ku) * third + io.a2 * kv;
+    io.yo[iu] = io.yn[iu] + io.a * ku;
+    io.yo[iv] = io.yn[iv] + io.a * kv;
+    io.acc[iu] = zu;
+    io.acc[iv] = zv;
+  } else if (MODE == 8) {
+    io.yo[iu] = io.acc[iu] + io.a * ku;
+    io.yo[iv] = io.acc[iv] + io.a * kv;
   } else {
     // Kahan: (hi, c) += incr, c holds minus the lost low part
     const T incr_u = io.a * (io.acc[iu] + ku), incr_v = io.a * (io.acc[iv] + kv);
@@ -1047,6 +1089,8 @@ struct twoldg_problem {
   void* gg = nullptr;       // gradient ghosts (rows, or rows x D on a full metric)
   void* grads = nullptr;    // (nb, D, nn)
   void* flux = nullptr;     // full metric: (nb, D, nn) F_d = S[d] . V at every node
+  void* cartf = nullptr;    // full metric: (nb, 2D, D, Nt) V at the face nodes
+  int32_t* cf = nullptr;    // full metric: each exchange entry's position in cartf
   void* tables = nullptr;   // Tables<T>
   void* coef = nullptr;     // (nb, 1+D*D, nn) in T
   void* dcoef = nullptr;    // diagonal metric (nb, 1+D, nn): sqrt(g), A_d
@@ -1203,6 +1247,7 @@ inline unsigned grid_for(int64_t n, int threads) { return (unsigned)((n + thread
 // The exchange operator as CSR rows over (block, dim, side, tangential node).
 struct Csr {
   std::vector<int32_t> rowptr, cb, cl;
+  std::vector<int64_t> cf;  // each entry's source in the face-compact layout
   std::vector<double> vu, vg;
   std::vector<int64_t> dir_rows;
   std::vector<int64_t> rad_rows;
@@ -1230,13 +1275,24 @@ int build_csr(const twoldg_desc* d, Csr& csr) {
   csr.vu.reserve(nrows);
   csr.vg.reserve(nrows);
   bool bad = false;
-  auto push = [&](int64_t blk, const int I[3], double vu, double vg) {
+  // an entry: source node I of block blk, which lies on that block's face
+  // fface = 2 d + s (the face that touches the row's, or the row's own face)
+  auto push = [&](int64_t blk, const int I[3], double vu, double vg, int fface) {
     int64_t l = I[0] + (int64_t)N * (D >= 2 ? I[1] : 0) + (int64_t)N * N * (D >= 3 ? I[2] : 0);
     for (int e = 0; e < D; ++e)
       if (I[e] < 0 || I[e] >= N) bad = true;
     if (blk < 0 || blk >= nb || l < 0 || l >= nn) bad = true;
+    const int fd = fface / 2;
+    if (I[fd] != (fface % 2 ? N - 1 : 0)) bad = true;
+    int64_t tl = 0, st = 1;
+    for (int e = 0; e < D; ++e)
+      if (e != fd) {
+        tl += st * I[e];
+        st *= N;
+      }
     csr.cb.push_back((int32_t)blk);
     csr.cl.push_back((int32_t)l);
+    csr.cf.push_back(((blk * 2 * D + fface) * D) * Nt + tl);
     csr.vu.push_back(vu);
     csr.vg.push_back(vg);
   };
@@ -1256,12 +1312,12 @@ int build_csr(const twoldg_desc* d, Csr& csr) {
               if (F.nb[0] < 0 || F.nb[0] >= nb) return fail(TWOLDG_EINVAL, "face neighbour out of range");
               I[dd] = s ? 0 : N - 1;
               for (int k = 0; k < nt; ++k) I[tdim[k]] = T[k];
-              push(F.nb[0], I, 1.0, 1.0);
+              push(F.nb[0], I, 1.0, 1.0, 2 * dd + 1 - s);
             } break;
             case TWOLDG_FACE_MIRROR: {
               I[dd] = s ? N - 1 : 0;
               for (int k = 0; k < nt; ++k) I[tdim[k]] = T[k];
-              push(b, I, 1.0, -1.0);
+              push(b, I, 1.0, -1.0, 2 * dd + s);
             } break;
             case TWOLDG_FACE_GLUE: {
               if (F.nb[0] < 0 || F.nb[0] >= nb) return fail(TWOLDG_EINVAL, "glue neighbour out of range");
@@ -1276,7 +1332,7 @@ int build_csr(const twoldg_desc* d, Csr& csr) {
                 if (k < 0 || k >= nt) return fail(TWOLDG_EINVAL, "glue map entry out of range");
                 I[ntd[j]] = m > 0 ? T[k] : N - 1 - T[k];
               }
-              push(F.nb[0], I, 1.0, 1.0);
+              push(F.nb[0], I, 1.0, 1.0, F.nface);
             } break;
             case TWOLDG_FACE_COARSER: {
               if (F.nb[0] < 0 || F.nb[0] >= nb) return fail(TWOLDG_EINVAL, "coarse neighbour out of range");
@@ -1297,7 +1353,7 @@ int build_csr(const twoldg_desc* d, Csr& csr) {
                   I[tdim[k]] = qc[k] * P1 + ic;
                   w *= et.interp[fn[k]][ic];
                 }
-                push(F.nb[0], I, w, w);
+                push(F.nb[0], I, w, w, 2 * dd + 1 - s);
               }
             } break;
             case TWOLDG_FACE_FINER: {
@@ -1324,7 +1380,7 @@ int build_csr(const twoldg_desc* d, Csr& csr) {
                   I[tdim[k]] = qfine * P1 + jj;
                   w *= et.restrict_[ic[k]][fj];
                 }
-                push(fb, I, w, w);
+                push(fb, I, w, w, 2 * dd + 1 - s);
               }
             } break;
             case TWOLDG_FACE_DIRICHLET:
@@ -1335,7 +1391,7 @@ int build_csr(const twoldg_desc* d, Csr& csr) {
               // face node's and is then corrected by k_radiative_grad
               I[dd] = s ? N - 1 : 0;
               for (int k = 0; k < nt; ++k) I[tdim[k]] = T[k];
-              push(b, I, 1.0, 1.0);
+              push(b, I, 1.0, 1.0, 2 * dd + s);
               csr.rad_rows.push_back(r);
             } break;
             default:
@@ -1398,7 +1454,9 @@ inline void set_clock(twoldg_problem* p, double t0, double dt) {
 // ---- launch helpers, dispatched on (D, P1) at run time ----------------------
 template <class T, int D, int P1>
 struct Launch {
-  static ExchangeRows rows(twoldg_problem* p) { return {p->nrows, p->rowptr, p->cb, p->cl}; }
+  static ExchangeRows rows(twoldg_problem* p) {
+    return {p->nrows, p->rowptr, p->cb, p->cl, p->cf};
+  }
   static DirichletRows dir(twoldg_problem* p) {
     return {p->ndir,     p->dir_rows,  p->dir_x,     p->wave_k[0], p->wave_k[1],
             p->wave_k[2], p->wave_w,   p->wave_phi,  p->clock,     p->clock_dt};
@@ -1423,21 +1481,26 @@ struct Launch {
     ghosts<0>(p, (const T*)p->cval_u, y, cdt, (T*)p->ug);
   }
   // the gradient (flat, or Cartesian on a full metric) of y and its ghosts, the
-  // Dirichlet data at the clock's time plus cdt
-  static void gradient(twoldg_problem* p, const T* y, double cdt) {
+  // Dirichlet data at the clock's time plus cdt; for the energy, the gradient at
+  // every node (in grads) and no ghosts
+  static void gradient(twoldg_problem* p, const T* y, double cdt, bool for_energy = false) {
     exchange_u(p, y, cdt);
-    const int64_t n = p->sh.nb * p->sh.nn;
     if (!p->full) {
       hipLaunchKernelGGL((k_gradient<T, D, P1>), block_node_grid(p->sh), dim3(256), 0, p->stream,
                          p->sh, (const Tables<T>*)p->tables, p->spacing, y, (const T*)p->ug,
                          (const T*)p->dcoef, (T*)p->grads);
-      ghosts<1>(p, (const T*)p->cval_g, (const T*)p->grads, cdt, (T*)p->gg);
-    } else {
-      hipLaunchKernelGGL((k_gradient_cart<T, D, P1>), block_node_grid(p->sh), dim3(256), 0,
-                         p->stream, p->sh, (const Tables<T>*)p->tables, p->spacing, y,
+      if (!for_energy) ghosts<1>(p, (const T*)p->cval_g, (const T*)p->grads, cdt, (T*)p->gg);
+    } else if (for_energy) {
+      hipLaunchKernelGGL((k_gradient_cart<T, D, P1, false>), block_node_grid(p->sh), dim3(256),
+                         0, p->stream, p->sh, (const Tables<T>*)p->tables, p->spacing, y,
                          (const T*)p->ug, (const T*)p->coef, p->flat, (T*)p->grads,
+                         (T*)nullptr);
+    } else {
+      hipLaunchKernelGGL((k_gradient_cart<T, D, P1, true>), block_node_grid(p->sh), dim3(256), 0,
+                         p->stream, p->sh, (const Tables<T>*)p->tables, p->spacing, y,
+                         (const T*)p->ug, (const T*)p->coef, p->flat, (T*)p->cartf,
                          (T*)p->flux);
-      ghosts<2>(p, (const T*)p->cval_u, (const T*)p->grads, cdt, (T*)p->gg);
+      ghosts<2>(p, (const T*)p->cval_u, (const T*)p->cartf, cdt, (T*)p->gg);
       if (p->nrad)
         hipLaunchKernelGGL((k_radiative_grad<T, D>), dim3(grid_for(p->nrad, 256)), dim3(256), 0,
                            p->stream, p->sh, p->nrad, p->rad_rows, (const T*)p->coef, y,
@@ -1774,27 +1837,49 @@ int rk4_steps(twoldg_problem* p, double t0, double dt, int64_t nsteps) {
   T* lo = (T*)p->lo;
   // one step at the clock's time t0 + s dt, then the clock's tick
   auto one_step = [&]() {
-    StageIO<T> io{yn, yn, acc, ys, lo, (T)(dt / 2)};
-    L::gradient(p, yn, 0.0);
-    L::template stage<1>(p, io);
-    L::truncate(p, ys);
-    io.ys = ys;
-    L::gradient(p, ys, dt / 2);
-    L::template stage<2>(p, io);
-    L::truncate(p, ys);
-    io.a = (T)dt;
-    L::gradient(p, ys, dt / 2);
-    L::template stage<2>(p, io);
-    L::truncate(p, ys);
-    io.a = (T)(dt / 6);
-    io.yo = yn;
-    L::gradient(p, ys, dt);
-    if (p->precision == TWOLDG_F32C) {
+    if constexpr (std::is_same<T, float>::value) {  // FP32c: three registers
+      StageIO<T> io{yn, yn, acc, ys, lo, (T)(dt / 2), (T)0};
+      L::gradient(p, yn, 0.0);
+      L::template stage<1>(p, io);
+      L::truncate(p, ys);
+      io.ys = ys;
+      L::gradient(p, ys, dt / 2);
+      L::template stage<2>(p, io);
+      L::truncate(p, ys);
+      io.a = (T)dt;
+      L::gradient(p, ys, dt / 2);
+      L::template stage<2>(p, io);
+      L::truncate(p, ys);
+      io.a = (T)(dt / 6);
+      io.yo = yn;
+      L::gradient(p, ys, dt);
       L::template stage<4>(p, io);
       L::truncate(p, yn);
       L::truncate(p, lo);
     } else {
-      L::template stage<3>(p, io);
+      // FP64 without the accumulator: y_2 in ys, y_3 in acc, then y_4 over
+      // y_3 and z over y_2
+      StageIO<T> io{yn, yn, nullptr, ys, nullptr, (T)(dt / 2), (T)0};
+      L::gradient(p, yn, 0.0);
+      L::template stage<5>(p, io);
+      L::truncate(p, ys);
+      io.ys = ys;
+      io.yo = acc;
+      L::gradient(p, ys, dt / 2);
+      L::template stage<5>(p, io);
+      L::truncate(p, acc);
+      io.ys = acc;
+      io.acc = ys;
+      io.yo = acc;
+      io.a = (T)dt;
+      io.a2 = (T)(dt / 3);
+      L::gradient(p, acc, dt / 2);
+      L::template stage<7>(p, io);
+      L::truncate(p, acc);
+      io.a = (T)(dt / 6);
+      io.yo = yn;
+      L::gradient(p, acc, dt);
+      L::template stage<8>(p, io);
       L::truncate(p, yn);
     }
     hipLaunchKernelGGL(k_tick, dim3(1), dim3(1), 0, p->stream, p->clock);
@@ -1828,7 +1913,7 @@ int rhs_impl(twoldg_problem* p, int src, int dst, double t) {
   const T* y = (const T*)p->slot[src];
   set_clock(p, t, 0.0);
   L::gradient(p, y, 0.0);
-  StageIO<T> io{y, y, nullptr, (T*)p->slot[dst], nullptr, (T)0};
+  StageIO<T> io{y, y, nullptr, (T*)p->slot[dst], nullptr, (T)0, (T)0};
   L::template stage<0>(p, io);
   HIP_TRY(hipGetLastError());
   return 0;
@@ -1848,7 +1933,7 @@ int energy_impl(twoldg_problem* p, int slot, double t, double* out) {
   using L = Launch<T, D, P1>;
   const T* y = (const T*)p->slot[slot];
   set_clock(p, t, 0.0);
-  L::gradient(p, y, 0.0);
+  L::gradient(p, y, 0.0, true);
   L::energy(p, y);
   double h[3] = {0, 0, 0};
   const bool bnd = p->nrad && p->rad_invR != 0;
