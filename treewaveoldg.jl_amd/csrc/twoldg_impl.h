@@ -99,10 +99,27 @@ __device__ __forceinline__ void unravel(int l, int N, int I[kMaxD]) {
   if (D >= 3) I[2] = (int)(ul / (uN * uN));
 }
 
-// Block-node launches: the grid's x is the block, its y with the thread the node
-// within the block, so no thread divides a 64-bit global index.
+// Block-node launches: a 1D grid whose workgroup serves one chunk of 256 nodes
+// of one block.  Workgroups go to the eight XCDs in turn (the workgroup index
+// mod 8); the mapping gives every chunk of a block the same XCD and dispatches
+// them close together (groups of 8 blocks x their chunks), so that the lines
+// of a block, which each chunk reads across its planes in the wide stencils,
+// come into one XCD's L2 once.  No thread divides a 64-bit global index.
+constexpr int kXcds = 8;
 inline dim3 block_node_grid(const Shape& sh) {
-  return dim3((unsigned)sh.nb, (unsigned)((sh.nn + 255) / 256));
+  const int64_t nc = (sh.nn + 255) / 256;
+  const int64_t groups = (sh.nb + kXcds - 1) / kXcds;
+  return dim3((unsigned)(groups * kXcds * nc));
+}
+// This workgroup's block b and this thread's node l; false past the last block
+// or node.  256 threads per workgroup.
+__device__ __forceinline__ bool block_node(const Shape& sh, int64_t* b, int* l) {
+  const unsigned nc = (unsigned)((sh.nn + 255) / 256);
+  const unsigned gsz = kXcds * nc;
+  const unsigned w = blockIdx.x, g = w / gsz, r = w % gsz;
+  *b = (int64_t)g * kXcds + r % kXcds;
+  *l = (int)((r / kXcds) * blockDim.x + threadIdx.x);
+  return *b < sh.nb && *l < sh.nn;
 }
 
 template <int D>
@@ -268,9 +285,9 @@ __global__ void k_gradient(Shape sh, const Tables<T>* __restrict__ tbp,
                            const double* __restrict__ spacing, const T* __restrict__ y,
                            const T* __restrict__ ug, const T* __restrict__ dcoef,
                            T* __restrict__ grads) {
-  const int64_t b = blockIdx.x;
-  const int l = (int)(blockIdx.y * blockDim.x + threadIdx.x);
-  if (l >= sh.nn) return;
+  int64_t b;
+  int l;
+  if (!block_node(sh, &b, &l)) return;
   const Tables<T>& tb = *tbp;
   const int N = sh.N;
   int I[kMaxD] = {0, 0, 0};
@@ -304,9 +321,9 @@ __global__ void k_gradient_cart(Shape sh, const Tables<T>* __restrict__ tbp,
                                 const T* __restrict__ ug, const T* __restrict__ coef,
                                 const uint8_t* __restrict__ flat, T* __restrict__ cart,
                                 T* __restrict__ flux) {
-  const int64_t b = blockIdx.x;
-  const int l = (int)(blockIdx.y * blockDim.x + threadIdx.x);
-  if (l >= sh.nn) return;
+  int64_t b;
+  int l;
+  if (!block_node(sh, &b, &l)) return;
   const Tables<T>& tb = *tbp;
   const int N = sh.N;
   int I[kMaxD] = {0, 0, 0};
@@ -450,9 +467,9 @@ __global__ void k_stage_flat(Shape sh, const Tables<T>* __restrict__ tbp,
                              const double* __restrict__ spacing, const T* __restrict__ grads,
                              const T* __restrict__ gg, const T* __restrict__ dcoef,
                              StageIO<T> io) {
-  const int64_t b = blockIdx.x;
-  const int l = (int)(blockIdx.y * blockDim.x + threadIdx.x);
-  if (l >= sh.nn) return;
+  int64_t b;
+  int l;
+  if (!block_node(sh, &b, &l)) return;
   const Tables<T>& tb = *tbp;
   const int N = sh.N;
   int I[kMaxD] = {0, 0, 0};
@@ -486,9 +503,9 @@ __global__ void k_stage_cart(Shape sh, const Tables<T>* __restrict__ tbp,
                              const double* __restrict__ spacing, const T* __restrict__ flux,
                              const T* __restrict__ cg, const T* __restrict__ coef,
                              const uint8_t* __restrict__ flat, StageIO<T> io) {
-  const int64_t b = blockIdx.x;
-  const int l = (int)(blockIdx.y * blockDim.x + threadIdx.x);
-  if (l >= sh.nn) return;
+  int64_t b;
+  int l;
+  if (!block_node(sh, &b, &l)) return;
   const Tables<T>& tb = *tbp;
   const int N = sh.N;
   int I[kMaxD] = {0, 0, 0};
