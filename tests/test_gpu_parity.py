@@ -397,6 +397,33 @@ def test_sevenpatch_errors_published_and_oracle():
     assert np.linalg.norm(u - o["y"]) <= SQRT_EPS * np.linalg.norm(o["y"])
 
 
+# The seven-patch convergence table (CODE.md:4248-4251, bin/sevenpatch.jl: roots
+# (2s, 2s) at scale s, E = 2, k = (1.1, -0.7, 0.5), phi = 0.3, cfl 1/2, t = 1):
+# u's L2 error, v's, and 1 - E1/E0 at s = 1, 2, 3, printed to three digits.
+SEVENPATCH_TABLE = {
+    (2, True): ((6.07e-2, 1.63e-2, 7.34e-3), (7.81e-2, 2.25e-2, 1.04e-2), (0.016, 0.030, 0.033)),
+    (2, False): ((4.97e-2, 3.87e-3, 9.09e-4), (9.33e-2, 1.45e-2, 6.44e-3), (0.032, 0.035, 0.036)),
+    (3, True): ((8.83e-3, 1.15e-3, 3.36e-4), (1.14e-2, 1.38e-3, 3.84e-4), (0.035, 0.036, 0.036)),
+    (3, False): ((2.32e-3, 2.38e-4, 7.09e-5), (6.85e-3, 9.03e-4, 2.50e-4), (0.035, 0.036, 0.036)),
+}
+
+
+@pytest.mark.parametrize("P,truncate", list(SEVENPATCH_TABLE),
+                         ids=["P2-trunc", "P2-plain", "P3-trunc", "P3-plain"])
+def test_sevenpatch_convergence_table_published(P, truncate):
+    """Every row of the published seven-patch table at its three scales (up to
+    1512 blocks and 1200 steps at s = 3), to the printed digits: the full metric,
+    the glues, the Dirichlet sphere and both truncation settings converge as the
+    reference's do."""
+    us, vs, dEs = SEVENPATCH_TABLE[(P, truncate)]
+    for s in (1, 2, 3):
+        r = T.sevenpatch_errors(P, 2, roots=(2 * s, 2 * s), truncate=truncate)
+        assert r["l2u"] == pytest.approx(us[s - 1], rel=5e-3), (s, r["l2u"])
+        assert r["l2v"] == pytest.approx(vs[s - 1], rel=5e-3), (s, r["l2v"])
+        assert 1 - r["energy"] / r["energy0"] == pytest.approx(dEs[s - 1], abs=5e-4), s
+        r["problem"].close()
+
+
 def test_sevenpatch_f32c_matches_oracle():
     """FP32 stencil with FP64-built coefficients stored as FP32 (the seven-patch
     ball's FP32 form) against the oracle's FP64 run to t = 1: state and l2 error."""
