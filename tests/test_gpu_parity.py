@@ -371,6 +371,26 @@ def test_ball_rhs_truncate_energy_match_oracle(P, rough):
     assert l2 == pytest.approx(ref.norms(u)[0], rel=1e-13)
 
 
+@pytest.mark.parametrize("P", [2, 3])
+def test_refined_ball_rhs_matches_oracle(P, rough):
+    """The seven-patch ball with the middle of its cube refined (roots (4, 2)):
+    coarse-fine faces on a full metric, whose Cartesian gradient ghosts are
+    interpolated and restricted from the face-compact gradient, against the
+    oracle's own refined ball: right-hand side at t = 0.37 and energy."""
+    from twoldg.geometry import sevenpatch_forest
+    forest, geo = sevenpatch_forest(P, 2, roots=(4, 2), refined=True)
+    wave = T.PlaneWave((1.1, -0.7, 0.5), 0.3)
+    prob = T.WaveProblem(forest, T.Element(P), geometry=geo, boundary=wave)
+    ref = O.Problem.ball(P, 2, roots=(4, 2), regions=[((1.0, 3.0),) * 3])
+    u = rough(prob.state_len, seed=17)
+    du = np.empty_like(u)
+    T.wave_rhs(du, u, prob, 0.37)
+    want = ref.rhs(u, 0.37)
+    assert np.max(np.abs(du - want)) <= 1e-12 * np.max(np.abs(want))
+    assert T.wave_energy(prob, u, 0.37) == pytest.approx(ref.energy(u, 0.37), rel=1e-13)
+    prob.close()
+
+
 def test_ball_truncation_is_projection(rough):
     """sqrt(g)-orthogonal projection: idempotent (sevenpatch_tests.jl:128-158)."""
     prob, _ = _ball()
