@@ -169,7 +169,7 @@ void destroy(twoldg_problem* p) {
                   p->tweights, p->tginv, (void*)p->spacing, (void*)p->rowptr, (void*)p->cb,
                   (void*)p->cl, p->cval_u, p->cval_g, (void*)p->dir_rows, (void*)p->dir_x,
                   (void*)p->part, (void*)p->red, (void*)p->clock, p->dcoef, p->lmode, p->ldual,
-                  (void*)p->fslot, (void*)p->fradial, (void*)p->fblk, (void*)p->oblk, p->frmodes, p->frduals, (void*)p->faslot, p->famodes, p->faweights, p->faginv,
+                  (void*)p->fslot, (void*)p->fradial, (void*)p->fblk, (void*)p->oblk, p->frmodes, p->frduals, (void*)p->faslot, p->faproj,
                   (void*)p->rad_rows, (void*)p->rpart, (void*)p->svn, (void*)p->svlo,
                   (void*)p->svs, (void*)p->svacc, p->flux, (void*)p->nbr6, p->uf[0], p->uf[1],
                   (void*)p->cfjobs, p->fgu, p->cartf, (void*)p->cf})
@@ -378,11 +378,30 @@ int create_buffers(twoldg_problem* p, const twoldg_desc* d, const Csr& csr) {
     if ((rc = h2d(p, p->faslot, d->faslot, nf * 4))) return rc;
     if ((rc = upload_as<T>(p, &p->frmodes, d->frmodes, nf * E * P1))) return rc;
     if ((rc = upload_as<T>(p, &p->frduals, d->frduals, nf * E * P1))) return rc;
-    if ((rc = upload_as<T>(p, &p->famodes, d->famodes, n2 * m2))) return rc;
-    if ((rc = upload_as<T>(p, &p->faweights, d->faweights, nA * E * E * n2))) return rc;
     std::vector<double> ginv;
     if ((rc = gram_inverses(d->fachol, nA * E * E, m2, ginv))) return rc;
-    if ((rc = upload_as<T>(p, &p->faginv, ginv.data(), nA * E * E * m2 * m2))) return rc;
+    // each (angular table, element position)'s plane projector M = Psi G^-1
+    // Psi^T W (Psi: the plane's Legendre basis, W: its weights), in FP64
+    const int64_t ns = nA * E * E;
+    std::vector<double> proj((size_t)(ns * n2 * n2)), B((size_t)(m2 * n2));
+    for (int64_t a = 0; a < ns; ++a) {
+      const double* G = ginv.data() + a * m2 * m2;  // symmetric
+      const double* w = d->faweights + a * n2;
+      for (int64_t q = 0; q < m2; ++q)  // B = G^-1 Psi^T W
+        for (int64_t t = 0; t < n2; ++t) {
+          double acc = 0;
+          for (int64_t r = 0; r < m2; ++r) acc += G[q + m2 * r] * d->famodes[t * m2 + r];
+          B[q * n2 + t] = acc * w[t];
+        }
+      double* M = proj.data() + a * n2 * n2;
+      for (int64_t t = 0; t < n2; ++t)
+        for (int64_t u = 0; u < n2; ++u) {
+          double acc = 0;
+          for (int64_t q = 0; q < m2; ++q) acc += d->famodes[t * m2 + q] * B[q * n2 + u];
+          M[t * n2 + u] = acc;
+        }
+    }
+    if ((rc = upload_as<T>(p, &p->faproj, proj.data(), ns * n2 * n2))) return rc;
 
     // the truncation's two block lists: the factored blocks with their tables,
     // and the others

@@ -636,9 +636,7 @@ struct FactoredTabs {
   const T* rmodes;
   const T* rduals;
   const int32_t* aslot;
-  const T* amodes;
-  const T* aweights;
-  const T* aginv;
+  const T* aproj;  // per (angular table, element position): Psi G^-1 Psi^T W on a plane
 };
 
 // EPW elements of one block per workgroup, each on its own (P+1)^3 threads
@@ -661,8 +659,8 @@ k_truncate_elem(Shape sh, int E, const Tables<T>* __restrict__ tbp,
   constexpr int P = P1 - 1;
   constexpr int n = P1 * P1 * P1, m = P * P * P;
   constexpr int NTE = trunc_elem_threads<P1>(), EPW = trunc_elem_epw<P1>();
-  constexpr int GS = KIND == 1 ? (P * P) * (P * P) : m * m;
-  constexpr int PS = KIND == 1 ? (P1 * P1) * (P * P) : n * m;
+  constexpr int GS = KIND == 1 ? (P1 * P1) * (P1 * P1) : m * m;
+  constexpr int PS = KIND == 1 ? 1 : n * m;
   __shared__ T sWA_[EPW][n];
   __shared__ T sc_[EPW][P1 * P * P];  // m <= (P+1) P^2: the element's, or the planes' coefficients
   __shared__ T sz_[EPW][P1 * P * P];
@@ -694,8 +692,9 @@ k_truncate_elem(Shape sh, int E, const Tables<T>* __restrict__ tbp,
   };
   if constexpr (KIND == 1) {
     // evolution.jl:306-380: along k, a <- a - mode (dual . a) on every line;
-    // then on each plane across k, a <- Psi G^-1 Psi^T W a
-    constexpr int n2 = P1 * P1, m2 = P * P;
+    // then on each plane across k, a <- M a with M = Psi G^-1 Psi^T W, the
+    // plane's projector formed on the host for the element's angular table
+    constexpr int n2 = P1 * P1;
     const int fs = fb.f, kr = fb.k, A = fb.A;
     const int t0 = kr == 0 ? 1 : 0, t1 = kr == 2 ? 1 : 2;
     const int J[3] = {J0, J1, J2};
@@ -703,12 +702,8 @@ k_truncate_elem(Shape sh, int E, const Tables<T>* __restrict__ tbp,
     const int st[3] = {1, P1, P1 * P1};
     const T* rm = F.rmodes + ((int64_t)fs * E + J[kr]) * P1;
     const T* rd = F.rduals + ((int64_t)fs * E + J[kr]) * P1;
-    const T* w = F.aweights + ((int64_t)A * E * E + ea) * n2;
-    const T* G = F.aginv + ((int64_t)A * E * E + ea) * m2 * m2;
-    for (int i = k; i < m2 * m2; i += NTE) sG[i] = G[i];
-    if (psi_lds)
-      for (int i = threadIdx.x; i < n2 * m2; i += blockDim.x) sPsi[i] = F.amodes[i];
-    const T* Psi = psi_lds ? sPsi : F.amodes;
+    const T* M = F.aproj + ((int64_t)A * E * E + ea) * n2 * n2;
+    for (int i = k; i < n2 * n2; i += NTE) sG[i] = M[i];
     int kk[3] = {0, 0, 0};
     if (k < n) {
       kk[0] = k % P1;
@@ -726,29 +721,15 @@ k_truncate_elem(Shape sh, int E, const Tables<T>* __restrict__ tbp,
       v = sWA[k] - rm[kk[kr]] * c;
     }
     lds_barrier();
-    const int pn = kk[t0] + P1 * kk[t1];  // the node's index in its plane
-    if (k < n) sWA[k] = w[pn] * v;
-    lds_barrier();
-    if (k < P1 * m2) {  // (plane i, mode q): Psi^T W a
-      const int i = k / m2, q = k % m2;
-      T cq = 0;
-      for (int r = 0; r < n2; ++r) {
-        const int node = i * st[kr] + (r % P1) * st[t0] + (r / P1) * st[t1];
-        cq += Psi[r * m2 + q] * sWA[node];
-      }
-      sc[k] = cq;
-    }
-    lds_barrier();
-    if (k < P1 * m2) {
-      const int i = k / m2, q = k % m2;
-      T zq = 0;
-      for (int r = 0; r < m2; ++r) zq += sG[q + m2 * r] * sc[i * m2 + r];
-      sz[k] = zq;
-    }
+    if (k < n) sWA[k] = v;
     lds_barrier();
     if (k < n) {
+      const int pn = kk[t0] + P1 * kk[t1];  // the node's index in its plane
+      const int plane = kk[kr] * st[kr];
       T z = 0;
-      for (int q = 0; q < m2; ++q) z += Psi[pn * m2 + q] * sz[kk[kr] * m2 + q];
+#pragma unroll
+      for (int r = 0; r < n2; ++r)
+        z += sG[pn * n2 + r] * sWA[plane + (r % P1) * st[t0] + (r / P1) * st[t1]];
       a[at(k)] = z;
     }
     return;
@@ -1128,9 +1109,7 @@ struct twoldg_problem {
   void* frmodes = nullptr;
   void* frduals = nullptr;
   int32_t* faslot = nullptr;
-  void* famodes = nullptr;
-  void* faweights = nullptr;
-  void* faginv = nullptr;
+  void* faproj = nullptr;  // (nangular, E^2, (P+1)^2, (P+1)^2): each plane's projector
   // radiative faces: their exchange rows (u-shaped) and the condition's constants
   int64_t nrad = 0;
   int64_t* rad_rows = nullptr;
@@ -1527,8 +1506,7 @@ struct Launch {
   static FactoredTabs<T> factored(twoldg_problem* p) {
     return {p->fblk,            p->nfblk ? p->oblk : nullptr,
             p->fslot,           p->fradial,          (const T*)p->frmodes,
-            (const T*)p->frduals, p->faslot,         (const T*)p->famodes,
-            (const T*)p->faweights, (const T*)p->faginv};
+            (const T*)p->frduals, p->faslot,         (const T*)p->faproj};
   }
   template <int MODE>
   static void stage(twoldg_problem* p, const StageIO<T>& io) {
