@@ -320,12 +320,7 @@ k_fused3(FusedArgs<T, TV> A) {
   }
   lds_barrier();
 
-  // four layers (E = 4) run unrolled: a loop's back edge joins the paths with
-  // and without the last phase's stores, and the wait for the next layer's
-  // loads at its top then covers those stores too (FP64 +2.6%, split +7% on
-  // the 320^3 forest); two layers (E = 2) stay a loop, which unrolled spills
-  constexpr int kLayerUnroll = E == 4 ? 4 : 1;
-#pragma unroll kLayerUnroll
+#pragma unroll 1
   for (int e = 0; e < E; ++e) {
     // ---- the layer's x/y ghosts, issued first ----
     load_ghosts(e, gv);

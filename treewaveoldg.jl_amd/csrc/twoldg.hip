@@ -1,23 +1,10 @@
 // twoldg.hip -- MI355X (gfx950) OLDG field-evolution library: host side and C ABI.
 // The kernels live in twoldg_impl.h, instantiated per arithmetic type and
 // dimension in inst.hip.
-#include <cstdlib>
 #include <map>
 #include <mutex>
 #include <tuple>
 #include "twoldg_impl.h"
-
-// Kernel arguments in host memory.  With HIP's device-memory kernel arguments
-// some waves of a new problem's first stage launches have read arguments of
-// earlier launches (another problem's buffers and time step: wrong values, no
-// fault; measured on MI355X: 3 of 7 runs of the fused tests without the
-// problem's warm-up step, 0 of 7 with the arguments in host memory).  The
-// runtime reads this setting when it starts, so it takes effect when this
-// library is loaded before the process's first HIP call; a value the caller
-// set is kept.
-__attribute__((constructor)) static void twoldg_host_kernargs() {
-  setenv("HIP_FORCE_DEV_KERNARG", "0", 0);
-}
 
 namespace twoldg_detail {
 thread_local std::string g_last_error;
