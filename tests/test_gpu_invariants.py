@@ -56,6 +56,7 @@ def _annulus(P=3):
 
 HALF2 = [((0.25, 0.75),) * 2]
 HALF3 = [((0.25, 0.75),) * 3]
+# three levels of nested cubes, finest half-width 1/8
 NESTED = [((0.25, 0.75),) * 3, ((0.375, 0.625),) * 3]
 
 PATHS = {
@@ -69,6 +70,10 @@ PATHS = {
     "3d-nested-f64": (lambda: _flat(3, 3, 2, 4, reflecting=True, regions=NESTED), "fused3d"),
     "3d-nested-f32c": (lambda: _flat(3, 3, 2, 4, precision="f32c", reflecting=True,
                                      regions=NESTED), "fused3d"),
+    "3d-nested-split": (lambda: _flat(3, 3, 2, 4, precision="split", reflecting=True,
+                                      regions=NESTED), "fused3d"),
+    "3d-nested-r8-split": (lambda: _flat(3, 3, 2, 8, precision="split", reflecting=True,
+                                         regions=NESTED), "fused3d"),
     "2d-batch-f64": (lambda: _flat(2, 4, 2, 4), "batch2d"),
     "2d-batch-f32c": (lambda: _flat(2, 4, 2, 4, precision="f32c"), "batch2d"),
     "ball7": (_ball7, "generic"),
