@@ -101,7 +101,7 @@ struct BatchArgs {
   T* uf_out;               // the next stage's ghosts (the neighbours' slots)
   T* pf_out;
   double coef;             // stage coefficient in units of dt: 1/2, 1 or 1/6
-  double coef2;            // mode 7: the final combination's, 1/6
+  double coef2;            // mode 13: z's coefficient of k3, 1/3
   int truncate;
 };
 

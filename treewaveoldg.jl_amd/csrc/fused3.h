@@ -13,9 +13,9 @@
 //     node's gradient (the wide derivative's row 0 or P without the ghost
 //     term).  A neighbour's face gradient is that part plus the ghost term
 //     formed with our own face value.
-// So one stage reads u, v of the stage state, y_n and the accumulator once and
-// writes the accumulator and the next state once (30 values per node and step
-// in FP64), plus faces, O(1/N) of that.
+// So one stage reads each of its operands (the stage state, y_n, the accumulator
+// or the other stage state) once and writes each result once (26 values per
+// node and step in FP64, 36 in FP32c), plus faces, O(1/N) of that.
 
 #pragma once
 
